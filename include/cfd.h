@@ -1,0 +1,212 @@
+/*
+ * cfd.h — C-ABI of the MI355X pseudo-time iteration library (libcfd_amd.so).
+ *
+ * The library runs the artificial-compressibility pseudo-time loop of the
+ * staggered-grid finite-volume solver on one AMD Instinct MI355X (gfx950):
+ * SSP-RK3 stages of u, v (QUICK or first-order upwind convection, central
+ * diffusion, Darcy-Forchheimer drag, buoyancy), the pressure update, and the
+ * energy (T) and species (C) transport, with their boundary conditions.
+ *
+ * Reference interface this replaces:
+ *   c_cuda/functions.h:7-21,36 (RESU..bcC, max_reduce) -> include/cfd_legacy.h
+ *   c_cuda/main.cu:203-277 (pseudo-time and physical-time loops) -> cfd_pseudo_steps,
+ *     cfd_run_to_tol, cfd_physical_step below
+ *   OpenMp_Multicore/main.f90:263-333 (same loops, Fortran)
+ *
+ * Array layout (identical to the reference C layout, c_cuda/main.cu:59-89):
+ *   1-based, index 0 is padding, j (y direction) is the contiguous axis.
+ *     p-like (p, pn, T, C, eps, L):  element (i,j) at i*(jmax+1)+j, i in [1,imax],   j in [1,jmax]
+ *                                    allocation (imax+1)*(jmax+1) doubles
+ *     u (um, um_tau, um_n_tau):      element (i,j) at i*(jmax+1)+j, i in [1,imax+1], j in [1,jmax]
+ *                                    allocation (imax+2)*(jmax+1) doubles
+ *     v (vm, vm_tau, vm_n_tau):      element (i,j) at i*(jmax+2)+j, i in [1,imax],   j in [1,jmax+1]
+ *                                    allocation (imax+1)*(jmax+2) doubles
+ *   Fortran callers (column-major, i fastest, no padding) use CFD_LAYOUT_FORTRAN in the
+ *   upload/download calls; the library converts at the boundary.
+ *
+ * Errors: every cfd_* function returns a status code (CFD_OK == 0). A text for the last
+ * failure on the calling thread is available from cfd_last_error(). A non-finite residual
+ * is a failure (CFD_ERR_NONFINITE), never a converged state.
+ *
+ * Threading: one context per host thread; a context owns one HIP stream. Contexts on the
+ * same device are independent.
+ */
+#ifndef CFD_AMD_CFD_H
+#define CFD_AMD_CFD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define CFD_OK               0
+#define CFD_ERR_ARG         -1   /* invalid argument / size / layout */
+#define CFD_ERR_HIP         -2   /* HIP runtime error (allocation, launch) */
+#define CFD_ERR_NONFINITE   -3   /* a residual became NaN or Inf */
+#define CFD_ERR_NOCONV      -4   /* iteration limit reached before the tolerance */
+#define CFD_ERR_DIVERGED    -5   /* residual grew past the divergence guard */
+#define CFD_ERR_STATE       -6   /* call out of order (no geometry, no context) */
+#define CFD_ERR_UNSUPPORTED -7   /* option not available in this build */
+
+/* ---- model switches --------------------------------------------------------------- */
+/* Convection scheme of the momentum residuals. */
+#define CFD_SCHEME_QUICK     0   /* QUICK interior + first-order upwind strips (equations.f90:11-113) */
+#define CFD_SCHEME_UPWIND    1   /* first-order upwind everywhere (equations.f90:120-240 on every row) */
+
+/* Boundary-condition set. */
+#define CFD_BC_REFERENCE     0   /* bottom inflow v=v_i, top extrapolation, slip side walls (boundary.f90) */
+#define CFD_BC_CHANNEL       1   /* left inflow u=u_in, right zero-gradient outflow (p=0), no-slip top/bottom,
+                                    T Dirichlet t_bottom/t_top, species slot on the left wall */
+
+/* Buoyancy term of the v-momentum residual. */
+#define CFD_BUOY_REFERENCE   0   /* invfr2 * (1 - 1/Tbar)  (equations.f90:405) */
+#define CFD_BUOY_BOUSSINESQ  1   /* ri * (Tbar - t_ref) */
+
+/* Semantics where the two reference implementations differ. */
+#define CFD_COMPAT_FORTRAN   0   /* OpenMp_Multicore semantics (default) */
+#define CFD_COMPAT_CUDA      1   /* c_cuda: V strips at j=2,jmax-1; solve_P stage 2 skips j=2;
+                                    residuals over the first n linear entries */
+
+/* Viscosity model in the momentum diffusion coefficients. */
+#define CFD_MU_CONST         0   /* eps/Re */
+#define CFD_MU_POLY4         1   /* eps*mu(T)/Re, mu(T) = sum_k mu_coef[k] T^k, T of the physical level */
+
+/* Storage precision of the fused pseudo-step. */
+#define CFD_PREC_FP64        0   /* FP64 fields and arithmetic (the drop-in) */
+#define CFD_PREC_FP32        1   /* FP32 u,v,T,C storage + FP32 stencil; FP64 pressure, FP64 residual norms */
+
+/* Field identifiers for upload / download / device pointers. */
+#define CFD_F_UM        0   /* u, physical level            (u layout) */
+#define CFD_F_VM        1   /* v, physical level            (v layout) */
+#define CFD_F_UM_TAU    2   /* u, pseudo-time iterate       (u layout) */
+#define CFD_F_VM_TAU    3   /* v, pseudo-time iterate       (v layout) */
+#define CFD_F_P         4   /* p, current pressure          (p layout) */
+#define CFD_F_T         5   /* T, physical level            (p layout) */
+#define CFD_F_T_TAU     6   /* T, iterate                   (p layout) */
+#define CFD_F_C         7   /* C, physical level            (p layout) */
+#define CFD_F_C_TAU     8   /* C, iterate                   (p layout) */
+#define CFD_F_EPS       9   /* porosity epsilon1            (p layout) */
+#define CFD_F_LP       10   /* porous-zone switch liga_poros (p layout) */
+#define CFD_NFIELDS    11
+
+#define CFD_LAYOUT_C        0   /* the 1-based padded C layout above */
+#define CFD_LAYOUT_FORTRAN  1   /* column-major (1:n, 1:m), i fastest, no padding */
+
+typedef struct cfd_params {
+    int32_t imax, jmax;         /* grid nodes in x and y (>= 6 each) */
+    int32_t scheme;             /* CFD_SCHEME_* */
+    int32_t bc;                 /* CFD_BC_* */
+    int32_t buoyancy;           /* CFD_BUOY_* */
+    int32_t compat;             /* CFD_COMPAT_* */
+    int32_t steady;             /* 1: the physical level follows the iterate (phi^n == phi^tau) */
+    int32_t mu_model;           /* CFD_MU_* */
+    double re;                  /* Reynolds number */
+    double pe;                  /* Peclet number (energy diffusivity 1/Pe) */
+    double sc;                  /* Schmidt number (species diffusivity 1/(Re Sc)) */
+    double darcy;               /* Darcy number */
+    double cf;                  /* Forchheimer coefficient */
+    double g;                   /* body force -g*eps in x-momentum */
+    double invfr2;              /* 1/Fr^2 for CFD_BUOY_REFERENCE */
+    double ri;                  /* Richardson number for CFD_BUOY_BOUSSINESQ */
+    double t_ref;               /* reference temperature for CFD_BUOY_BOUSSINESQ */
+    double v_i;                 /* reference BC set: bottom inflow velocity */
+    double tinf;                /* reference BC set: bottom temperature */
+    double dtau, dt;            /* pseudo and physical time steps */
+    double beta;                /* artificial compressibility */
+    double b_art;               /* bulk artificial viscosity */
+    double u_in;                /* channel BC set: inflow velocity */
+    double t_bottom, t_top;     /* channel BC set: wall temperatures */
+    double slot_lo, slot_hi;    /* channel BC set: y-range of the species slot on the left wall */
+    double slot_value;          /* channel BC set: species value inside the slot (0 outside) */
+    double mu_coef[5];          /* CFD_MU_POLY4 coefficients c0..c4 */
+} cfd_params;
+
+/* Reference defaults (OpenMp_Multicore/comum.f90:29-115, main.f90:252-253, input/iterations.dat):
+ * Re=Pe=Sc=1, Da=1e-2, Cf=1.75/sqrt(150*0.5^3), g=9.80665, Fr=1, v_i=0.5, Tinf=1,
+ * dtau=0.05, dt=0.005, beta=1300, b_art=0, QUICK, reference BCs. */
+void cfd_params_default(cfd_params *p, int32_t imax, int32_t jmax);
+
+/* ---- geometry ---------------------------------------------------------------------- */
+/* Node coordinates x[1..imax], y[1..jmax] (arrays of imax+1 / jmax+1 doubles, index 0 unused)
+ * give the staggered midpoints and face areas by the reference formulas
+ * (nonsymetric_mesh.f90:48-101). Output arrays follow the C-ABI sizes of comum.cu:133-147. */
+typedef struct cfd_geometry {
+    double *x, *y, *xm, *ym;
+    double *areau_n, *areau_s, *areau_e, *areau_w;
+    double *areav_n, *areav_s, *areav_e, *areav_w;
+} cfd_geometry;
+
+/* Host-side mesh generator: nodes on [a,b] with n points; P=1 is uniform, P<1 clusters
+ * toward both ends with the tanh stretch s(eta) = P*eta + (1-P)*(1 - tanh(Q(1-eta))/tanh(Q))
+ * applied on each half (nonsymetric_mesh.f90:17-29). out[1..n] written, out[0] untouched. */
+int cfd_mesh_1d(int32_t n, double a, double b, double P, double Q, double *out);
+/* Midpoints and areas from x,y (host arrays, C-ABI sizes). */
+int cfd_geometry_derive(int32_t imax, int32_t jmax, const double *x, const double *y,
+                        double *xm, double *ym,
+                        double *areau_n, double *areau_s, double *areau_e, double *areau_w,
+                        double *areav_n, double *areav_s, double *areav_e, double *areav_w);
+
+/* ---- context API -------------------------------------------------------------------- */
+typedef struct cfd_ctx cfd_ctx;
+
+int  cfd_ctx_create(const cfd_params *params, int32_t precision, cfd_ctx **out);
+int  cfd_ctx_destroy(cfd_ctx *ctx);
+int  cfd_ctx_get_params(const cfd_ctx *ctx, cfd_params *out);
+/* Replace the scalar parameters (sizes, precision and switches must not change). */
+int  cfd_ctx_set_params(cfd_ctx *ctx, const cfd_params *params);
+/* Host node coordinates (x[1..imax], y[1..jmax]); derives and uploads the staggered geometry. */
+int  cfd_ctx_set_mesh(cfd_ctx *ctx, const double *x, const double *y);
+/* Copy a host field into the context (and convert from `layout`). */
+int  cfd_ctx_upload(cfd_ctx *ctx, int32_t field, const double *host, int32_t layout);
+/* Copy a context field to the host. Fields CFD_F_* plus CFD_F_PN (last pressure update). */
+int  cfd_ctx_download(cfd_ctx *ctx, int32_t field, double *host, int32_t layout);
+/* Device pointer of a field (FP64 contexts only; the pointer changes after each step,
+ * because the iterate and the new level swap by pointer). */
+int  cfd_ctx_device_ptr(cfd_ctx *ctx, int32_t field, void **out);
+/* Reset pn := p, T_tau := T, C_tau := C, um_tau := um, vm_tau := vm on the device (the
+ * reference leaves pn, T_tau, C_tau unset before the first pseudo-step, main.f90:252-256). */
+int  cfd_ctx_sync_levels(cfd_ctx *ctx);
+
+/* Run `nsteps` pseudo-steps (U, V, P, T, C, then the tau-level swap), recording the
+ * residuals max|r3| of u, v, p after the last step into res[0..2] (may be NULL). With
+ * history != NULL, history[3*k+0..2] receives the residuals of step k. No host
+ * synchronisation happens between steps unless history is requested. */
+int  cfd_pseudo_steps(cfd_ctx *ctx, int32_t nsteps, double res[3], double *history);
+
+/* Iterate until max(res_u,res_v,res_p) < tol (test skipped on the first step, as
+ * main.f90:289), checking every `check_every` steps, at most itc_max steps. A residual that
+ * exceeds `diverge_factor` times the smallest residual seen returns CFD_ERR_DIVERGED
+ * (diverge_factor <= 0 disables the guard). */
+int  cfd_run_to_tol(cfd_ctx *ctx, double tol, int64_t itc_max, int32_t check_every,
+                    double diverge_factor, int64_t *iters, double res[3]);
+
+/* One physical time step: pseudo-iterate to `eps` (at most itc_max steps, first-step
+ * quirk of main.f90:224,267 reproduced when first != 0), then um=um_n_tau, vm=vm_n_tau,
+ * T=T_n_tau, C=C_n_tau (main.f90:301-304). */
+int  cfd_physical_step(cfd_ctx *ctx, double eps, int64_t itc_max, int32_t first,
+                       int64_t *iters, double res[3]);
+
+/* Capture `steps_per_graph` fused pseudo-steps into a hipGraph and replay it `replays`
+ * times (steps_per_graph must be even so the ping-pong buffers return to their slots). */
+int  cfd_pseudo_steps_graph(cfd_ctx *ctx, int32_t steps_per_graph, int32_t replays, double res[3]);
+
+/* Mean kernel time per pseudo-step of the last cfd_pseudo_steps / _graph call, measured
+ * with HIP events on the context's stream (milliseconds). */
+int  cfd_ctx_last_timing(const cfd_ctx *ctx, double *ms_per_step);
+
+/* Which kernel implementation the fused step uses (0 = per-level kernels, 1 = row-streaming
+ * single-pass kernel). Mainly for tests that pin both against each other. */
+int  cfd_ctx_set_kernel(cfd_ctx *ctx, int32_t kernel);
+
+const char *cfd_last_error(void);
+const char *cfd_version(void);
+
+#define CFD_F_PN 11   /* download only: the pressure written by the last step */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFD_AMD_CFD_H */

@@ -1,0 +1,206 @@
+"""Seeded inputs, array layouts and case builders for the pseudo-time iteration.
+
+Layouts (include/cfd.h; the reference C layout of c_cuda/main.cu:59-89):
+  p-like: 2-D array (imax+1, jmax+1), element (i, j) at [i, j], index 0 is padding
+  u     : 2-D array (imax+2, jmax+1)
+  v     : 2-D array (imax+1, jmax+2)
+Fortran arrays (1:n, 1:m) are column-major; ``to_fortran``/``from_fortran`` convert.
+
+The random generator is SplitMix64 (Steele, Lea, Flood 2014) mapped to doubles in [0, 1)
+by the top 53 bits, so every input of the tests and benchmarks is reproducible from a seed
+on any host.
+"""
+import math
+import numpy as np
+
+from .params import make_params, BC_CHANNEL, BUOY_BOUSSINESQ, SCHEME_UPWIND, MU_POLY4
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, n):
+    """n consecutive SplitMix64 outputs for the given seed (uint64 array)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n, a=0.0, b=1.0):
+    """n doubles uniform on [a, b) from SplitMix64(seed)."""
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return a + (b - a) * u
+
+
+# ---- layouts --------------------------------------------------------------------------
+def shape_of(kind, imax, jmax):
+    return {"p": (imax + 1, jmax + 1), "u": (imax + 2, jmax + 1), "v": (imax + 1, jmax + 2)}[kind]
+
+
+def fortran_shape(kind, imax, jmax):
+    return {"p": (imax, jmax), "u": (imax + 1, jmax), "v": (imax, jmax + 1)}[kind]
+
+
+def zeros(kind, imax, jmax, dtype=np.float64):
+    return np.zeros(shape_of(kind, imax, jmax), dtype=dtype)
+
+
+def to_fortran(a, kind, imax, jmax):
+    n, m = fortran_shape(kind, imax, jmax)
+    return np.asfortranarray(a[1:n + 1, 1:m + 1])
+
+
+def from_fortran(f, kind, imax, jmax):
+    n, m = fortran_shape(kind, imax, jmax)
+    a = zeros(kind, imax, jmax)
+    a[1:n + 1, 1:m + 1] = np.asarray(f).reshape((n, m), order="F")
+    return a
+
+
+FIELD_KIND = {"um": "u", "vm": "v", "um_tau": "u", "vm_tau": "v", "um_n_tau": "u",
+              "vm_n_tau": "v", "p": "p", "pn": "p", "T": "p", "T_tau": "p", "T_n_tau": "p",
+              "C": "p", "C_tau": "p", "C_n_tau": "p", "eps": "p", "lp": "p"}
+
+
+# ---- geometry ---------------------------------------------------------------------------
+def uniform_nodes(n, a=0.0, b=1.0):
+    """x[1..n] = a + (i-1)(b-a)/(n-1); x[0] = 0 (padding)."""
+    x = np.zeros(n + 1)
+    x[1:] = a + (np.arange(n, dtype=np.float64)) * ((b - a) / (n - 1))
+    return x
+
+
+def tanh_nodes(n, a, b, P, Q):
+    """Nodes on [a, b] clustered toward both ends by the reference stretch
+    s = P*eta + (1-P)*(1 - tanh(Q(1-eta))/tanh(Q)) on each half
+    (OpenMp_Multicore/nonsymetric_mesh.f90:17-29, restated for a general interval)."""
+    x = uniform_nodes(n, a, b)
+    c = n // 2 + 1
+    xn, x0 = x[n], x[1]
+    # x[c] is re-read each pass: the reference loops start at the split node and update it
+    for i in range(c, n + 1):
+        eta = (x[i] - xn) / (x[c] - xn)
+        s = P * eta + (1.0 - P) * (1.0 - math.tanh(Q * (1.0 - eta)) / math.tanh(Q))
+        x[i] = xn - s * (xn - x[c])
+    for i in range(c, 0, -1):
+        eta = (x[i] - x0) / (x[c] - x0)
+        s = P * eta + (1.0 - P) * (1.0 - math.tanh(Q * (1.0 - eta)) / math.tanh(Q))
+        x[i] = x0 + s * (x[c] - x0)
+    return x
+
+
+def derive_geometry(x, y):
+    """Staggered midpoints and face areas (nonsymetric_mesh.f90:48-101), C-ABI sizes."""
+    imax, jmax = len(x) - 1, len(y) - 1
+    xm = np.zeros(imax + 2); ym = np.zeros(jmax + 2)
+    xm[1] = -(x[2] + x[1]) * 0.5
+    xm[imax + 1] = (x[imax] + x[imax - 1]) * 0.5 + (x[imax] - x[imax - 1])
+    xm[2:imax + 1] = (x[2:imax + 1] + x[1:imax]) * 0.5
+    ym[1] = y[1] - (y[2] - y[1]) * 0.5
+    ym[jmax + 1] = (y[jmax] + y[jmax - 1]) * 0.5 + (y[jmax] - y[jmax - 1])
+    ym[2:jmax + 1] = (y[2:jmax + 1] + y[1:jmax]) * 0.5
+    aun = np.zeros(imax + 2); aue = np.zeros(jmax + 1)
+    avn = np.zeros(imax + 1); ave = np.zeros(jmax + 2)
+    aun[2:imax + 1] = x[2:imax + 1] - x[1:imax]
+    aue[1:jmax + 1] = ym[2:jmax + 2] - ym[1:jmax + 1]
+    avn[1:imax + 1] = xm[2:imax + 2] - xm[1:imax + 1]
+    ave[2:jmax + 1] = y[2:jmax + 1] - y[1:jmax]
+    return dict(xm=xm, ym=ym, aun=aun, aue=aue, avn=avn, ave=ave)
+
+
+def porous_block(x, y, x0, x1, y0, y1, eps_lo, eps_hi, seed):
+    """eps, lp (p layout): cells with x0<=x<=x1 and y0<=y<=y1 get lp=1 and a porosity
+    drawn per cell from U[eps_lo, eps_hi); all other cells eps=1, lp=0."""
+    imax, jmax = len(x) - 1, len(y) - 1
+    eps = zeros("p", imax, jmax); lp = zeros("p", imax, jmax)
+    eps[1:, 1:] = 1.0
+    X, Y = np.meshgrid(x[1:], y[1:], indexing="ij")
+    inside = (X >= x0) & (X <= x1) & (Y >= y0) & (Y <= y1)
+    draws = uniform(seed, imax * jmax, eps_lo, eps_hi).reshape(imax, jmax)
+    eps[1:, 1:] = np.where(inside, draws, 1.0)
+    lp[1:, 1:] = np.where(inside, 1.0, 0.0)
+    return eps, lp
+
+
+# ---- state ------------------------------------------------------------------------------
+def perturbed_state(imax, jmax, seed, amp=1e-3, u0=0.0, v0=0.0, p0=0.0, T0=1.0, C0=0.0,
+                    perturb_scalars=False):
+    """u=u0, v=v0, p=p0 (+/- amp uniform per DOF, seed), T=T0, C=C0 on every entry of the
+    reference's (1-based) index range."""
+    st = {}
+    n = 0
+    for name, kind, base in (("um", "u", u0), ("vm", "v", v0), ("p", "p", p0),
+                             ("T", "p", T0), ("C", "p", C0)):
+        a = zeros(kind, imax, jmax)
+        fs = fortran_shape(kind, imax, jmax)
+        cnt = fs[0] * fs[1]
+        pert = name in ("um", "vm", "p") or perturb_scalars
+        vals = base + (uniform(seed * 1000003 + n, cnt, -amp, amp) if pert else 0.0)
+        a[1:fs[0] + 1, 1:fs[1] + 1] = np.reshape(vals, fs) if pert else base
+        st[name] = a
+        n += 1
+    return st
+
+
+def full_state(base):
+    """Expand (um, vm, p, T, C) to the 10 input fields of a pseudo-step with the
+    unset levels pinned: um_tau=um, vm_tau=vm, T_tau=T, C_tau=C (pn=p is implied)."""
+    st = {k: np.array(v, copy=True) for k, v in base.items()}
+    st["um_tau"] = st["um"].copy(); st["vm_tau"] = st["vm"].copy()
+    st["T_tau"] = st["T"].copy(); st["C_tau"] = st["C"].copy()
+    return st
+
+
+# ---- BASELINE configurations ---------------------------------------------------------
+def config_c1():
+    """51x51 unit square, left inflow u=1, right outflow, no-slip walls, Re=100, beta=1,
+    global first-order upwind, no porosity/buoyancy/species; seeded +/-1e-3 perturbation."""
+    n = 51
+    x = uniform_nodes(n); y = uniform_nodes(n)
+    prm = make_params(n, n, scheme=SCHEME_UPWIND, bc=BC_CHANNEL, re=100.0, beta=1.0,
+                      g=0.0, invfr2=0.0, darcy=1.0, cf=0.0, u_in=1.0,
+                      t_bottom=1.0, t_top=1.0, slot_lo=1.0, slot_hi=0.0)
+    eps = zeros("p", n, n); eps[1:, 1:] = 1.0
+    lp = zeros("p", n, n)
+    st = perturbed_state(n, n, seed=1, amp=1e-3)
+    return dict(name="c1", params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
+
+
+def config_c2(n=204):
+    """204x204 uniform, QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2),
+    Da=1e-3, Cf=0.55, Boussinesq Ri=1 with hot bottom (1) and cold top (0)."""
+    x = uniform_nodes(n); y = uniform_nodes(n)
+    prm = make_params(n, n, bc=BC_CHANNEL, re=200.0, beta=1.0, g=0.0, darcy=1e-3, cf=0.55,
+                      buoyancy=BUOY_BOUSSINESQ, ri=1.0, t_ref=0.5, invfr2=0.0,
+                      u_in=1.0, t_bottom=1.0, t_top=0.0, slot_lo=1.0, slot_hi=0.0)
+    eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
+    st = perturbed_state(n, n, seed=2, amp=1e-3, T0=0.5)
+    return dict(name="c2", params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
+
+
+def mu_poly_coefficients(seed=3):
+    """mu(T)/mu_ref = c0 + c1 T + ... + c4 T^4, a plausible gas fit (mu rises ~T^0.7 over
+    T in [0,1] nondimensional), with seeded +/-2% perturbations of each coefficient."""
+    base = np.array([1.0, 0.70, -0.12, 0.03, -0.004])
+    return tuple(base * (1.0 + uniform(seed, 5, -0.02, 0.02)))
+
+
+def config_c3(imax=2048, jmax=2048):
+    """Full physics at scale: tanh-clustered mesh (Q=2) toward the walls, QUICK, porous
+    middle-third block (eps~U[0.4,0.9]), Boussinesq Ri=2, species slot Z=1 of width 0.1 on
+    the left wall, Sc=0.7, mu(T) polynomial (seed 3)."""
+    x = tanh_nodes(imax, 0.0, 1.0, 0.0, 2.0)
+    y = tanh_nodes(jmax, 0.0, 1.0, 0.0, 2.0)
+    prm = make_params(imax, jmax, bc=BC_CHANNEL, re=200.0, beta=1.0, g=0.0, darcy=1e-3,
+                      cf=0.55, buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, invfr2=0.0,
+                      sc=0.7, pe=200.0 * 0.7, u_in=1.0, t_bottom=1.0, t_top=0.0,
+                      slot_lo=0.45, slot_hi=0.55, slot_value=1.0,
+                      mu_model=MU_POLY4, mu_coef=mu_poly_coefficients(3))
+    eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
+    st = perturbed_state(imax, jmax, seed=3, amp=1e-3, T0=0.5)
+    return dict(name="c3_%dx%d" % (imax, jmax), params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
