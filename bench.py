@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark: pseudo-time iteration throughput (cell-updates/s) on one MI355X.
+
+Workload (default): configuration 3a — 2048 x 2048 tanh-clustered mesh, QUICK, porous
+middle-third block, Boussinesq Ri=2, species slot, Sc=0.7, mu(T) polynomial; FP64 drop-in.
+A "step" is one pseudo-step (three SSP-RK3 stages of u, v, p, T, C with their boundary
+conditions and the residual maxima) over the whole grid; one cell-update is one pseudo-step
+at one p-cell, so value = imax * jmax * steps / seconds.
+
+Prints one JSON line. `roofline` prices the dominant kernel against HBM (this path is
+memory-bound and uses no MFMA): achieved = algorithmic bytes per pseudo-step (B bytes per
+cell: every field read once, every output written once) / mean kernel time per step from
+HIP events on the context's stream. `cpu_baseline` times the CPU oracle (the C
+restatement, OpenMP) on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "solver-cfd_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from cfd_amd import lib as L  # noqa: E402
+from cfd_amd import cases as C  # noqa: E402
+from cfd_amd.params import PREC_FP64, PREC_FP32  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell-update (reads + writes, each field once)
+BYTES_PER_CELL = {
+    ("c3", PREC_FP64): 128,   # reads um vm um_tau vm_tau p T T_tau C C_tau eps lp, writes 5
+    ("c2", PREC_FP64): 128,
+    ("c3", PREC_FP32): 72,    # 10 FP32 reads + FP64 p, 4 FP32 writes + FP64 pn
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, seconds):
+    """Time the CPU oracle on the same workload: whole pseudo-steps until `seconds` pass."""
+    import oracle as O
+    O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(cfg["params"], cfg["x"], cfg["y"], cfg["eps"], cfg["lp"])
+    s, cs = o.make_state(cfg["state"])
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.pseudo_steps(s, cs, 1, history=False)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and n >= 1:
+            break
+    cells = cfg["params"].imax * cfg["params"].jmax
+    return dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
+                sample="%d pseudo-step(s) of the same %dx%d workload, %.1f s, C oracle -O2 OpenMP"
+                       % (n, cfg["params"].imax, cfg["params"].jmax, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c3", choices=["c3", "c2"])
+    ap.add_argument("--imax", type=int, default=2048)
+    ap.add_argument("--jmax", type=int, default=2048)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--kernel", type=int, default=1, help="0: per-operator path, 1: fused")
+    ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: eager)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_c3_2048x2048_fp64.json"))
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the pseudo-time "
+              "iteration runs on one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    prec = PREC_FP64 if args.precision == "fp64" else PREC_FP32
+    if args.config == "c3":
+        cfg = C.config_c3(args.imax, args.jmax)
+    else:
+        cfg = C.config_c2(args.imax)
+    prm = cfg["params"]
+    cells = prm.imax * prm.jmax
+    log("workload %s %dx%d %s, building context" % (cfg["name"], prm.imax, prm.jmax, args.precision))
+    ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], precision=prec)
+    ctx.set_kernel(args.kernel)
+    ctx.set_state(cfg["state"])
+    ctx.pseudo_steps(max(args.warmup, 1))
+    log("warmup done (%d steps), timing %d steps" % (args.warmup, args.steps))
+
+    t0 = time.perf_counter()
+    if args.graph:
+        reps = max(1, args.steps // args.graph)
+        res = ctx.pseudo_steps_graph(args.graph, reps)
+        steps = reps * args.graph
+    else:
+        res = ctx.pseudo_steps(args.steps)
+        steps = args.steps
+    wall = time.perf_counter() - t0
+    ms_event = ctx.last_ms_per_step()
+    if not np.all(np.isfinite(res)):
+        raise SystemExit("non-finite residual %s" % res)
+    log("timed: %.3f s wall, %.4f ms/step (events), residuals %s" % (wall, ms_event, res))
+
+    value = cells * steps / wall
+    B = BYTES_PER_CELL.get((args.config, prec), 128)
+    achieved = B * cells / (ms_event * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            t = json.load(f)
+        if t.get("imax") == prm.imax and t.get("jmax") == prm.jmax and t.get("kernel") == args.kernel:
+            traffic = t.get("bytes_per_step")
+    out = {
+        "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == PREC_FP64 else "f32+f64",
+        "data": "synthetic (seeded SplitMix64 inputs of configuration 3: mesh, porosity, state)",
+        "config": {"workload": "%s fused pseudo-step, %s" % (cfg["name"], args.precision),
+                   "imax": prm.imax, "jmax": prm.jmax, "kernel": args.kernel,
+                   "graph_steps": args.graph, "bytes_per_cell": B},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernel_ms_per_step": ms_event,
+    }
+    ctx.close()
+    if not args.no_cpu:
+        log("cpu baseline (%.0f s sample)" % args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -156,13 +156,25 @@ def full_state(base):
     return st
 
 
+def stable_dt(x, y, nu, kappa, umax, dtau, safety=0.4):
+    """Physical step dt that keeps the explicit pseudo-time update stable.
+
+    The pseudo-step advances d(phi)/d(tau) = (phi^n - phi) + dt R(phi) with SSP-RK3 over
+    dtau, so dtau (1 + dt |lambda_max(R)|) must stay inside RK3's stability interval (~2.5).
+    |lambda_max| is bounded by the diffusive 4 D (1/dx^2 + 1/dy^2) plus the convective
+    2 |u| (1/dx + 1/dy) of the smallest cell."""
+    dx = float(np.min(np.diff(x[1:]))); dy = float(np.min(np.diff(y[1:])))
+    lam = 4.0 * max(nu, kappa) * (1.0 / dx ** 2 + 1.0 / dy ** 2) + 2.0 * umax * (1.0 / dx + 1.0 / dy)
+    return (safety * 2.5 / dtau - 1.0) / lam
+
+
 # ---- BASELINE configurations ---------------------------------------------------------
 def config_c1():
     """51x51 unit square, left inflow u=1, right outflow, no-slip walls, Re=100, beta=1,
     global first-order upwind, no porosity/buoyancy/species; seeded +/-1e-3 perturbation."""
     n = 51
     x = uniform_nodes(n); y = uniform_nodes(n)
-    prm = make_params(n, n, scheme=SCHEME_UPWIND, bc=BC_CHANNEL, re=100.0, beta=1.0,
+    prm = make_params(n, n, scheme=SCHEME_UPWIND, bc=BC_CHANNEL, re=100.0, pe=100.0, beta=1.0,
                       g=0.0, invfr2=0.0, darcy=1.0, cf=0.0, u_in=1.0,
                       t_bottom=1.0, t_top=1.0, slot_lo=1.0, slot_hi=0.0)
     eps = zeros("p", n, n); eps[1:, 1:] = 1.0
@@ -175,7 +187,7 @@ def config_c2(n=204):
     """204x204 uniform, QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2),
     Da=1e-3, Cf=0.55, Boussinesq Ri=1 with hot bottom (1) and cold top (0)."""
     x = uniform_nodes(n); y = uniform_nodes(n)
-    prm = make_params(n, n, bc=BC_CHANNEL, re=200.0, beta=1.0, g=0.0, darcy=1e-3, cf=0.55,
+    prm = make_params(n, n, bc=BC_CHANNEL, re=200.0, pe=140.0, beta=1.0, g=0.0, darcy=1e-3, cf=0.55,
                       buoyancy=BUOY_BOUSSINESQ, ri=1.0, t_ref=0.5, invfr2=0.0,
                       u_in=1.0, t_bottom=1.0, t_top=0.0, slot_lo=1.0, slot_hi=0.0)
     eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
@@ -196,11 +208,14 @@ def config_c3(imax=2048, jmax=2048):
     the left wall, Sc=0.7, mu(T) polynomial (seed 3)."""
     x = tanh_nodes(imax, 0.0, 1.0, 0.0, 2.0)
     y = tanh_nodes(jmax, 0.0, 1.0, 0.0, 2.0)
-    prm = make_params(imax, jmax, bc=BC_CHANNEL, re=200.0, beta=1.0, g=0.0, darcy=1e-3,
+    mu = mu_poly_coefficients(3)
+    nu_max = sum(abs(c) for c in mu) / 200.0
+    dt = stable_dt(x, y, nu_max, 1.0 / (200.0 * 0.7), 1.5, 0.05)
+    prm = make_params(imax, jmax, dt=dt, bc=BC_CHANNEL, re=200.0, beta=1.0, g=0.0, darcy=1e-3,
                       cf=0.55, buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, invfr2=0.0,
                       sc=0.7, pe=200.0 * 0.7, u_in=1.0, t_bottom=1.0, t_top=0.0,
                       slot_lo=0.45, slot_hi=0.55, slot_value=1.0,
-                      mu_model=MU_POLY4, mu_coef=mu_poly_coefficients(3))
+                      mu_model=MU_POLY4, mu_coef=mu)
     eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
     st = perturbed_state(imax, jmax, seed=3, amp=1e-3, T0=0.5)
     return dict(name="c3_%dx%d" % (imax, jmax), params=prm, x=x, y=y, eps=eps, lp=lp, state=st)

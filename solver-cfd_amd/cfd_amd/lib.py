@@ -1,0 +1,216 @@
+"""ctypes binding of libcfd_amd.so (include/cfd.h, include/cfd_legacy.h).
+
+The library is loaded from this package directory; a missing library is an error, never a
+silent fallback. ``Context`` wraps one cfd_ctx: numpy arrays in the C layout go in and out,
+all computation happens in the HIP kernels.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import CfdParams, LAYOUT_C, PREC_FP64, FIELD_NAMES
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfd_amd.so")
+
+CFD_OK = 0
+ERRORS = {-1: "CFD_ERR_ARG", -2: "CFD_ERR_HIP", -3: "CFD_ERR_NONFINITE", -4: "CFD_ERR_NOCONV",
+          -5: "CFD_ERR_DIVERGED", -6: "CFD_ERR_STATE", -7: "CFD_ERR_UNSUPPORTED"}
+
+# every symbol include/cfd.h and include/cfd_legacy.h declare
+EXPORTS_CFD = [
+    "cfd_params_default", "cfd_mesh_1d", "cfd_geometry_derive", "cfd_ctx_create", "cfd_ctx_destroy",
+    "cfd_ctx_get_params", "cfd_ctx_set_params", "cfd_ctx_set_mesh", "cfd_ctx_upload", "cfd_ctx_download",
+    "cfd_ctx_device_ptr", "cfd_ctx_sync_levels", "cfd_pseudo_steps", "cfd_run_to_tol", "cfd_physical_step",
+    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_last_error", "cfd_version",
+]
+EXPORTS_LEGACY = [
+    "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
+    "solve_Z", "solve_C", "bcUV", "bcP", "bcZ", "bcC", "max_reduce", "alocar_globais", "desalocar_globais",
+    "cfd_legacy_init", "cfd_legacy_set_params", "cfd_legacy_set_mesh", "cfd_legacy_context",
+]
+
+
+class CfdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, code), msg))
+        self.code = code
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def load():
+    """Load libcfd_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError("libcfd_amd.so not built: run `python -m cfd_amd.build` in solver-cfd_amd/ "
+                                "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    P = ctypes.POINTER(CfdParams)
+    sig = {
+        "cfd_params_default": (None, [P, i32, i32]),
+        "cfd_mesh_1d": (ctypes.c_int, [i32, dbl, dbl, dbl, dbl, _dp]),
+        "cfd_geometry_derive": (ctypes.c_int, [i32, i32, _dp, _dp] + [_dp] * 10),
+        "cfd_ctx_create": (ctypes.c_int, [P, i32, ctypes.POINTER(_vp)]),
+        "cfd_ctx_destroy": (ctypes.c_int, [_vp]),
+        "cfd_ctx_get_params": (ctypes.c_int, [_vp, P]),
+        "cfd_ctx_set_params": (ctypes.c_int, [_vp, P]),
+        "cfd_ctx_set_mesh": (ctypes.c_int, [_vp, _dp, _dp]),
+        "cfd_ctx_upload": (ctypes.c_int, [_vp, i32, _dp, i32]),
+        "cfd_ctx_download": (ctypes.c_int, [_vp, i32, _dp, i32]),
+        "cfd_ctx_device_ptr": (ctypes.c_int, [_vp, i32, ctypes.POINTER(_vp)]),
+        "cfd_ctx_sync_levels": (ctypes.c_int, [_vp]),
+        "cfd_pseudo_steps": (ctypes.c_int, [_vp, i32, _dp, _dp]),
+        "cfd_run_to_tol": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp]),
+        "cfd_physical_step": (ctypes.c_int, [_vp, dbl, i64, i32, ctypes.POINTER(i64), _dp]),
+        "cfd_pseudo_steps_graph": (ctypes.c_int, [_vp, i32, i32, _dp]),
+        "cfd_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
+        "cfd_ctx_set_kernel": (ctypes.c_int, [_vp, i32]),
+        "cfd_last_error": (ctypes.c_char_p, []),
+        "cfd_version": (ctypes.c_char_p, []),
+        "cfd_legacy_init": (ctypes.c_int, [P]),
+        "cfd_legacy_set_params": (ctypes.c_int, [P]),
+        "cfd_legacy_set_mesh": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
+        "cfd_legacy_context": (_vp, []),
+        "RESU": (None, [_vp] * 4), "RESV": (None, [_vp] * 5), "RESZ": (None, [_vp] * 4),
+        "RESC": (None, [_vp] * 4), "upwind_U_pair": (None, [_vp] * 4), "upwind_V_pair": (None, [_vp] * 5),
+        "solve_U": (dbl, [_vp] * 10), "solve_V": (dbl, [_vp] * 11), "solve_P": (dbl, [_vp] * 4),
+        "solve_Z": (None, [_vp] * 6), "solve_C": (None, [_vp] * 6),
+        "bcUV": (None, [_vp] * 2), "bcP": (None, [_vp]), "bcZ": (None, [_vp]), "bcC": (None, [_vp]),
+        "max_reduce": (dbl, [_vp, ctypes.c_int, ctypes.c_int]),
+        "alocar_globais": (None, []), "desalocar_globais": (None, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return load().cfd_last_error().decode()
+
+
+def check(code):
+    if code != CFD_OK:
+        raise CfdError(code, last_error())
+    return code
+
+
+def _ptr(a):
+    assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One solver context (cfd_ctx) on the current HIP device."""
+
+    def __init__(self, params, x, y, eps=None, lp=None, precision=PREC_FP64):
+        L = load()
+        self.params = params
+        self.imax, self.jmax = params.imax, params.jmax
+        h = _vp()
+        check(L.cfd_ctx_create(ctypes.byref(params), precision, ctypes.byref(h)))
+        self.h = h
+        x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+        check(L.cfd_ctx_set_mesh(self.h, _ptr(x), _ptr(y)))
+        if eps is not None:
+            self.upload("eps", eps)
+        if lp is not None:
+            self.upload("lp", lp)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cfd_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _shape(self, name):
+        from .cases import shape_of, FIELD_KIND
+        return shape_of(FIELD_KIND[name], self.imax, self.jmax)
+
+    def upload(self, name, arr, layout=LAYOUT_C):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if layout == LAYOUT_C and a.shape != self._shape(name):
+            raise ValueError("%s: shape %s, expected %s" % (name, a.shape, self._shape(name)))
+        check(load().cfd_ctx_upload(self.h, FIELD_NAMES.index(name), _ptr(a), layout))
+
+    def download(self, name, layout=LAYOUT_C):
+        if layout == LAYOUT_C:
+            out = np.zeros(self._shape(name))
+        else:
+            from .cases import fortran_shape, FIELD_KIND
+            n, m = fortran_shape(FIELD_KIND[name], self.imax, self.jmax)
+            out = np.zeros(n * m)
+        check(load().cfd_ctx_download(self.h, FIELD_NAMES.index(name), _ptr(out), layout))
+        return out
+
+    def set_state(self, st):
+        """st: dict of any of um, vm, um_tau, vm_tau, p, T, T_tau, C, C_tau (C layout).
+        Missing iterates are pinned to their physical level (um_tau := um, ...)."""
+        for k in ("um", "vm", "p", "T", "C"):
+            if k in st:
+                self.upload(k, st[k])
+        for k, base in (("um_tau", "um"), ("vm_tau", "vm"), ("T_tau", "T"), ("C_tau", "C")):
+            src = st.get(k, st.get(base))
+            if src is not None:
+                self.upload(k, src)
+
+    def set_params(self, params):
+        check(load().cfd_ctx_set_params(self.h, ctypes.byref(params)))
+        self.params = params
+
+    def set_kernel(self, k):
+        check(load().cfd_ctx_set_kernel(self.h, int(k)))
+
+    def pseudo_steps(self, n, history=False):
+        res = np.zeros(3)
+        hist = np.zeros((n, 3)) if history else None
+        check(load().cfd_pseudo_steps(self.h, int(n), _ptr(res), _ptr(hist) if history else None))
+        return (res, hist) if history else res
+
+    def pseudo_steps_graph(self, steps_per_graph, replays):
+        res = np.zeros(3)
+        check(load().cfd_pseudo_steps_graph(self.h, int(steps_per_graph), int(replays), _ptr(res)))
+        return res
+
+    def run_to_tol(self, tol, itc_max, check_every=100, diverge_factor=0.0, raise_on_fail=True):
+        it = ctypes.c_int64(0)
+        res = np.zeros(3)
+        code = load().cfd_run_to_tol(self.h, float(tol), int(itc_max), int(check_every), float(diverge_factor),
+                                     ctypes.byref(it), _ptr(res))
+        if raise_on_fail:
+            check(code)
+        return code, it.value, res
+
+    def physical_step(self, eps, itc_max, first):
+        it = ctypes.c_int64(0)
+        res = np.zeros(3)
+        check(load().cfd_physical_step(self.h, float(eps), int(itc_max), 1 if first else 0, ctypes.byref(it),
+                                       _ptr(res)))
+        return it.value, res
+
+    def last_ms_per_step(self):
+        v = ctypes.c_double(0)
+        check(load().cfd_ctx_last_timing(self.h, ctypes.byref(v)))
+        return v.value
+
+    def device_ptr(self, name):
+        p = _vp()
+        check(load().cfd_ctx_device_ptr(self.h, FIELD_NAMES.index(name), ctypes.byref(p)))
+        return p.value

@@ -1,0 +1,993 @@
+/*
+ * cfd_host.c — host side of libcfd_amd: contexts, drivers and the drop-in entry points.
+ *
+ * Plain C11 over the HIP runtime API. Kernels are launched through the C launchers of
+ * cfd_ops.hip / cfd_fused.hip (cfd_internal.h).
+ *
+ * Two step implementations share one state:
+ *   exact  — the reference call structure (solve_U, solve_V, solve_P, solve_Z, solve_C of
+ *            OpenMp_Multicore/equations.f90 with the bcUV side effects on um_tau/vm_tau),
+ *            one kernel per operator; used for the first step after new inputs (when the
+ *            iterates may violate the boundary conditions) and by the legacy names;
+ *   fused  — cfdk_fused_step on boundary-consistent iterates.
+ * The tau-level update of main.f90:292-296 is a pointer swap.
+ */
+#include "cfd_internal.h"
+#include "../../include/cfd_legacy.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------------------ */
+/* errors                                                                                */
+static _Thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(CFD_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+#define CHK(call)                                                                      \
+    do {                                                                               \
+        int r_ = (call);                                                               \
+        if (r_ != CFD_OK) {                                                            \
+            if (r_ == CFD_ERR_HIP && !g_err[0])                                        \
+                fail(r_, "kernel launch failed in %s: %s", #call,                      \
+                     hipGetErrorString(hipGetLastError()));                            \
+            return r_;                                                                 \
+        }                                                                              \
+    } while (0)
+
+const char *cfd_last_error(void) { return g_err; }
+const char *cfd_version(void) { return "cfd_amd 0.1 (gfx950)"; }
+
+/* ------------------------------------------------------------------------------------ */
+/* parameters and mesh                                                                  */
+void cfd_params_default(cfd_params *p, int32_t imax, int32_t jmax)
+{
+    memset(p, 0, sizeof *p);
+    p->imax = imax; p->jmax = jmax;
+    p->scheme = CFD_SCHEME_QUICK; p->bc = CFD_BC_REFERENCE; p->buoyancy = CFD_BUOY_REFERENCE;
+    p->compat = CFD_COMPAT_FORTRAN; p->steady = 0; p->mu_model = CFD_MU_CONST;
+    p->re = 1.0; p->pe = 1.0; p->sc = 1.0; p->darcy = 1.0e-2;
+    p->cf = 1.75 / sqrt(150.0 * 0.5 * 0.5 * 0.5);   /* comum.f90:31, porosidade = 0.5 */
+    p->g = 9.80665; p->invfr2 = 1.0; p->ri = 0.0; p->t_ref = 0.0;
+    p->v_i = 0.5; p->tinf = 1.0; p->dtau = 5.0e-2; p->dt = 0.5e-2;
+    p->beta = 1300.0; p->b_art = 0.0;
+    p->u_in = 1.0; p->t_bottom = 1.0; p->t_top = 0.0;
+    p->slot_lo = 0.0; p->slot_hi = 0.0; p->slot_value = 1.0;
+    p->mu_coef[0] = 1.0;
+}
+
+static int check_params(const cfd_params *p)
+{
+    if (!p) return fail(CFD_ERR_ARG, "params is NULL");
+    if (p->imax < 6 || p->jmax < 6) return fail(CFD_ERR_ARG, "grid %dx%d too small (need >= 6x6)", p->imax, p->jmax);
+    if ((int64_t)p->imax * p->jmax > (int64_t)1 << 31) return fail(CFD_ERR_ARG, "grid too large");
+    if (p->scheme < 0 || p->scheme > 1) return fail(CFD_ERR_ARG, "unknown scheme %d", p->scheme);
+    if (p->bc < 0 || p->bc > 1) return fail(CFD_ERR_ARG, "unknown bc set %d", p->bc);
+    if (p->buoyancy < 0 || p->buoyancy > 1) return fail(CFD_ERR_ARG, "unknown buoyancy model %d", p->buoyancy);
+    if (p->compat < 0 || p->compat > 1) return fail(CFD_ERR_ARG, "unknown compat %d", p->compat);
+    if (p->mu_model < 0 || p->mu_model > 1) return fail(CFD_ERR_ARG, "unknown mu model %d", p->mu_model);
+    if (!(p->re > 0) || !(p->pe > 0) || !(p->sc > 0) || !(p->darcy > 0))
+        return fail(CFD_ERR_ARG, "Re, Pe, Sc and Da must be positive");
+    return CFD_OK;
+}
+
+/* Two-sided tanh stretch of nonsymetric_mesh.f90:17-29 on [a, b]. */
+int cfd_mesh_1d(int32_t n, double a, double b, double P, double Q, double *x)
+{
+    if (n < 3 || !x) return fail(CFD_ERR_ARG, "cfd_mesh_1d: n=%d", n);
+    if (P != 1.0 && !(Q > 0)) return fail(CFD_ERR_ARG, "cfd_mesh_1d: Q must be positive");
+    const double h = (b - a) / (n - 1);
+    for (int i = 1; i <= n; ++i) x[i] = a + (double)(i - 1) * h;
+    if (P == 1.0) return CFD_OK;
+    const int c = n / 2 + 1;
+    const double xn = x[n], x0 = x[1], tq = tanh(Q);
+    for (int i = c; i <= n; ++i) {
+        const double eta = (x[i] - xn) / (x[c] - xn);
+        const double s = P * eta + (1.0 - P) * (1.0 - tanh(Q * (1.0 - eta)) / tq);
+        x[i] = xn - s * (xn - x[c]);
+    }
+    for (int i = c; i >= 1; --i) {
+        const double eta = (x[i] - x0) / (x[c] - x0);
+        const double s = P * eta + (1.0 - P) * (1.0 - tanh(Q * (1.0 - eta)) / tq);
+        x[i] = x0 + s * (x[c] - x0);
+    }
+    return CFD_OK;
+}
+
+/* nonsymetric_mesh.f90:48-101 */
+int cfd_geometry_derive(int32_t imax, int32_t jmax, const double *x, const double *y,
+                        double *xm, double *ym,
+                        double *aun, double *aus, double *aue, double *auw,
+                        double *avn, double *avs, double *ave, double *avw)
+{
+    if (imax < 3 || jmax < 3 || !x || !y) return fail(CFD_ERR_ARG, "cfd_geometry_derive: bad arguments");
+    xm[1] = -(x[2] + x[1]) * 0.5;
+    xm[imax + 1] = (x[imax] + x[imax - 1]) * 0.5 + (x[imax] - x[imax - 1]);
+    for (int i = 2; i <= imax; ++i) xm[i] = (x[i] + x[i - 1]) * 0.5;
+    ym[1] = y[1] - (y[2] - y[1]) * 0.5;
+    ym[jmax + 1] = (y[jmax] + y[jmax - 1]) * 0.5 + (y[jmax] - y[jmax - 1]);
+    for (int j = 2; j <= jmax; ++j) ym[j] = (y[j] + y[j - 1]) * 0.5;
+    for (int i = 2; i <= imax; ++i) { aun[i] = x[i] - x[i - 1]; aus[i] = aun[i]; }
+    for (int j = 1; j <= jmax; ++j) { aue[j] = ym[j + 1] - ym[j]; auw[j] = aue[j]; }
+    for (int i = 1; i <= imax; ++i) { avn[i] = xm[i + 1] - xm[i]; avs[i] = avn[i]; }
+    for (int j = 2; j <= jmax; ++j) { ave[j] = y[j] - y[j - 1]; avw[j] = ave[j]; }
+    return CFD_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* context                                                                               */
+enum { S_UM, S_VM, S_UMT, S_VMT, S_UMN, S_VMN, S_P, S_PN, S_T, S_TT, S_TN, S_C, S_CT, S_CN, S_NSTATE };
+enum { X_RU, X_RV, X_RZ, X_RC, X_UI, X_VI, X_ZI, X_CI, X_PI, X_RP, X_RESU, X_RESV, X_RESP,
+       X_U1, X_U2, X_V1, X_V2, X_Z1, X_Z2, X_C1, X_C2, X_NSCR };
+
+struct cfd_ctx {
+    cfd_params P;
+    int32_t prec, kernel, device;
+    int owns_stream;
+    hipStream_t s;
+    cfd_devgeo g;
+    cfd_devfactors fac;
+    int have_mesh, bc_clean, uv_reads_pn, first_step, have_fused, pending_in_tau;
+    size_t np, nu, nv;
+    double *st[S_NSTATE];
+    double *scr[X_NSCR];
+    uint64_t *d_hist;      /* 3 uint64 per step */
+    int64_t hist_cap;
+    uint64_t *h_hist;      /* pinned mirror */
+    hipEvent_t ev0, ev1;
+    double last_ms_per_step;
+};
+
+static size_t kind_count(const cfd_ctx *c, char kind)
+{
+    return kind == 'u' ? c->nu : (kind == 'v' ? c->nv : c->np);
+}
+
+static int dalloc(double **p, size_t n)
+{
+    HIPCHK(hipMalloc((void **)p, n * sizeof(double)));
+    HIPCHK(hipMemset(*p, 0, n * sizeof(double)));
+    return CFD_OK;
+}
+
+static int ensure_hist(cfd_ctx *c, int64_t n)
+{
+    if (n <= c->hist_cap) return CFD_OK;
+    if (c->d_hist) { hipFree(c->d_hist); c->d_hist = NULL; }
+    if (c->h_hist) { hipHostFree(c->h_hist); c->h_hist = NULL; }
+    int64_t cap = n < 64 ? 64 : n;
+    HIPCHK(hipMalloc((void **)&c->d_hist, (size_t)cap * 3 * sizeof(uint64_t)));
+    HIPCHK(hipHostMalloc((void **)&c->h_hist, (size_t)cap * 3 * sizeof(uint64_t), hipHostMallocDefault));
+    c->hist_cap = cap;
+    return CFD_OK;
+}
+
+static int ctx_alloc(cfd_ctx *c)
+{
+    const int im = c->P.imax, jm = c->P.jmax;
+    c->np = (size_t)(im + 1) * (size_t)(jm + 1);
+    c->nu = (size_t)(im + 2) * (size_t)(jm + 1);
+    c->nv = (size_t)(im + 1) * (size_t)(jm + 2);
+    cfd_devgeo *g = &c->g;
+    g->imax = im; g->jmax = jm;
+    CHK(dalloc(&g->x, im + 1)); CHK(dalloc(&g->y, jm + 1));
+    CHK(dalloc(&g->xm, im + 2)); CHK(dalloc(&g->ym, jm + 2));
+    CHK(dalloc(&g->aun, im + 2)); CHK(dalloc(&g->aus, im + 2));
+    CHK(dalloc(&g->aue, jm + 1)); CHK(dalloc(&g->auw, jm + 1));
+    CHK(dalloc(&g->avn, im + 1)); CHK(dalloc(&g->avs, im + 1));
+    CHK(dalloc(&g->ave, jm + 2)); CHK(dalloc(&g->avw, jm + 2));
+    CHK(dalloc(&g->eps, c->np)); CHK(dalloc(&g->lp, c->np));
+    CHK(dalloc(&c->fac.rdx, im + 2)); CHK(dalloc(&c->fac.rdy, jm + 2));
+    CHK(dalloc(&c->fac.rdxm, im + 2)); CHK(dalloc(&c->fac.rdym, jm + 2));
+    static const char kinds[S_NSTATE] = {'u', 'v', 'u', 'v', 'u', 'v', 'p', 'p', 'p', 'p', 'p', 'p', 'p', 'p'};
+    for (int k = 0; k < S_NSTATE; ++k) CHK(dalloc(&c->st[k], kind_count(c, kinds[k])));
+    static const char skinds[X_NSCR] = {'u', 'v', 'p', 'p', 'u', 'v', 'p', 'p', 'p', 'p', 'u', 'v', 'p',
+                                        'u', 'u', 'v', 'v', 'p', 'p', 'p', 'p'};
+    for (int k = 0; k < X_NSCR; ++k) CHK(dalloc(&c->scr[k], kind_count(c, skinds[k])));
+    CHK(ensure_hist(c, 64));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    /* eps = 1 everywhere until a porosity field is uploaded */
+    CHK(cfdk_fill(g->eps, 1.0, (long)c->np, c->s));
+    return CFD_OK;
+}
+
+static void ctx_free(cfd_ctx *c)
+{
+    cfd_devgeo *g = &c->g;
+    double *geo[] = {g->x, g->y, g->xm, g->ym, g->aun, g->aus, g->aue, g->auw, g->avn, g->avs,
+                     g->ave, g->avw, g->eps, g->lp, c->fac.rdx, c->fac.rdy, c->fac.rdxm, c->fac.rdym};
+    for (size_t k = 0; k < sizeof geo / sizeof geo[0]; ++k) if (geo[k]) hipFree(geo[k]);
+    for (int k = 0; k < S_NSTATE; ++k) if (c->st[k]) hipFree(c->st[k]);
+    for (int k = 0; k < X_NSCR; ++k) if (c->scr[k]) hipFree(c->scr[k]);
+    if (c->d_hist) hipFree(c->d_hist);
+    if (c->h_hist) hipHostFree(c->h_hist);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->owns_stream && c->s) hipStreamDestroy(c->s);
+}
+
+static int ctx_init(cfd_ctx *c, const cfd_params *params, int32_t precision, int own_stream)
+{
+    CHK(check_params(params));
+    if (precision != CFD_PREC_FP64 && precision != CFD_PREC_FP32)
+        return fail(CFD_ERR_ARG, "unknown precision %d", precision);
+    c->P = *params;
+    c->prec = precision;
+    HIPCHK(hipGetDevice(&c->device));
+    if (own_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        c->owns_stream = 1;
+    } else {
+        c->s = 0;   /* legacy default stream, ordered with the caller's own launches */
+    }
+    c->first_step = 1;
+    c->have_fused = cfdk_fused_available();
+    c->kernel = 1;
+    return ctx_alloc(c);
+}
+
+int cfd_ctx_create(const cfd_params *params, int32_t precision, cfd_ctx **out)
+{
+    if (!out) return fail(CFD_ERR_ARG, "out is NULL");
+    *out = NULL;
+    cfd_ctx *c = (cfd_ctx *)calloc(1, sizeof *c);
+    if (!c) return fail(CFD_ERR_ARG, "out of host memory");
+    int r = ctx_init(c, params, precision, 1);
+    if (r != CFD_OK) { ctx_free(c); free(c); return r; }
+    *out = c;
+    return CFD_OK;
+}
+
+int cfd_ctx_destroy(cfd_ctx *c)
+{
+    if (!c) return CFD_OK;
+    if (c->s || c->owns_stream) hipStreamSynchronize(c->s);
+    ctx_free(c);
+    free(c);
+    return CFD_OK;
+}
+
+int cfd_ctx_get_params(const cfd_ctx *c, cfd_params *out)
+{
+    if (!c || !out) return fail(CFD_ERR_ARG, "NULL argument");
+    *out = c->P;
+    return CFD_OK;
+}
+
+int cfd_ctx_set_params(cfd_ctx *c, const cfd_params *p)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    CHK(check_params(p));
+    if (p->imax != c->P.imax || p->jmax != c->P.jmax)
+        return fail(CFD_ERR_ARG, "grid size cannot change (%dx%d -> %dx%d)", c->P.imax, c->P.jmax, p->imax, p->jmax);
+    c->P = *p;
+    return CFD_OK;
+}
+
+int cfd_ctx_set_kernel(cfd_ctx *c, int32_t kernel)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (kernel < 0 || kernel > 1) return fail(CFD_ERR_ARG, "unknown kernel %d", kernel);
+    c->kernel = kernel;
+    return CFD_OK;
+}
+
+static int upload_geometry(cfd_ctx *c, const double *x, const double *y)
+{
+    const int im = c->P.imax, jm = c->P.jmax;
+    double *h = (double *)calloc((size_t)(12 * (im + 3) + 12 * (jm + 3)), sizeof(double));
+    if (!h) return fail(CFD_ERR_ARG, "out of host memory");
+    double *xm = h, *aun = xm + im + 3, *aus = aun + im + 3, *avn = aus + im + 3, *avs = avn + im + 3;
+    double *ym = avs + im + 3, *aue = ym + jm + 3, *auw = aue + jm + 3, *ave = auw + jm + 3, *avw = ave + jm + 3;
+    int r = cfd_geometry_derive(im, jm, x, y, xm, ym, aun, aus, aue, auw, avn, avs, ave, avw);
+    if (r == CFD_OK) {
+        struct { double *d; const double *h; size_t n; } cp[] = {
+            {c->g.x, x, im + 1}, {c->g.y, y, jm + 1}, {c->g.xm, xm, im + 2}, {c->g.ym, ym, jm + 2},
+            {c->g.aun, aun, im + 2}, {c->g.aus, aus, im + 2}, {c->g.aue, aue, jm + 1}, {c->g.auw, auw, jm + 1},
+            {c->g.avn, avn, im + 1}, {c->g.avs, avs, im + 1}, {c->g.ave, ave, jm + 2}, {c->g.avw, avw, jm + 2}};
+        for (size_t k = 0; k < sizeof cp / sizeof cp[0] && r == CFD_OK; ++k) {
+            hipError_t e = hipMemcpy(cp[k].d, cp[k].h, cp[k].n * sizeof(double), hipMemcpyHostToDevice);
+            if (e != hipSuccess) r = fail(CFD_ERR_HIP, "geometry upload: %s", hipGetErrorString(e));
+        }
+    }
+    free(h);
+    if (r == CFD_OK) r = cfdk_factors(&c->g, &c->fac, c->s);
+    return r;
+}
+
+int cfd_ctx_set_mesh(cfd_ctx *c, const double *x, const double *y)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (!x || !y) return fail(CFD_ERR_ARG, "x or y is NULL");
+    for (int i = 2; i <= c->P.imax; ++i)
+        if (!(x[i] > x[i - 1])) return fail(CFD_ERR_ARG, "x must increase strictly (x[%d] <= x[%d])", i, i - 1);
+    for (int j = 2; j <= c->P.jmax; ++j)
+        if (!(y[j] > y[j - 1])) return fail(CFD_ERR_ARG, "y must increase strictly (y[%d] <= y[%d])", j, j - 1);
+    CHK(upload_geometry(c, x, y));
+    c->have_mesh = 1;
+    return CFD_OK;
+}
+
+static double *field_ptr(cfd_ctx *c, int32_t f, char *kind)
+{
+    switch (f) {
+    case CFD_F_UM: *kind = 'u'; return c->st[S_UM];
+    case CFD_F_VM: *kind = 'v'; return c->st[S_VM];
+    case CFD_F_UM_TAU: *kind = 'u'; return c->st[S_UMT];
+    case CFD_F_VM_TAU: *kind = 'v'; return c->st[S_VMT];
+    case CFD_F_P: *kind = 'p'; return c->st[S_P];
+    case CFD_F_T: *kind = 'p'; return c->st[S_T];
+    case CFD_F_T_TAU: *kind = 'p'; return c->st[S_TT];
+    case CFD_F_C: *kind = 'p'; return c->st[S_C];
+    case CFD_F_C_TAU: *kind = 'p'; return c->st[S_CT];
+    case CFD_F_EPS: *kind = 'p'; return c->g.eps;
+    case CFD_F_LP: *kind = 'p'; return c->g.lp;
+    case CFD_F_PN: *kind = 'p'; return c->uv_reads_pn ? c->st[S_PN] : c->st[S_P];
+    default: return NULL;
+    }
+}
+
+/* Fortran shape (n, m) of a layout kind; C row stride ld. */
+static void kind_dims(const cfd_ctx *c, char kind, int *n, int *m, int *ld)
+{
+    const int im = c->P.imax, jm = c->P.jmax;
+    if (kind == 'u') { *n = im + 1; *m = jm; *ld = jm + 1; }
+    else if (kind == 'v') { *n = im; *m = jm + 1; *ld = jm + 2; }
+    else { *n = im; *m = jm; *ld = jm + 1; }
+}
+
+int cfd_ctx_upload(cfd_ctx *c, int32_t field, const double *host, int32_t layout)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (!host) return fail(CFD_ERR_ARG, "host pointer is NULL");
+    if (field == CFD_F_PN) return fail(CFD_ERR_ARG, "CFD_F_PN is download-only");
+    char kind;
+    double *d = field_ptr(c, field, &kind);
+    if (!d) return fail(CFD_ERR_ARG, "unknown field %d", field);
+    int n, m, ld;
+    kind_dims(c, kind, &n, &m, &ld);
+    if (layout == CFD_LAYOUT_C) {
+        HIPCHK(hipMemcpyAsync(d, host, kind_count(c, kind) * sizeof(double), hipMemcpyHostToDevice, c->s));
+    } else if (layout == CFD_LAYOUT_FORTRAN) {
+        double *tmp = NULL;
+        HIPCHK(hipMalloc((void **)&tmp, (size_t)n * m * sizeof(double)));
+        hipError_t e = hipMemcpyAsync(tmp, host, (size_t)n * m * sizeof(double), hipMemcpyHostToDevice, c->s);
+        int r = e == hipSuccess ? cfdk_from_fortran(d, tmp, n, m, ld, c->s) : CFD_ERR_HIP;
+        hipStreamSynchronize(c->s);
+        hipFree(tmp);
+        if (r != CFD_OK) return fail(CFD_ERR_HIP, "Fortran-layout upload failed");
+    } else {
+        return fail(CFD_ERR_ARG, "unknown layout %d", layout);
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (field == CFD_F_P) c->uv_reads_pn = 0;
+    if (field != CFD_F_EPS && field != CFD_F_LP) { c->bc_clean = 0; c->first_step = 1; }
+    return CFD_OK;
+}
+
+static int materialize(cfd_ctx *c);
+
+int cfd_ctx_download(cfd_ctx *c, int32_t field, double *host, int32_t layout)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (!host) return fail(CFD_ERR_ARG, "host pointer is NULL");
+    CHK(materialize(c));
+    char kind;
+    double *d = field_ptr(c, field, &kind);
+    if (!d) return fail(CFD_ERR_ARG, "unknown field %d", field);
+    int n, m, ld;
+    kind_dims(c, kind, &n, &m, &ld);
+    if (layout == CFD_LAYOUT_C) {
+        HIPCHK(hipMemcpyAsync(host, d, kind_count(c, kind) * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    } else if (layout == CFD_LAYOUT_FORTRAN) {
+        double *tmp = NULL;
+        HIPCHK(hipMalloc((void **)&tmp, (size_t)n * m * sizeof(double)));
+        int r = cfdk_to_fortran(tmp, d, n, m, ld, c->s);
+        hipError_t e = r == CFD_OK ? hipMemcpyAsync(host, tmp, (size_t)n * m * sizeof(double), hipMemcpyDeviceToHost, c->s)
+                                   : hipErrorUnknown;
+        hipStreamSynchronize(c->s);
+        hipFree(tmp);
+        if (e != hipSuccess) return fail(CFD_ERR_HIP, "Fortran-layout download failed");
+    } else {
+        return fail(CFD_ERR_ARG, "unknown layout %d", layout);
+    }
+    return CFD_OK;
+}
+
+int cfd_ctx_device_ptr(cfd_ctx *c, int32_t field, void **out)
+{
+    if (!c || !out) return fail(CFD_ERR_ARG, "NULL argument");
+    CHK(materialize(c));
+    char kind;
+    double *d = field_ptr(c, field, &kind);
+    if (!d) return fail(CFD_ERR_ARG, "unknown field %d", field);
+    *out = d;
+    return CFD_OK;
+}
+
+int cfd_ctx_sync_levels(cfd_ctx *c)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    CHK(cfdk_copy(c->st[S_UMT], c->st[S_UM], (long)c->nu, c->s));
+    CHK(cfdk_copy(c->st[S_VMT], c->st[S_VM], (long)c->nv, c->s));
+    CHK(cfdk_copy(c->st[S_TT], c->st[S_T], (long)c->np, c->s));
+    CHK(cfdk_copy(c->st[S_CT], c->st[S_C], (long)c->np, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->uv_reads_pn = 0;
+    c->bc_clean = 0;
+    c->first_step = 1;
+    return CFD_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* exact step: the reference call structure                                             */
+typedef struct {
+    const cfd_devgeo *g;
+    const cfd_params *P;
+    hipStream_t s;
+} xenv;
+
+static int x_residual(const xenv *e, int region, const double *res, uint64_t *slot)
+{
+    const int im = e->g->imax, jm = e->g->jmax;
+    if (e->P->compat == CFD_COMPAT_CUDA) {
+        /* max_reduce(dev_res, n, m) over the first n*m linear entries (max_reduce.cu:14-41),
+         * called with (imax-1, jmax-2) for u, (imax, jmax+1)... as solve_*.cu pass them. */
+        long n = region == 0 ? (long)(im - 1) * (jm - 2) : (region == 1 ? (long)im * (jm + 1) : (long)im * jm);
+        return cfdk_maxabs_linear(res, n, slot, e->s);
+    }
+    if (region == 0) return cfdk_maxabs(res, jm + 1, 3, im - 1, 2, jm - 1, slot, e->s);
+    if (region == 1) return cfdk_maxabs(res, jm + 2, 2, im - 1, 3, jm - 1, slot, e->s);
+    return cfdk_maxabs(res, jm + 1, 2, im - 1, 2, jm - 1, slot, e->s);
+}
+
+/* equations.f90:244-301; T (may be NULL) feeds the mu(T) model only. */
+static int x_solve_u(const xenv *e, const double *um, double *um_tau, double *vm_tau, double *um_n_tau,
+                     const double *p, const double *T, double *ui, double *ru, double *res, uint64_t *slot)
+{
+    const cfd_devgeo *g = e->g; const cfd_params *P = e->P;
+    CHK(cfdk_resu(g, P, 0, um_tau, vm_tau, p, T, ru, e->s));
+    CHK(cfdk_stage(g, P, 0, 1, um, um_tau, ru, NULL, ui, res, e->s));
+    CHK(cfdk_bc_uv(g, P, ui, vm_tau, e->s));
+    CHK(cfdk_resu(g, P, 0, ui, vm_tau, p, T, ru, e->s));
+    CHK(cfdk_stage(g, P, 0, 2, um, um_tau, ru, ui, ui, res, e->s));
+    CHK(cfdk_bc_uv(g, P, ui, vm_tau, e->s));
+    CHK(cfdk_resu(g, P, 0, ui, vm_tau, p, T, ru, e->s));
+    CHK(cfdk_stage(g, P, 0, 3, um, um_tau, ru, ui, um_n_tau, res, e->s));
+    CHK(cfdk_bc_uv(g, P, um_n_tau, vm_tau, e->s));
+    return slot ? x_residual(e, 0, res, slot) : CFD_OK;
+}
+
+/* equations.f90:546-603 */
+static int x_solve_v(const xenv *e, const double *vm, double *um_tau, double *vm_tau, double *vm_n_tau,
+                     const double *p, const double *T, double *vi, double *rv, double *res, uint64_t *slot)
+{
+    const cfd_devgeo *g = e->g; const cfd_params *P = e->P;
+    CHK(cfdk_resv(g, P, 0, um_tau, vm_tau, p, T, rv, e->s));
+    CHK(cfdk_stage(g, P, 1, 1, vm, vm_tau, rv, NULL, vi, res, e->s));
+    CHK(cfdk_bc_uv(g, P, um_tau, vi, e->s));
+    CHK(cfdk_resv(g, P, 0, um_tau, vi, p, T, rv, e->s));
+    CHK(cfdk_stage(g, P, 1, 2, vm, vm_tau, rv, vi, vi, res, e->s));
+    CHK(cfdk_bc_uv(g, P, um_tau, vi, e->s));
+    CHK(cfdk_resv(g, P, 0, um_tau, vi, p, T, rv, e->s));
+    CHK(cfdk_stage(g, P, 1, 3, vm, vm_tau, rv, vi, vm_n_tau, res, e->s));
+    CHK(cfdk_bc_uv(g, P, um_tau, vm_n_tau, e->s));
+    return slot ? x_residual(e, 1, res, slot) : CFD_OK;
+}
+
+/* equations.f90:607-665 */
+static int x_solve_p(const xenv *e, const double *p, const double *um_n, const double *vm_n, double *pn,
+                     double *rp, double *pi, double *res, uint64_t *slot)
+{
+    const cfd_devgeo *g = e->g; const cfd_params *P = e->P;
+    CHK(cfdk_pstage(g, P, 1, p, um_n, vm_n, rp, pi, pn, res, e->s));
+    CHK(cfdk_bc_s(g, P, 0, pi, e->s));
+    CHK(cfdk_pstage(g, P, 2, p, um_n, vm_n, rp, pi, pn, res, e->s));
+    CHK(cfdk_bc_s(g, P, 0, pi, e->s));
+    CHK(cfdk_pstage(g, P, 3, p, um_n, vm_n, rp, pi, pn, res, e->s));
+    CHK(cfdk_bc_s(g, P, 0, pn, e->s));
+    return slot ? x_residual(e, 2, res, slot) : CFD_OK;
+}
+
+/* equations.f90:669-719 (species = 0) and 769-819 (species = 1) */
+static int x_solve_s(const xenv *e, int species, const double *um_n, const double *vm_n, const double *z,
+                     double *z_n_tau, const double *z_tau, double *zi, double *rz)
+{
+    const cfd_devgeo *g = e->g; const cfd_params *P = e->P;
+    const int kind = species ? 2 : 1;
+    CHK(cfdk_ressc(g, P, species, um_n, vm_n, z_tau, rz, e->s));
+    CHK(cfdk_stage(g, P, 2, 1, z, z_tau, rz, NULL, zi, NULL, e->s));
+    CHK(cfdk_bc_s(g, P, kind, zi, e->s));
+    CHK(cfdk_ressc(g, P, species, um_n, vm_n, zi, rz, e->s));
+    CHK(cfdk_stage(g, P, 2, 2, z, z_tau, rz, zi, zi, NULL, e->s));
+    CHK(cfdk_bc_s(g, P, kind, zi, e->s));
+    CHK(cfdk_ressc(g, P, species, um_n, vm_n, zi, rz, e->s));
+    CHK(cfdk_stage(g, P, 2, 3, z, z_tau, rz, zi, z_n_tau, NULL, e->s));
+    CHK(cfdk_bc_s(g, P, kind, z_n_tau, e->s));
+    return CFD_OK;
+}
+
+static void swap_ptr(double **a, double **b) { double *t = *a; *a = *b; *b = t; }
+
+static void tau_swap(cfd_ctx *c)
+{
+    swap_ptr(&c->st[S_UMT], &c->st[S_UMN]);
+    swap_ptr(&c->st[S_VMT], &c->st[S_VMN]);
+    swap_ptr(&c->st[S_P], &c->st[S_PN]);
+    swap_ptr(&c->st[S_TT], &c->st[S_TN]);
+    swap_ptr(&c->st[S_CT], &c->st[S_CN]);
+    c->uv_reads_pn = 0;
+    c->pending_in_tau = 1;
+}
+
+/* One pseudo-step (main.f90:269-280) through the per-operator kernels; residuals into slot[0..2]. */
+static int step_exact(cfd_ctx *c, uint64_t *slot)
+{
+    xenv e = {&c->g, &c->P, c->s};
+    const int st = c->P.steady;
+    double **S = c->st, **X = c->scr;
+    const double *um = st ? S[S_UMT] : S[S_UM];
+    const double *vm = st ? S[S_VMT] : S[S_VM];
+    const double *T = st ? S[S_TT] : S[S_T];
+    const double *C = st ? S[S_CT] : S[S_C];
+    const double *p_uv = c->uv_reads_pn ? S[S_PN] : S[S_P];
+    if (c->P.bc == CFD_BC_REFERENCE && !c->uv_reads_pn) {
+        /* The reference's vm_n_tau keeps its previous values where the interior update does
+         * not write; the corner BC (imax, jmax+1) reads two of them. With ping-pong buffers
+         * the previous step's output is vm_tau, so those two cells are seeded from it (zero
+         * before the first step, as a freshly allocated array). After an exit without the
+         * tau update the buffer already holds the previous output. */
+        const int im = c->P.imax, jm = c->P.jmax;
+        const size_t k = (size_t)im * (jm + 2) + (jm - 2);
+        if (c->first_step) {
+            CHK(cfdk_fill(S[S_VMN] + k, 0.0, 2, c->s));
+        } else {
+            CHK(cfdk_copy(S[S_VMN] + k, S[S_VMT] + k, 2, c->s));
+        }
+    }
+    CHK(cfdk_zero_u64(slot, 3, c->s));
+    CHK(x_solve_u(&e, um, S[S_UMT], S[S_VMT], S[S_UMN], p_uv, T, X[X_UI], X[X_RU], X[X_RESU], slot + 0));
+    CHK(x_solve_v(&e, vm, S[S_UMT], S[S_VMT], S[S_VMN], p_uv, T, X[X_VI], X[X_RV], X[X_RESV], slot + 1));
+    CHK(x_solve_p(&e, S[S_P], S[S_UMN], S[S_VMN], S[S_PN], X[X_RP], X[X_PI], X[X_RESP], slot + 2));
+    CHK(x_solve_s(&e, 0, S[S_UMN], S[S_VMN], T, S[S_TN], S[S_TT], X[X_ZI], X[X_RZ]));
+    CHK(x_solve_s(&e, 1, S[S_UMN], S[S_VMN], C, S[S_CN], S[S_CT], X[X_CI], X[X_RC]));
+    c->bc_clean = 1;
+    c->first_step = 0;
+    return CFD_OK;
+}
+
+/* Fused step: boundary-consistent iterates in, outputs with unmaterialised boundaries. */
+static int step_fused(cfd_ctx *c, uint64_t *slot)
+{
+    const int st = c->P.steady;
+    double **S = c->st, **X = c->scr;
+    cfd_fused_args a;
+    memset(&a, 0, sizeof a);
+    a.um = st ? S[S_UMT] : S[S_UM];
+    a.vm = st ? S[S_VMT] : S[S_VM];
+    a.T = st ? S[S_TT] : S[S_T];
+    a.C = st ? S[S_CT] : S[S_C];
+    a.um_tau = S[S_UMT]; a.vm_tau = S[S_VMT];
+    a.p = c->uv_reads_pn ? S[S_PN] : S[S_P];
+    a.T_tau = S[S_TT]; a.C_tau = S[S_CT];
+    a.um_n = S[S_UMN]; a.vm_n = S[S_VMN]; a.pn = S[S_PN]; a.T_n = S[S_TN]; a.C_n = S[S_CN];
+    a.u1 = X[X_U1]; a.u2 = X[X_U2]; a.v1 = X[X_V1]; a.v2 = X[X_V2];
+    a.z1 = X[X_Z1]; a.z2 = X[X_Z2]; a.c1 = X[X_C1]; a.c2 = X[X_C2];
+    a.res_max = slot;
+    CHK(cfdk_zero_u64(slot, 3, c->s));
+    CHK(cfdk_fused_step(&c->g, &c->fac, &c->P, &a, c->kernel, c->s));
+    c->bc_clean = 2;          /* boundaries of the outputs not stored */
+    c->pending_in_tau = 0;    /* outputs in the *_n buffers until the swap */
+    c->first_step = 0;
+    return CFD_OK;
+}
+
+static int materialize(cfd_ctx *c)
+{
+    if (c->bc_clean != 2) return CFD_OK;
+    /* The last fused step's outputs lack their boundary cells: in the *_tau buffers after
+     * the swap, in the *_n buffers after an exit. The step's input v iterate (the other
+     * buffer) supplies the pre-BC cells the reference corner (imax, jmax+1) reads. */
+    double **S = c->st;
+    if (c->pending_in_tau)
+        CHK(cfdk_materialize_bc(&c->g, &c->P, S[S_UMT], S[S_VMT], S[S_P], S[S_TT], S[S_CT], S[S_VMN], c->s));
+    else
+        CHK(cfdk_materialize_bc(&c->g, &c->P, S[S_UMN], S[S_VMN], S[S_PN], S[S_TN], S[S_CN], S[S_VMT], c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->bc_clean = 1;
+    return CFD_OK;
+}
+
+static int one_step(cfd_ctx *c, uint64_t *slot)
+{
+    if (c->kernel == 1 && c->have_fused && c->bc_clean && !c->uv_reads_pn && c->prec == CFD_PREC_FP64)
+        return step_fused(c, slot);
+    CHK(materialize(c));
+    return step_exact(c, slot);
+}
+
+static double u64_to_res(uint64_t b)
+{
+    double d;
+    memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+static int check_ready(cfd_ctx *c)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (!c->have_mesh) return fail(CFD_ERR_STATE, "no mesh: call cfd_ctx_set_mesh first");
+    return CFD_OK;
+}
+
+int cfd_pseudo_steps(cfd_ctx *c, int32_t nsteps, double res[3], double *history)
+{
+    CHK(check_ready(c));
+    if (nsteps < 0) return fail(CFD_ERR_ARG, "nsteps < 0");
+    if (nsteps == 0) return CFD_OK;
+    const int64_t nslots = history ? nsteps : 1;
+    CHK(ensure_hist(c, nslots));
+    HIPCHK(hipEventRecord(c->ev0, c->s));
+    for (int32_t k = 0; k < nsteps; ++k) {
+        uint64_t *slot = c->d_hist + 3 * (history ? k : 0);
+        CHK(one_step(c, slot));
+        tau_swap(c);
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->s));
+    HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, (size_t)nslots * 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms_per_step = (double)ms / nsteps;
+    const uint64_t *last = c->h_hist + 3 * (nslots - 1);
+    if (res) for (int q = 0; q < 3; ++q) res[q] = u64_to_res(last[q]);
+    if (history) for (int64_t k = 0; k < 3 * nslots; ++k) history[k] = u64_to_res(c->h_hist[k]);
+    for (int q = 0; q < 3; ++q)
+        if (!isfinite(u64_to_res(last[q]))) return fail(CFD_ERR_NONFINITE, "residual %d is not finite", q);
+    return CFD_OK;
+}
+
+int cfd_run_to_tol(cfd_ctx *c, double tol, int64_t itc_max, int32_t check_every,
+                   double diverge_factor, int64_t *iters, double res[3])
+{
+    CHK(check_ready(c));
+    if (check_every < 1) check_every = 1;
+    CHK(ensure_hist(c, check_every));
+    int64_t done = 0;
+    double best = INFINITY, r[3] = {0, 0, 0};
+    HIPCHK(hipEventRecord(c->ev0, c->s));
+    while (done < itc_max) {
+        int32_t n = (int32_t)((itc_max - done) < check_every ? (itc_max - done) : check_every);
+        for (int32_t k = 0; k < n; ++k) {
+            CHK(one_step(c, c->d_hist + 3 * k));
+            tau_swap(c);
+        }
+        HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, (size_t)n * 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (int32_t k = 0; k < n; ++k) {
+            for (int q = 0; q < 3; ++q) r[q] = u64_to_res(c->h_hist[3 * k + q]);
+            const double err = fmax(fmax(r[0], r[1]), r[2]);
+            const int64_t itc = done + k + 1;
+            if (!isfinite(r[0]) || !isfinite(r[1]) || !isfinite(r[2])) {
+                if (iters) *iters = itc;
+                if (res) memcpy(res, r, sizeof r);
+                return fail(CFD_ERR_NONFINITE, "non-finite residual at pseudo-step %lld", (long long)itc);
+            }
+            if (itc != 1 && err < tol) {
+                done += n;
+                HIPCHK(hipEventRecord(c->ev1, c->s));
+                HIPCHK(hipEventSynchronize(c->ev1));
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, c->ev0, c->ev1);
+                c->last_ms_per_step = (double)ms / (double)done;
+                if (iters) *iters = itc;
+                if (res) memcpy(res, r, sizeof r);
+                return CFD_OK;
+            }
+            if (err < best) best = err;
+            if (diverge_factor > 0 && err > diverge_factor * best) {
+                if (iters) *iters = itc;
+                if (res) memcpy(res, r, sizeof r);
+                return fail(CFD_ERR_DIVERGED, "residual %.3e exceeds %.1f x best %.3e at step %lld",
+                            err, diverge_factor, best, (long long)itc);
+            }
+        }
+        done += n;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->s));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms_per_step = done ? (double)ms / (double)done : 0.0;
+    if (iters) *iters = done;
+    if (res) memcpy(res, r, sizeof r);
+    return fail(CFD_ERR_NOCONV, "tolerance %.3e not reached in %lld pseudo-steps (last %.3e)", tol,
+                (long long)done, fmax(fmax(r[0], r[1]), r[2]));
+}
+
+/* main.f90:263-305, one pass of the physical-time loop body. */
+int cfd_physical_step(cfd_ctx *c, double eps, int64_t itc_max, int32_t first, int64_t *iters, double res[3])
+{
+    CHK(check_ready(c));
+    CHK(ensure_hist(c, 1));
+    int64_t itc = first ? 1 : 0, taken = 0;
+    double r[3] = {0, 0, 0};
+    while (itc < itc_max) {
+        CHK(one_step(c, c->d_hist));
+        ++taken;
+        HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (int q = 0; q < 3; ++q) r[q] = u64_to_res(c->h_hist[q]);
+        const double err = fmax(fmax(r[0], r[1]), r[2]);
+        if (!isfinite(r[0]) || !isfinite(r[1]) || !isfinite(r[2])) {
+            if (iters) *iters = taken;
+            return fail(CFD_ERR_NONFINITE, "non-finite residual in physical step");
+        }
+        if (itc != 1 && err < eps) {
+            /* exit without the tau update: U/V now read pn, P reads the old p */
+            CHK(materialize(c));
+            c->uv_reads_pn = 1;
+            double **S = c->st;
+            CHK(cfdk_copy(S[S_UM], S[S_UMN], (long)c->nu, c->s));
+            CHK(cfdk_copy(S[S_VM], S[S_VMN], (long)c->nv, c->s));
+            CHK(cfdk_copy(S[S_T], S[S_TN], (long)c->np, c->s));
+            CHK(cfdk_copy(S[S_C], S[S_CN], (long)c->np, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            if (iters) *iters = taken;
+            if (res) memcpy(res, r, sizeof r);
+            return CFD_OK;
+        }
+        tau_swap(c);
+        ++itc;
+    }
+    CHK(materialize(c));
+    double **S = c->st;
+    CHK(cfdk_copy(S[S_UM], S[S_UMT], (long)c->nu, c->s));
+    CHK(cfdk_copy(S[S_VM], S[S_VMT], (long)c->nv, c->s));
+    CHK(cfdk_copy(S[S_T], S[S_TT], (long)c->np, c->s));
+    CHK(cfdk_copy(S[S_C], S[S_CT], (long)c->np, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (iters) *iters = taken;
+    if (res) memcpy(res, r, sizeof r);
+    return CFD_OK;
+}
+
+int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays, double res[3])
+{
+    CHK(check_ready(c));
+    if (steps_per_graph < 2 || (steps_per_graph & 1)) return fail(CFD_ERR_ARG, "steps_per_graph must be even and >= 2");
+    if (replays < 1) return fail(CFD_ERR_ARG, "replays must be >= 1");
+    /* the first step after new inputs goes through the exact path, outside the graph */
+    if (!(c->bc_clean && c->kernel == 1 && c->have_fused && !c->uv_reads_pn && c->prec == CFD_PREC_FP64)) {
+        CHK(ensure_hist(c, 1));
+        CHK(one_step(c, c->d_hist));
+        tau_swap(c);
+    }
+    CHK(ensure_hist(c, 1));
+    hipGraph_t graph = NULL;
+    hipGraphExec_t exec = NULL;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    int r = CFD_OK;
+    for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) {
+        r = one_step(c, c->d_hist);
+        tau_swap(c);
+    }
+    hipError_t e = hipStreamEndCapture(c->s, &graph);
+    if (r != CFD_OK) { if (graph) hipGraphDestroy(graph); return r; }
+    if (e != hipSuccess) return fail(CFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, NULL, NULL, 0);
+    if (e != hipSuccess) { hipGraphDestroy(graph); return fail(CFD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+    HIPCHK(hipEventRecord(c->ev0, c->s));
+    for (int32_t q = 0; q < replays && e == hipSuccess; ++q) e = hipGraphLaunch(exec, c->s);
+    hipEventRecord(c->ev1, c->s);
+    hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s);
+    hipError_t e2 = hipStreamSynchronize(c->s);
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(CFD_ERR_HIP, "graph replay failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms_per_step = (double)ms / ((double)steps_per_graph * replays);
+    double rr[3];
+    for (int q = 0; q < 3; ++q) rr[q] = u64_to_res(c->h_hist[q]);
+    if (res) memcpy(res, rr, sizeof rr);
+    for (int q = 0; q < 3; ++q)
+        if (!isfinite(rr[q])) return fail(CFD_ERR_NONFINITE, "residual %d is not finite", q);
+    return CFD_OK;
+}
+
+int cfd_ctx_last_timing(const cfd_ctx *c, double *ms)
+{
+    if (!c || !ms) return fail(CFD_ERR_ARG, "NULL argument");
+    *ms = c->last_ms_per_step;
+    return CFD_OK;
+}
+
+/* ====================================================================================== */
+/* Legacy entry points (c_cuda/functions.h:7-21,36): a default context on the default     */
+/* stream, configured by cfd_legacy_init / cfd_legacy_set_mesh, field arrays owned by the */
+/* caller (device pointers in the C layout).                                              */
+/* ====================================================================================== */
+static cfd_ctx *g_legacy = NULL;
+
+int cfd_legacy_init(const cfd_params *params)
+{
+    if (g_legacy) { ctx_free(g_legacy); free(g_legacy); g_legacy = NULL; }
+    cfd_ctx *c = (cfd_ctx *)calloc(1, sizeof *c);
+    if (!c) return fail(CFD_ERR_ARG, "out of host memory");
+    int r = ctx_init(c, params, CFD_PREC_FP64, 0);
+    if (r != CFD_OK) { ctx_free(c); free(c); return r; }
+    g_legacy = c;
+    return CFD_OK;
+}
+
+int cfd_legacy_set_params(const cfd_params *params)
+{
+    if (!g_legacy) return fail(CFD_ERR_STATE, "legacy context not initialised (cfd_legacy_init)");
+    return cfd_ctx_set_params(g_legacy, params);
+}
+
+int cfd_legacy_set_mesh(const double *x, const double *y, const double *eps, const double *lp)
+{
+    if (!g_legacy) return fail(CFD_ERR_STATE, "legacy context not initialised (cfd_legacy_init)");
+    CHK(cfd_ctx_set_mesh(g_legacy, x, y));
+    if (eps) CHK(cfd_ctx_upload(g_legacy, CFD_F_EPS, eps, CFD_LAYOUT_C));
+    if (lp) CHK(cfd_ctx_upload(g_legacy, CFD_F_LP, lp, CFD_LAYOUT_C));
+    return CFD_OK;
+}
+
+cfd_ctx *cfd_legacy_context(void) { return g_legacy; }
+
+void alocar_globais(void)
+{
+    if (!g_legacy) {
+        cfd_params p;
+        cfd_params_default(&p, 51, 84);
+        if (cfd_legacy_init(&p) != CFD_OK) return;
+    }
+}
+
+void desalocar_globais(void)
+{
+    if (g_legacy) { hipDeviceSynchronize(); ctx_free(g_legacy); free(g_legacy); g_legacy = NULL; }
+}
+
+#define LEGACY_CTX(ret)                                                                  \
+    cfd_ctx *c = g_legacy;                                                               \
+    if (!c) { fail(CFD_ERR_STATE, "legacy context not initialised (cfd_legacy_init)"); return ret; } \
+    xenv e = {&c->g, &c->P, c->s};                                                       \
+    (void)e;
+
+static double legacy_read(cfd_ctx *c, uint64_t *slot)
+{
+    uint64_t b = 0;
+    if (hipMemcpy(&b, slot, sizeof b, hipMemcpyDeviceToHost) != hipSuccess) {
+        fail(CFD_ERR_HIP, "residual read-back failed");
+        return NAN;
+    }
+    (void)c;
+    return u64_to_res(b);
+}
+
+void RESU(double *dev_um, double *dev_vm, double *dev_p, double *dev_ru)
+{
+    LEGACY_CTX();
+    cfdk_resu(&c->g, &c->P, 0, dev_um, dev_vm, dev_p, NULL, dev_ru, c->s);
+}
+
+void RESV(double *dev_um, double *dev_vm, double *dev_p, double *dev_t, double *dev_rv)
+{
+    LEGACY_CTX();
+    cfdk_resv(&c->g, &c->P, 0, dev_um, dev_vm, dev_p, dev_t, dev_rv, c->s);
+}
+
+void RESZ(double *dev_um_n, double *dev_vm_n, double *dev_z, double *dev_rz)
+{
+    LEGACY_CTX();
+    cfdk_ressc(&c->g, &c->P, 0, dev_um_n, dev_vm_n, dev_z, dev_rz, c->s);
+}
+
+void RESC(double *dev_um_n, double *dev_vm_n, double *dev_c, double *dev_rc)
+{
+    LEGACY_CTX();
+    cfdk_ressc(&c->g, &c->P, 1, dev_um_n, dev_vm_n, dev_c, dev_rc, c->s);
+}
+
+void bcUV(double *dev_um, double *dev_vm)
+{
+    LEGACY_CTX();
+    cfdk_bc_uv(&c->g, &c->P, dev_um, dev_vm, c->s);
+}
+
+void bcP(double *dev_pn) { LEGACY_CTX(); cfdk_bc_s(&c->g, &c->P, 0, dev_pn, c->s); }
+void bcZ(double *dev_zt) { LEGACY_CTX(); cfdk_bc_s(&c->g, &c->P, 1, dev_zt, c->s); }
+void bcC(double *dev_cn) { LEGACY_CTX(); cfdk_bc_s(&c->g, &c->P, 2, dev_cn, c->s); }
+
+double solve_U(double *dev_um, double *dev_vm, double *dev_um_n, double *dev_um_tau, double *dev_vm_tau,
+               double *dev_um_n_tau, double *dev_p, double *dev_ui, double *dev_ru, double *dev_res_u)
+{
+    LEGACY_CTX(NAN);
+    (void)dev_vm; (void)dev_um_n;
+    uint64_t *slot = c->d_hist;
+    if (cfdk_zero_u64(slot, 1, c->s) != CFD_OK) return NAN;
+    if (x_solve_u(&e, dev_um, dev_um_tau, dev_vm_tau, dev_um_n_tau, dev_p, NULL, dev_ui, dev_ru, dev_res_u, slot) != CFD_OK)
+        return NAN;
+    return legacy_read(c, slot);
+}
+
+double solve_V(double *dev_um, double *dev_vm, double *dev_vm_n, double *dev_um_tau, double *dev_vm_tau,
+               double *dev_vm_n_tau, double *dev_p, double *dev_t, double *dev_vi, double *dev_rv, double *dev_res_v)
+{
+    LEGACY_CTX(NAN);
+    (void)dev_um; (void)dev_vm_n;
+    uint64_t *slot = c->d_hist + 1;
+    if (cfdk_zero_u64(slot, 1, c->s) != CFD_OK) return NAN;
+    if (x_solve_v(&e, dev_vm, dev_um_tau, dev_vm_tau, dev_vm_n_tau, dev_p, dev_t, dev_vi, dev_rv, dev_res_v, slot) != CFD_OK)
+        return NAN;
+    return legacy_read(c, slot);
+}
+
+double solve_P(double *dev_p, double *dev_um_n, double *dev_vm_n, double *dev_pn)
+{
+    LEGACY_CTX(NAN);
+    uint64_t *slot = c->d_hist + 2;
+    if (cfdk_zero_u64(slot, 1, c->s) != CFD_OK) return NAN;
+    if (x_solve_p(&e, dev_p, dev_um_n, dev_vm_n, dev_pn, c->scr[X_RP], c->scr[X_PI], c->scr[X_RESP], slot) != CFD_OK)
+        return NAN;
+    return legacy_read(c, slot);
+}
+
+void solve_Z(double *dev_um_n, double *dev_vm_n, double *dev_z, double *dev_z_n_tau, double *dev_z_tau, double *dev_rz)
+{
+    LEGACY_CTX();
+    x_solve_s(&e, 0, dev_um_n, dev_vm_n, dev_z, dev_z_n_tau, dev_z_tau, c->scr[X_ZI], dev_rz);
+}
+
+void solve_C(double *dev_um_n, double *dev_vm_n, double *dev_c, double *dev_c_n_tau, double *dev_c_tau, double *dev_rc)
+{
+    LEGACY_CTX();
+    x_solve_s(&e, 1, dev_um_n, dev_vm_n, dev_c, dev_c_n_tau, dev_c_tau, c->scr[X_CI], dev_rc);
+}
+
+/* Strip-only residuals: the upwind rows and columns of RESU / RESV. */
+void upwind_U_pair(double *dev_um, double *dev_vm, double *dev_p, double *dev_ru)
+{
+    LEGACY_CTX();
+    cfdk_resu(&c->g, &c->P, 1, dev_um, dev_vm, dev_p, NULL, dev_ru, c->s);
+}
+
+void upwind_V_pair(double *dev_um, double *dev_vm, double *dev_p, double *dev_t, double *dev_rv)
+{
+    LEGACY_CTX();
+    cfdk_resv(&c->g, &c->P, 1, dev_um, dev_vm, dev_p, dev_t, dev_rv, c->s);
+}
+
+/* max_reduce.cu:14-41: max |a| over the first imax*jmax linear entries. */
+double max_reduce(double *dev_matriz_linearizada, int n_i, int n_j)
+{
+    LEGACY_CTX(NAN);
+    uint64_t *slot = c->d_hist;
+    if (cfdk_zero_u64(slot, 1, c->s) != CFD_OK) return NAN;
+    if (cfdk_maxabs_linear(dev_matriz_linearizada, (long)n_i * n_j, slot, c->s) != CFD_OK) return NAN;
+    return legacy_read(c, slot);
+}

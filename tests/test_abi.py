@@ -1,0 +1,104 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares; host-only helpers
+(mesh, geometry, defaults, argument checks) behave. No GPU compute here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cfd_amd import lib as L
+from cfd_amd import cases as C
+from cfd_amd.params import CfdParams, make_params, CF_REFERENCE
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt))
+    return {n for n in names if n not in {"if", "sizeof", "defined"}}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return L.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    for header in ("cfd.h", "cfd_legacy.h"):
+        for name in sorted(_declared(header)):
+            assert hasattr(lib, name), "%s declares %s but the library does not export it" % (header, name)
+    assert set(L.EXPORTS_CFD) <= _declared("cfd.h")
+    assert set(L.EXPORTS_LEGACY) <= _declared("cfd_legacy.h")
+
+
+def test_params_struct_layout_matches_header(lib):
+    p = CfdParams()
+    lib.cfd_params_default(ctypes.byref(p), 51, 84)
+    ref = make_params(51, 84)
+    for name, _ in CfdParams._fields_:
+        a, b = getattr(p, name), getattr(ref, name)
+        if name == "mu_coef":
+            assert list(a) == list(b)
+        else:
+            assert a == b, name
+    assert p.cf == CF_REFERENCE
+
+
+def test_mesh_1d_matches_python_builder(lib):
+    for n, P, Q in ((33, 1.0, 1.0), (51, 0.0, 2.0), (40, 0.6, 1.8)):
+        x = np.zeros(n + 1)
+        assert lib.cfd_mesh_1d(n, 0.0, 1.0, P, Q, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+        ref = C.tanh_nodes(n, 0.0, 1.0, P, Q) if P != 1.0 else C.uniform_nodes(n)
+        assert np.allclose(x[1:], ref[1:], rtol=0, atol=1e-15)
+        assert np.all(np.diff(x[1:]) > 0)
+
+
+def test_mesh_1d_matches_reference_formula_on_centred_domain(lib):
+    """Against the oracle's literal restatement of nonsymetric_mesh.f90:7-29 (itself pinned
+    to the reference's mesh routine by test_oracle_golden)."""
+    import oracle as O
+    O.build()
+    n, Lx = 51, 12.0
+    x = np.zeros(n + 1)
+    lib.cfd_mesh_1d(n, -Lx / 2, Lx / 2, 0.6, 1.8, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    ref = O.mesh_ref_x(n, Lx, 0.6, 1.8)
+    assert np.allclose(x[1:], ref[1:], rtol=0, atol=1e-13)
+
+
+def test_geometry_derive_matches_reference_mesh(lib):
+    from helpers import load_golden
+    prm, _, _, _, _, _, out, meta = load_golden("mesh51")
+    im, jm = 51, 84
+    x, y = out["x"], out["y"]
+    arrs = dict(xm=np.zeros(im + 2), ym=np.zeros(jm + 2), aun=np.zeros(im + 2), aus=np.zeros(im + 2),
+                aue=np.zeros(jm + 1), auw=np.zeros(jm + 1), avn=np.zeros(im + 1), avs=np.zeros(im + 1),
+                ave=np.zeros(jm + 2), avw=np.zeros(jm + 2))
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.cfd_geometry_derive(im, jm, dp(x), dp(y), *[dp(arrs[k]) for k in
+                                   ("xm", "ym", "aun", "aus", "aue", "auw", "avn", "avs", "ave", "avw")]) == 0
+    for k in ("xm", "ym", "aun", "aue", "avn", "ave"):
+        assert np.array_equal(arrs[k][1:], out[k][1:]), k
+
+
+def test_version_and_error_text(lib):
+    assert b"gfx950" in lib.cfd_version()
+    assert lib.cfd_mesh_1d(2, 0.0, 1.0, 1.0, 1.0, None) == -1
+    assert "cfd_mesh_1d" in L.last_error()
+
+
+def test_fortran_binding_module_compiles():
+    """The iso_c_binding module compiles with the image's flang (when present)."""
+    import shutil
+    import subprocess
+    import tempfile
+    fc = "/opt/rocm/lib/llvm/bin/flang"
+    if not os.path.exists(fc):
+        pytest.skip("no Fortran compiler in this image")
+    src = os.path.join(ROOT, "solver-cfd_amd", "fortran", "cfd_amd.f90")
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([fc, "-c", src, "-module-dir", d, "-o", os.path.join(d, "m.o")],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

@@ -1,0 +1,244 @@
+"""GPU parity: the HIP library against the reference's golden vectors and the CPU oracle.
+
+Bar: the drop-in (per-operator, reference call structure) path computes with one IEEE
+rounding per operation in the reference's order, so it is held to 1e-13 relative per array
+(bit equality is reported); the north-star bar for the FP64 drop-in is 1e-10 max relative
+difference per field after 2000 pseudo-steps.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cfd_amd import lib as L
+from cfd_amd import cases as C
+from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, COMPAT_CUDA, BUOY_BOUSSINESQ, MU_POLY4,
+                            LAYOUT_FORTRAN)
+from helpers import load_golden, max_rel, field_rel, random_state, small_case
+from hipmem import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = 1e-13
+FIELDS_OUT = ("um_tau", "vm_tau", "p", "T_tau", "C_tau")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    O.build()
+    L.load()
+    yield
+
+
+# ---- legacy entry points (device pointers, default context) -----------------------------
+def test_legacy_entry_points_match_reference():
+    say("case: legacy entry points, ops24")
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    lib = L.load()
+    L.check(lib.cfd_legacy_init(ctypes.byref(prm)))
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    L.check(lib.cfd_legacy_set_mesh(dp(x), dp(y), dp(eps), dp(lp)))
+    try:
+        d = {k: DeviceArray(st[k]) for k in ("um", "vm", "p", "T", "C")}
+        ru = DeviceArray(np.zeros_like(st["um"])); rv = DeviceArray(np.zeros_like(st["vm"]))
+        rz = DeviceArray(np.zeros_like(st["T"])); rc = DeviceArray(np.zeros_like(st["C"]))
+        lib.RESU(d["um"].ptr, d["vm"].ptr, d["p"].ptr, ru.ptr)
+        lib.RESV(d["um"].ptr, d["vm"].ptr, d["p"].ptr, d["T"].ptr, rv.ptr)
+        lib.RESZ(d["um"].ptr, d["vm"].ptr, d["T"].ptr, rz.ptr)
+        lib.RESC(d["um"].ptr, d["vm"].ptr, d["C"].ptr, rc.ptr)
+        errs = {"ru": max_rel(ru.get(), out["ru"]), "rv": max_rel(rv.get(), out["rv"]),
+                "rz": max_rel(rz.get(), out["rz"]), "rc": max_rel(rc.get(), out["rc"])}
+        exact = {"ru": np.array_equal(ru.get(), out["ru"]), "rv": np.array_equal(rv.get(), out["rv"])}
+        say("operators", errs, "bit-exact", exact)
+        assert max(errs.values()) <= TIGHT
+        # strip-only entry points write exactly the reference's strips
+        ru2 = DeviceArray(np.zeros_like(st["um"]))
+        lib.upwind_U_pair(d["um"].ptr, d["vm"].ptr, d["p"].ptr, ru2.ptr)
+        g = ru2.get(); im, jm = prm.imax, prm.jmax
+        strip = np.zeros_like(g, dtype=bool)
+        strip[2:im + 1, 2] = strip[2:im + 1, jm - 1] = True
+        strip[2, 2:jm] = strip[im, 2:jm] = True
+        assert np.all(g[~strip] == 0.0)
+        assert max_rel(g[strip], out["ru"][strip]) <= TIGHT
+        # boundary conditions
+        u, v = DeviceArray(st["um"]), DeviceArray(st["vm"])
+        lib.bcUV(u.ptr, v.ptr)
+        assert max_rel(u.get(), out["bc_um"]) <= TIGHT and max_rel(v.get(), out["bc_vm"]) <= TIGHT
+        for fn, key, src in ((lib.bcP, "bc_p", "p"), (lib.bcZ, "bc_T", "T"), (lib.bcC, "bc_C", "C")):
+            a = DeviceArray(st[src]); fn(a.ptr)
+            assert np.array_equal(a.get(), out[key]), key
+        # the five solves in driver order, with the bcUV side effects on the tau levels
+        umt, vmt = DeviceArray(st["um"]), DeviceArray(st["vm"])
+        umn, vmn = DeviceArray(np.zeros_like(st["um"])), DeviceArray(np.zeros_like(st["vm"]))
+        pn = DeviceArray(st["p"])
+        ui, ru3, resu = (DeviceArray(np.zeros_like(st["um"])) for _ in range(3))
+        vi, rv3, resv = (DeviceArray(np.zeros_like(st["vm"])) for _ in range(3))
+        r = lib.solve_U(d["um"].ptr, d["vm"].ptr, None, umt.ptr, vmt.ptr, umn.ptr, pn.ptr, ui.ptr, ru3.ptr, resu.ptr)
+        assert abs(r - out["su_res"][0]) <= TIGHT * abs(out["su_res"][0])
+        assert max_rel(umn.get(), out["su_um_n_tau"]) <= TIGHT
+        assert max_rel(vmt.get(), out["su_vm_tau"]) <= TIGHT
+        r = lib.solve_V(d["um"].ptr, d["vm"].ptr, None, umt.ptr, vmt.ptr, vmn.ptr, pn.ptr, d["T"].ptr, vi.ptr,
+                        rv3.ptr, resv.ptr)
+        assert abs(r - out["sv_res"][0]) <= TIGHT * abs(out["sv_res"][0])
+        assert max_rel(vmn.get(), out["sv_vm_n_tau"]) <= TIGHT
+        assert max_rel(umt.get(), out["sv_um_tau"]) <= TIGHT
+        pnew = DeviceArray(np.zeros_like(st["p"]))
+        r = lib.solve_P(d["p"].ptr, umn.ptr, vmn.ptr, pnew.ptr)
+        assert abs(r - out["sp_res"][0]) <= TIGHT * abs(out["sp_res"][0])
+        assert max_rel(pnew.get(), out["sp_pn"]) <= TIGHT
+        tn, cn = DeviceArray(np.zeros_like(st["T"])), DeviceArray(np.zeros_like(st["C"]))
+        rzz, rcc = DeviceArray(np.zeros_like(st["T"])), DeviceArray(np.zeros_like(st["C"]))
+        lib.solve_Z(umn.ptr, vmn.ptr, d["T"].ptr, tn.ptr, d["T"].ptr, rzz.ptr)
+        lib.solve_C(umn.ptr, vmn.ptr, d["C"].ptr, cn.ptr, d["C"].ptr, rcc.ptr)
+        assert max_rel(tn.get(), out["sz_T_n_tau"]) <= TIGHT
+        assert max_rel(cn.get(), out["sc_C_n_tau"]) <= TIGHT
+        # max_reduce: max |a| over the first n*m entries
+        a = np.linspace(-3.0, 2.0, 50)
+        da = DeviceArray(a)
+        assert lib.max_reduce(da.ptr, 5, 10) == 3.0
+        assert lib.max_reduce(da.ptr, 2, 5) == 3.0
+    finally:
+        lib.desalocar_globais()
+
+
+# ---- context API: trajectories against the reference ---------------------------------------
+def _ctx_from_golden(name, kernel=0):
+    prm, x, y, eps, lp, st, out, meta = load_golden(name)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_kernel(kernel)
+    ctx.set_state(st)
+    return ctx, prm, out, meta
+
+
+@pytest.mark.parametrize("name", ["steps24", "steady24", "c1ref51", "c2ref48"])
+def test_exact_path_trajectory_matches_reference(name):
+    say("case: exact-path trajectory", name)
+    ctx, prm, out, meta = _ctx_from_golden(name, kernel=0)
+    res, hist = ctx.pseudo_steps(meta["nsteps"], history=True)
+    h_err = max_rel(hist, out["hist"])
+    errs = {k: field_rel(ctx.download(k if k != "p" else "p"), out[k]) for k in FIELDS_OUT}
+    say(name, "history rel", h_err, "fields", errs)
+    assert h_err <= 1e-12
+    assert max(errs.values()) <= 1e-10
+    ctx.close()
+
+
+def test_physical_steps_match_reference():
+    say("case: physical steps phys24")
+    prm, x, y, eps, lp, st, out, meta = load_golden("phys24")
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_kernel(0)
+    ctx.set_state(st)
+    taken = [ctx.physical_step(meta["eps_conv"], meta["itc_max"], t == 0)[0] for t in range(meta["nphys"])]
+    assert taken == list(out["taken"].astype(int))
+    for k in ("um", "vm", "um_tau", "vm_tau", "p", "pn", "T", "T_tau", "C", "C_tau"):
+        assert field_rel(ctx.download(k), out[k]) <= 1e-10, k
+    ctx.close()
+
+
+# ---- configurations beyond the reference's BC set: against the CPU oracle -------------------
+def _compare_with_oracle(prm, x, y, eps, lp, st, nsteps, kernel=0, tol=1e-10):
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    h_ref = o.pseudo_steps(s, cs, nsteps)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_kernel(kernel)
+    ctx.set_state(st)
+    res, hist = ctx.pseudo_steps(nsteps, history=True)
+    errs = {k: field_rel(ctx.download(k), s[k]) for k in FIELDS_OUT}
+    h_err = max_rel(hist, h_ref)
+    ctx.close()
+    return h_err, errs
+
+
+def test_config1_channel_upwind_matches_oracle():
+    say("case: configuration 1 (channel BCs, global upwind), 2000 steps")
+    cfg = C.config_c1()
+    h_err, errs = _compare_with_oracle(cfg["params"], cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], cfg["state"], 2000)
+    say("c1", h_err, errs)
+    assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
+
+
+def test_boussinesq_mu_species_slot_match_oracle():
+    say("case: channel + Boussinesq + mu(T) + slot, 48x40, 300 steps")
+    n, m = 48, 40
+    x = C.tanh_nodes(n, 0.0, 1.0, 0.0, 2.0); y = C.tanh_nodes(m, 0.0, 1.0, 0.0, 2.0)
+    eps, lp = C.porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
+    prm = make_params(n, m, bc=BC_CHANNEL, re=200.0, beta=1.0, g=0.0, darcy=1e-3, cf=0.55,
+                      buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, sc=0.7, pe=140.0,
+                      slot_lo=0.45, slot_hi=0.55, mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3))
+    st = C.perturbed_state(n, m, seed=3, amp=1e-3, T0=0.5)
+    h_err, errs = _compare_with_oracle(prm, x, y, eps, lp, st, 300)
+    say("c3-physics", h_err, errs)
+    assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
+
+
+def test_cuda_compat_matches_oracle():
+    say("case: CUDA compat semantics, 24x20, 100 steps")
+    prm, x, y, eps, lp = small_case(compat=COMPAT_CUDA, b_art=0.0)
+    st = C.perturbed_state(24, 20, seed=31, amp=1e-2, p0=1.0, T0=1.0, C0=0.5, perturb_scalars=True)
+    h_err, errs = _compare_with_oracle(prm, x, y, eps, lp, st, 100)
+    assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
+
+
+# ---- boundary layouts and failure handling ----------------------------------------------------
+def test_fortran_layout_round_trip():
+    prm, x, y, eps, lp = small_case()
+    st = random_state(24, 20, seed=4)
+    ctx = L.Context(prm, x, y, eps, lp)
+    f = C.to_fortran(st["um"], "u", 24, 20).ravel(order="F")
+    ctx.upload("um", f, layout=LAYOUT_FORTRAN)
+    assert np.array_equal(ctx.download("um")[1:, 1:], st["um"][1:, 1:])
+    back = ctx.download("um", layout=LAYOUT_FORTRAN)
+    assert np.array_equal(back, f)
+    ctx.close()
+
+
+def test_nonfinite_residual_is_an_error():
+    prm, x, y, eps, lp = small_case()
+    st = C.perturbed_state(24, 20, seed=5, amp=1e-2, p0=1.0, T0=1.0, C0=0.5)
+    st["um"][10, 10] = np.nan
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_state(st)
+    with pytest.raises(L.CfdError) as e:
+        ctx.pseudo_steps(3)
+    assert e.value.code == -3
+    ctx.close()
+
+
+def test_itc_max_exhaustion_and_divergence_guard():
+    prm, x, y, eps, lp = small_case()
+    st = C.perturbed_state(24, 20, seed=6, amp=1e-2, p0=1.0, T0=1.0, C0=0.5)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_state(st)
+    code, it, res = ctx.run_to_tol(1e-30, 40, check_every=10, raise_on_fail=False)
+    assert code == -4 and it == 40
+    # a pseudo-step 100x too large blows up: the guard stops it as diverged or non-finite
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "dtau")}
+    ctx.set_params(make_params(24, 20, dtau=5.0, **d))
+    code, it, res = ctx.run_to_tol(1e-12, 4000, check_every=50, diverge_factor=1e3, raise_on_fail=False)
+    assert code in (-5, -3)
+    ctx.close()
+
+
+def test_run_to_tol_matches_reference_exit_step():
+    """cfd_run_to_tol with check_every=1 stops at the first step with err < tol (and
+    itc != 1), as main.f90:289 does; the oracle gives the same step."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("steps24")
+    tol = float(np.max(out["hist"], axis=1)[150])
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    n_ref = o.physical_step(s, cs, tol, 10 ** 6, True)[0]
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_kernel(0)
+    ctx.set_state(st)
+    code, it, res = ctx.run_to_tol(tol, 10 ** 6, check_every=1)
+    assert it == n_ref
+    ctx.close()

@@ -1,0 +1,255 @@
+"""The CPU oracle against the reference's own Fortran (golden vectors) and closed forms.
+
+The golden vectors of tests/golden/ were produced by the flang-built OpenMp_Multicore hot
+path (tests/golden/make_golden.py). Both sides compute with one IEEE rounding per
+operation in the same order, so the oracle must match them exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cfd_amd import cases as C
+from cfd_amd.params import make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4
+from helpers import load_golden, small_case, random_state
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    O.build()
+
+
+def _oracle(name):
+    prm, x, y, eps, lp, st, out, meta = load_golden(name)
+    return O.Oracle(prm, x, y, eps, lp), prm, st, out, meta
+
+
+def test_operators_match_reference_exactly():
+    o, prm, st, out, _ = _oracle("ops24")
+    um, vm, p, T, Cc = (st[k] for k in ("um", "vm", "p", "T", "C"))
+    assert np.array_equal(o.resu(um, vm, p, T), out["ru"])
+    assert np.array_equal(o.resv(um, vm, p, T), out["rv"])
+    assert np.array_equal(o.resz(um, vm, T), out["rz"])
+    assert np.array_equal(o.resc(um, vm, Cc), out["rc"])
+    assert np.array_equal(o.upwind_rows_u(um, vm, p, T), out["ru_up"])
+    assert np.array_equal(o.upwind_rows_v(um, vm, p, T), out["rv_up"])
+
+
+def test_boundary_conditions_match_reference_exactly():
+    o, prm, st, out, _ = _oracle("ops24")
+    u, v = st["um"].copy(), st["vm"].copy()
+    o.bcuv(u, v)
+    assert np.array_equal(u, out["bc_um"]) and np.array_equal(v, out["bc_vm"])
+    for fn, key, src in ((o.bcp, "bc_p", "p"), (o.bcz, "bc_T", "T"), (o.bcc, "bc_C", "C")):
+        a = st[src].copy()
+        fn(a)
+        assert np.array_equal(a, out[key]), key
+
+
+def test_solves_match_reference_exactly():
+    o, prm, st, out, _ = _oracle("ops24")
+    s, _cs = o.make_state(st)
+    r = o.solve_u(s["um"], s["um_tau"], s["vm_tau"], s["um_n_tau"], s["pn"], s["T"])
+    assert r == out["su_res"][0]
+    assert np.array_equal(s["um_n_tau"], out["su_um_n_tau"])
+    assert np.array_equal(s["vm_tau"], out["su_vm_tau"])          # bcUV side effect
+    r = o.solve_v(s["vm"], s["um_tau"], s["vm_tau"], s["vm_n_tau"], s["pn"], s["T"])
+    assert r == out["sv_res"][0]
+    assert np.array_equal(s["vm_n_tau"], out["sv_vm_n_tau"])
+    assert np.array_equal(s["um_tau"], out["sv_um_tau"])
+    r = o.solve_p(s["p"], s["um_n_tau"], s["vm_n_tau"], s["pn"])
+    assert r == out["sp_res"][0] and np.array_equal(s["pn"], out["sp_pn"])
+    o.solve_z(s["um_n_tau"], s["vm_n_tau"], s["T"], s["T_n_tau"], s["T_tau"])
+    assert np.array_equal(s["T_n_tau"], out["sz_T_n_tau"])
+    o.solve_c(s["um_n_tau"], s["vm_n_tau"], s["C"], s["C_n_tau"], s["C_tau"])
+    assert np.array_equal(s["C_n_tau"], out["sc_C_n_tau"])
+
+
+@pytest.mark.parametrize("name", ["steps24", "steady24", "c1ref51", "c2ref48"])
+def test_trajectories_match_reference_exactly(name):
+    o, prm, st, out, meta = _oracle(name)
+    s, cs = o.make_state(st)
+    h = o.pseudo_steps(s, cs, meta["nsteps"])
+    assert np.array_equal(h, out["hist"])
+    for k in ("um_tau", "vm_tau", "p", "T_tau", "C_tau"):
+        assert np.array_equal(s[k], out[k]), k
+
+
+def test_physical_steps_match_reference_exactly():
+    o, prm, st, out, meta = _oracle("phys24")
+    s, cs = o.make_state(st)
+    taken = [o.physical_step(s, cs, meta["eps_conv"], meta["itc_max"], t == 0)[0] for t in range(meta["nphys"])]
+    assert taken == list(out["taken"].astype(int))
+    assert min(taken) < meta["itc_max"] - 1      # the converged-exit path is exercised
+    for k in ("um", "vm", "um_tau", "vm_tau", "p", "pn", "T", "T_tau", "C", "C_tau"):
+        assert np.array_equal(s[k], out[k]), k
+
+
+def test_mesh_matches_reference_exactly():
+    prm, x, y, eps, lp, st, out, meta = load_golden("mesh51")
+    X = O.mesh_ref_x(51, meta["Lhori"], meta["Px"], meta["Q"])
+    assert np.array_equal(X[1:], out["x"][1:])
+    # staggered midpoints and areas from the reference's own x, y
+    o = O.Oracle(prm, out["x"], out["y"], out["eps"], out["lp"])
+    g = o.geometry()
+    for k in ("xm", "ym", "aun", "aue", "avn", "ave"):
+        assert np.array_equal(g[k][1:], out[k][1:]), k
+
+
+# ---- closed forms (App. B of the formulation; independent of the golden vectors) -------
+def _uniform(n=12, m=10, **kw):
+    x = C.uniform_nodes(n, 0.0, 1.1); y = C.uniform_nodes(m, 0.0, 0.9)
+    eps = C.zeros("p", n, m); eps[1:, 1:] = 1.0
+    lp = C.zeros("p", n, m)
+    prm = make_params(n, m, **kw)
+    return O.Oracle(prm, x, y, eps, lp), x, y
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_uniform_flow_has_zero_momentum_residual(scheme):
+    """u = U, v = V constant, p constant, no body force, no porosity: every convective
+    and diffusive flux balances, so RU = RV = 0 exactly up to rounding."""
+    o, x, y = _uniform(scheme=scheme, g=0.0, invfr2=0.0, re=7.0)
+    n, m = 12, 10
+    um = np.full(C.shape_of("u", n, m), 0.8); vm = np.full(C.shape_of("v", n, m), -0.3)
+    p = np.full(C.shape_of("p", n, m), 2.0); T = np.full(C.shape_of("p", n, m), 1.0)
+    assert np.abs(o.resu(um, vm, p, T)).max() < 1e-12
+    assert np.abs(o.resv(um, vm, p, T)).max() < 1e-12
+
+
+def test_linear_pressure_gives_exact_gradient():
+    """u = v = 0, p = a x + b y: RU = -a (cell eps = 1), RV = -b, from the q_art term."""
+    o, x, y = _uniform(g=0.0, invfr2=0.0)
+    n, m = 12, 10
+    a, b = 0.7, -1.9
+    p = np.zeros(C.shape_of("p", n, m))
+    p[1:, 1:] = a * x[1:, None] + b * y[None, 1:]
+    um = np.zeros(C.shape_of("u", n, m)); vm = np.zeros(C.shape_of("v", n, m))
+    T = np.ones(C.shape_of("p", n, m))
+    ru = o.resu(um, vm, p, T); rv = o.resv(um, vm, p, T)
+    assert np.allclose(ru[2:n + 1, 2:m], -a, rtol=0, atol=1e-12)
+    assert np.allclose(rv[2:n, 2:m + 1], -b, rtol=0, atol=1e-12)
+
+
+def test_body_force_and_buoyancy_forms():
+    """Uniform state: RU = -g eps; RV = invfr2 (1 - 1/T) (reference) or Ri (T - Tref)."""
+    o, x, y = _uniform(g=2.5, invfr2=0.64)
+    n, m = 12, 10
+    um = np.zeros(C.shape_of("u", n, m)); vm = np.zeros(C.shape_of("v", n, m))
+    p = np.zeros(C.shape_of("p", n, m)); T = np.full(C.shape_of("p", n, m), 1.25)
+    assert np.allclose(o.resu(um, vm, p, T)[2:n + 1, 2:m], -2.5, atol=1e-12)
+    assert np.allclose(o.resv(um, vm, p, T)[2:n, 2:m + 1], 0.64 * (1 - 1 / 1.25), atol=1e-12)
+    o2, _, _ = _uniform(g=0.0, buoyancy=BUOY_BOUSSINESQ, ri=3.0, t_ref=0.5)
+    assert np.allclose(o2.resv(um, vm, p, T)[2:n, 2:m + 1], 3.0 * (1.25 - 0.5), atol=1e-12)
+
+
+def test_quadratic_scalar_diffusion_is_exact():
+    """u = v = 0, T = x^2 + 2 y^2 on a uniform grid: RZ = (2 + 4)/Pe at interior nodes."""
+    o, x, y = _uniform(pe=4.0)
+    n, m = 12, 10
+    T = np.zeros(C.shape_of("p", n, m)); T[1:, 1:] = x[1:, None] ** 2 + 2 * y[None, 1:] ** 2
+    um = np.zeros(C.shape_of("u", n, m)); vm = np.zeros(C.shape_of("v", n, m))
+    rz = o.resz(um, vm, T)
+    assert np.allclose(rz[2:n, 2:m], 6.0 / 4.0, rtol=1e-10)
+
+
+def test_continuity_telescopes():
+    """solve_P: sum of RP over the interior equals the net boundary flux, and the three
+    RK stages give pn = p + dtau*beta*RP up to rounding (equations.f90:620-658)."""
+    o, x, y = _uniform(beta=1.7, dtau=0.05)
+    n, m = 12, 10
+    st = random_state(n, m, seed=3)
+    pn = np.zeros_like(st["p"])
+    o.solve_p(st["p"], st["um"], st["vm"], pn)
+    g = o.geometry()
+    # interior rp from pn
+    rp = (pn[2:n, 2:m] - st["p"][2:n, 2:m]) / (0.05 * 1.7)
+    um, vm = st["um"], st["vm"]
+    inflow = (um[2, 2:m] * g["aue"][2:m]).sum() - (um[n, 2:m] * g["aue"][2:m]).sum() \
+        + (vm[2:n, 2] * g["avn"][2:n]).sum() - (vm[2:n, m] * g["avn"][2:n]).sum()
+    assert abs(rp.sum() - inflow) < 1e-10
+
+
+def test_rk3_of_zero_residual_is_identity():
+    """R = 0 and phi^n = phi^tau: the three stages return phi^tau unchanged."""
+    o, x, y = _uniform(g=0.0, invfr2=0.0)
+    n, m = 12, 10
+    T = np.full(C.shape_of("p", n, m), 1.0)
+    um = np.zeros(C.shape_of("u", n, m)); vm = np.zeros(C.shape_of("v", n, m))
+    out = np.zeros_like(T)
+    o.solve_z(um, vm, T, out, T)
+    assert np.array_equal(out[1:, 1:], T[1:, 1:])
+
+
+def test_channel_bc_relations():
+    """The channel set: inflow u_in, zero-gradient outflow, no-slip walls, p = 0 at the
+    outlet, Dirichlet wall temperatures, species slot on the left wall."""
+    n, m = 14, 12
+    prm, x, y, eps, lp = small_case(n, m, bc=BC_CHANNEL, u_in=1.3, t_bottom=2.0, t_top=-1.0,
+                                    slot_lo=0.3, slot_hi=0.6, slot_value=0.9)
+    o = O.Oracle(prm, x, y, eps, lp)
+    st = random_state(n, m, seed=9)
+    u, v = st["um"].copy(), st["vm"].copy()
+    o.bcuv(u, v)
+    assert np.all(u[1:3, 1:m + 1] == 1.3)
+    assert np.all(u[3:n, 1] == 0) and np.all(u[3:n, m] == 0)
+    assert np.array_equal(u[n, 2:m], u[n - 1, 2:m]) and np.array_equal(u[n + 1, 2:m], u[n, 2:m])
+    assert np.all(v[1:n + 1, 1:3] == 0) and np.all(v[2:n + 1, m:m + 2] == 0) and np.all(v[1, 1:] == 0)
+    assert np.array_equal(v[n, 3:m], v[n - 1, 3:m])
+    p = st["p"].copy(); o.bcp(p)
+    assert np.all(p[n, 1:] == 0) and np.array_equal(p[1, 2:m], p[2, 2:m])
+    T = st["T"].copy(); o.bcz(T)
+    assert np.all(T[1:n + 1, 1] == 2.0) and np.all(T[1:n + 1, m] == -1.0)
+    Cc = st["C"].copy(); o.bcc(Cc)
+    inslot = (y[1:] >= 0.3) & (y[1:] <= 0.6)
+    assert np.array_equal(Cc[1, 1:], np.where(inslot, 0.9, 0.0))
+    # idempotent: applying twice changes nothing
+    u2, v2 = u.copy(), v.copy(); o.bcuv(u2, v2)
+    assert np.array_equal(u, u2) and np.array_equal(v, v2)
+
+
+def test_global_upwind_equals_reference_rows():
+    """CFD_SCHEME_UPWIND is the reference's upwind row operator on every row."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    p2 = make_params(prm.imax, prm.jmax, **{k: getattr(prm, k) for k, _ in type(prm)._fields_
+                                           if k not in ("imax", "jmax", "scheme", "mu_coef")},
+                     scheme=SCHEME_UPWIND)
+    o = O.Oracle(p2, x, y, eps, lp)
+    um, vm, pp, T = (st[k] for k in ("um", "vm", "p", "T"))
+    assert np.array_equal(o.resu(um, vm, pp, T), out["ru_up"])
+    assert np.array_equal(o.resv(um, vm, pp, T), out["rv_up"])
+
+
+def test_mu_poly_reduces_to_constant():
+    """mu(T) with c = (1, 0, 0, 0, 0) is the constant-viscosity reference operator; a
+    constant mu = 2 doubles the viscous coefficients exactly as Re/2 does."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    base = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "mu_model")}
+    um, vm, pp, T = (st[k] for k in ("um", "vm", "p", "T"))
+    o1 = O.Oracle(make_params(prm.imax, prm.jmax, mu_model=MU_POLY4, mu_coef=(1, 0, 0, 0, 0), **base), x, y, eps, lp)
+    assert np.array_equal(o1.resu(um, vm, pp, T), out["ru"])
+    assert np.array_equal(o1.resv(um, vm, pp, T), out["rv"])
+    b2 = dict(base); b2["re"] = base["re"] / 2
+    o2 = O.Oracle(make_params(prm.imax, prm.jmax, mu_model=MU_POLY4, mu_coef=(2, 0, 0, 0, 0), **base), x, y, eps, lp)
+    o3 = O.Oracle(make_params(prm.imax, prm.jmax, **b2), x, y, eps, lp)
+    # Darcy drag also uses Re, so compare with drag off
+    lp0 = np.zeros_like(lp)
+    o2 = O.Oracle(make_params(prm.imax, prm.jmax, mu_model=MU_POLY4, mu_coef=(2, 0, 0, 0, 0), **base), x, y, eps, lp0)
+    o3 = O.Oracle(make_params(prm.imax, prm.jmax, **b2), x, y, eps, lp0)
+    assert np.allclose(o2.resu(um, vm, pp, T), o3.resu(um, vm, pp, T), rtol=1e-13, atol=1e-13)
+
+
+def test_steady_mode_is_dual_time_with_tracking_level():
+    """steady=1 equals the dual-time step with the physical level set to the iterate."""
+    prm, x, y, eps, lp = small_case()
+    st = C.perturbed_state(24, 20, seed=21, amp=1e-2, p0=1.0, T0=1.0, C0=0.5, perturb_scalars=True)
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "steady")}
+    o1 = O.Oracle(make_params(24, 20, steady=1, **d), x, y, eps, lp)
+    o0 = O.Oracle(make_params(24, 20, steady=0, **d), x, y, eps, lp)
+    s1, c1 = o1.make_state(st)
+    s0, c0 = o0.make_state(st)
+    for _ in range(5):
+        o1.pseudo_steps(s1, c1, 1)
+        s0["um"][:] = s0["um_tau"]; s0["vm"][:] = s0["vm_tau"]; s0["T"][:] = s0["T_tau"]; s0["C"][:] = s0["C_tau"]
+        o0.pseudo_steps(s0, c0, 1)
+    for k in ("um_tau", "vm_tau", "p", "T_tau", "C_tau"):
+        assert np.array_equal(s1[k], s0[k]), k
