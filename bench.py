@@ -107,6 +107,7 @@ def main():
         steps = args.steps
     wall = time.perf_counter() - t0
     ms_event = ctx.last_ms_per_step()
+    n_fused, n_exact = ctx.step_counts()
     if not np.all(np.isfinite(res)):
         raise SystemExit("non-finite residual %s" % res)
     log("timed: %.3f s wall, %.4f ms/step (events), residuals %s" % (wall, ms_event, res))
@@ -128,7 +129,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 inputs of configuration 3: mesh, porosity, state)",
         "config": {"workload": "%s fused pseudo-step, %s" % (cfg["name"], args.precision),
                    "imax": prm.imax, "jmax": prm.jmax, "kernel": args.kernel,
-                   "graph_steps": args.graph, "bytes_per_cell": B},
+                   "graph_steps": args.graph, "bytes_per_cell": B,
+                   "steps_fused": n_fused, "steps_exact": n_exact},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernel_ms_per_step": ms_event,

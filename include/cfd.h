@@ -201,6 +201,11 @@ int  cfd_ctx_last_timing(const cfd_ctx *ctx, double *ms_per_step);
  * single-pass kernel). Mainly for tests that pin both against each other. */
 int  cfd_ctx_set_kernel(cfd_ctx *ctx, int32_t kernel);
 
+/* Pseudo-steps taken by each path since the context was created: fused kernels and the
+ * exact per-operator path (the latter runs the first step after new inputs, and every step
+ * with b_art != 0 or CFD_COMPAT_CUDA). */
+int  cfd_ctx_step_counts(const cfd_ctx *ctx, int64_t *fused, int64_t *exact);
+
 const char *cfd_last_error(void);
 const char *cfd_version(void);
 

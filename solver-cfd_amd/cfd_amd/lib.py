@@ -23,7 +23,8 @@ EXPORTS_CFD = [
     "cfd_params_default", "cfd_mesh_1d", "cfd_geometry_derive", "cfd_ctx_create", "cfd_ctx_destroy",
     "cfd_ctx_get_params", "cfd_ctx_set_params", "cfd_ctx_set_mesh", "cfd_ctx_upload", "cfd_ctx_download",
     "cfd_ctx_device_ptr", "cfd_ctx_sync_levels", "cfd_pseudo_steps", "cfd_run_to_tol", "cfd_physical_step",
-    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_last_error", "cfd_version",
+    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_ctx_step_counts",
+    "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
     "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
@@ -73,6 +74,7 @@ def load():
         "cfd_pseudo_steps_graph": (ctypes.c_int, [_vp, i32, i32, _dp]),
         "cfd_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
         "cfd_ctx_set_kernel": (ctypes.c_int, [_vp, i32]),
+        "cfd_ctx_step_counts": (ctypes.c_int, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -209,6 +211,12 @@ class Context:
         v = ctypes.c_double(0)
         check(load().cfd_ctx_last_timing(self.h, ctypes.byref(v)))
         return v.value
+
+    def step_counts(self):
+        """(fused, exact): pseudo-steps taken by each path so far."""
+        f, e = ctypes.c_int64(0), ctypes.c_int64(0)
+        check(load().cfd_ctx_step_counts(self.h, ctypes.byref(f), ctypes.byref(e)))
+        return f.value, e.value
 
     def device_ptr(self, name):
         p = _vp()

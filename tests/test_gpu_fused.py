@@ -1,0 +1,147 @@
+"""GPU: the fused row-streaming pseudo-step (K1 u/v/p, K2 T/C) against the reference's
+golden vectors, the CPU oracle and the exact per-operator path.
+
+The fused kernels evaluate QUICK in flux form with precomputed reciprocals and FMA, so they
+differ from the literal reference arithmetic by rounding: the bar is the north-star 1e-10
+max relative difference per field (after up to 2000 pseudo-steps).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cfd_amd import lib as L
+from cfd_amd import cases as C
+from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4)
+from helpers import load_golden, max_rel, field_rel
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10   # per field: max |a - b| / max |b|
+FIELDS = ("um_tau", "vm_tau", "p", "T_tau", "C_tau")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    O.build()
+    L.load()
+
+
+@pytest.mark.parametrize("name", ["c1ref51", "c2ref48", "steady24"])
+def test_fused_trajectory_matches_reference(name):
+    say("case: fused trajectory vs Fortran golden", name)
+    prm, x, y, eps, lp, st, out, meta = load_golden(name)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_state(st)
+    res, hist = ctx.pseudo_steps(meta["nsteps"], history=True)
+    nf, ne = ctx.step_counts()
+    errs = {k: max_rel(ctx.download(k), out[k]) for k in FIELDS}
+    h = max_rel(hist, out["hist"])
+    say(name, "fused steps", nf, "exact steps", ne, "history", h, "fields", errs)
+    assert nf == meta["nsteps"] - 1 and ne == 1
+    assert h <= TOL and max(errs.values()) <= TOL
+    ctx.close()
+
+
+def _vs_oracle(prm, x, y, eps, lp, st, n):
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    h_ref = o.pseudo_steps(s, cs, n)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_state(st)
+    res, hist = ctx.pseudo_steps(n, history=True)
+    nf, ne = ctx.step_counts()
+    errs = {k: max_rel(ctx.download(k), s[k]) for k in FIELDS}
+    ctx.close()
+    return max_rel(hist, h_ref), errs, nf
+
+
+def _multiblock_case(imax, jmax, bc, **kw):
+    x = C.tanh_nodes(imax, 0.0, 1.0, 0.3, 2.0)
+    y = C.tanh_nodes(jmax, 0.0, 1.0, 0.3, 2.0)
+    eps, lp = C.porous_block(x, y, 0.3, 0.65, 0.2, 0.7, 0.4, 0.9, seed=8)
+    nu = 1.0 / kw.get("re", 100.0)
+    dt = C.stable_dt(x, y, 2 * nu, 1.0 / kw.get("pe", 100.0), 1.5, 0.05)
+    d = dict(re=100.0, pe=100.0, sc=0.8, beta=1.0, darcy=1e-3, cf=0.55, g=0.0, dt=dt, bc=bc)
+    d.update(kw)
+    prm = make_params(imax, jmax, **d)
+    st = C.perturbed_state(imax, jmax, seed=9, amp=5e-2, u0=0.2, v0=0.05, p0=0.3, T0=1.1, C0=0.2,
+                           perturb_scalars=True)
+    return prm, x, y, eps, lp, st
+
+
+@pytest.mark.parametrize("shape", [(120, 600), (300, 260), (20, 1030)])
+def test_fused_multiblock_reference_bcs(shape):
+    """Several row and column blocks (block edges inside the domain), reference BC set,
+    reference buoyancy, QUICK."""
+    say("case: multiblock reference BCs", shape)
+    prm, x, y, eps, lp, st = _multiblock_case(*shape, bc=0, invfr2=0.7, g=1.5)
+    h, errs, nf = _vs_oracle(prm, x, y, eps, lp, st, 60)
+    say(shape, h, errs, nf)
+    assert nf == 59 and h <= TOL and max(errs.values()) <= TOL
+
+
+@pytest.mark.parametrize("variant", ["channel", "upwind", "mu_bouss_slot", "steady"])
+def test_fused_multiblock_channel_variants(variant):
+    say("case: multiblock channel", variant)
+    kw = dict()
+    if variant == "upwind":
+        kw = dict(scheme=SCHEME_UPWIND)
+    elif variant == "mu_bouss_slot":
+        kw = dict(buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, slot_lo=0.4, slot_hi=0.6, slot_value=1.0,
+                  mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3), t_bottom=1.0, t_top=0.0)
+    elif variant == "steady":
+        kw = dict(steady=1, buoyancy=BUOY_BOUSSINESQ, ri=1.0, t_ref=0.5)
+    prm, x, y, eps, lp, st = _multiblock_case(140, 520, bc=BC_CHANNEL, invfr2=0.0, **kw)
+    h, errs, nf = _vs_oracle(prm, x, y, eps, lp, st, 60)
+    say(variant, h, errs, nf)
+    assert nf == 59 and h <= TOL and max(errs.values()) <= TOL
+
+
+def test_fused_matches_exact_path_at_scale():
+    """Configuration 3 physics at 1024 x 512 (the second configuration-3 mesh): 40 steps of
+    the fused path against the exact per-operator path on the same GPU."""
+    say("case: c3 1024x512 fused vs exact, 40 steps")
+    cfg = C.config_c3(1024, 512)
+    outs = []
+    for kernel in (0, 1):
+        ctx = L.Context(cfg["params"], cfg["x"], cfg["y"], cfg["eps"], cfg["lp"])
+        ctx.set_kernel(kernel)
+        ctx.set_state(cfg["state"])
+        res, hist = ctx.pseudo_steps(40, history=True)
+        outs.append(({k: ctx.download(k) for k in FIELDS}, hist, ctx.step_counts()))
+        ctx.close()
+    (fe, he, ce), (ff, hf, cf) = outs
+    assert ce == (0, 40) and cf == (39, 1)
+    errs = {k: max_rel(ff[k], fe[k]) for k in FIELDS}
+    say("c3 1024x512", max_rel(hf, he), errs)
+    assert max_rel(hf, he) <= TOL and max(errs.values()) <= TOL
+
+
+def test_graph_replays_match_eager():
+    """A captured hipGraph of fused steps, replayed several times with host work and
+    synchronisation between replays, gives the same state as eager launches."""
+    say("case: hipGraph replay vs eager")
+    prm, x, y, eps, lp, st = _multiblock_case(100, 300, bc=BC_CHANNEL, invfr2=0.0)
+    ctxs = []
+    for mode in ("eager", "graph"):
+        ctx = L.Context(prm, x, y, eps, lp)
+        ctx.set_state(st)
+        ctx.pseudo_steps(1)                    # exact first step in both
+        if mode == "eager":
+            ctx.pseudo_steps(24)
+        else:
+            for _ in range(3):
+                ctx.pseudo_steps_graph(8, 1)
+                _ = ctx.download("p")          # eager work between replays
+        ctxs.append(ctx)
+    for k in FIELDS:
+        a, b = ctxs[0].download(k), ctxs[1].download(k)
+        assert np.array_equal(a, b), k
+    for c in ctxs:
+        c.close()
