@@ -7,10 +7,11 @@ A "step" is one pseudo-step (three SSP-RK3 stages of u, v, p, T, C with their bo
 conditions and the residual maxima) over the whole grid; one cell-update is one pseudo-step
 at one p-cell, so value = imax * jmax * steps / seconds.
 
-Prints one JSON line. `roofline` prices the dominant kernel against HBM (this path is
-memory-bound and uses no MFMA): achieved = algorithmic bytes per pseudo-step (B bytes per
-cell: every field read once, every output written once) / mean kernel time per step from
-HIP events on the context's stream. `cpu_baseline` times the CPU oracle (the C
+Prints one JSON line. `roofline` prices the dominant kernel (the fused u/v/p kernel) against
+HBM (this path uses no MFMA): achieved = its algorithmic bytes per launch (88 B per cell:
+every field it reads once, every output once, times imax*jmax cells) / its mean launch
+duration from HIP events recorded around it on the context's stream. `step_roofline` does
+the same for the whole pseudo-step (128 B per cell). `cpu_baseline` times the CPU oracle (the C
 restatement, OpenMP) on a bounded sample of the same workload on this host.
 """
 import argparse
@@ -30,11 +31,15 @@ from cfd_amd import cases as C  # noqa: E402
 from cfd_amd.params import PREC_FP64, PREC_FP32  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell-update (reads + writes, each field once)
+# Algorithmic bytes per cell (each field read once, each output written once; 1-D geometry
+# is negligible). Whole pseudo-step (FP64, full physics): reads um vm um_tau vm_tau p T T_tau
+# C C_tau eps lp (11 x 8) + writes um_n vm_n pn T_n C_n (5 x 8) = 128. Per kernel:
+#   u/v/p kernel: reads um vm um_tau vm_tau p T eps lp, writes um_n vm_n pn  -> 88
+#   T/C kernel:   reads um_n vm_n T T_tau C C_tau eps lp, writes T_n C_n     -> 80
+# (the kernels share um_n/vm_n/T/eps/lp, so the per-kernel sums exceed the step's 128).
 BYTES_PER_CELL = {
-    ("c3", PREC_FP64): 128,   # reads um vm um_tau vm_tau p T T_tau C C_tau eps lp, writes 5
-    ("c2", PREC_FP64): 128,
-    ("c3", PREC_FP32): 72,    # 10 FP32 reads + FP64 p, 4 FP32 writes + FP64 pn
+    ("step", PREC_FP64): 128, ("k1", PREC_FP64): 88, ("k2", PREC_FP64): 80,
+    ("step", PREC_FP32): 72,
 }
 
 
@@ -76,6 +81,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_c3_2048x2048_fp64.json"))
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in tile shapes")
+    ap.add_argument("--tiles", default="", help="hb1,hb2: set the row-block heights instead of autotuning")
+    ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with plain copy kernels of known size (memory-counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -94,6 +104,15 @@ def main():
     ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], precision=prec)
     ctx.set_kernel(args.kernel)
     ctx.set_state(cfg["state"])
+    tiles = None
+    if args.tiles:
+        hb1, hb2 = (int(v) for v in args.tiles.split(","))
+        ctx.set_tiles(hb1, hb2)
+        tiles = {"hb1": hb1, "hb2": hb2}
+    elif args.kernel == 1 and not args.no_autotune:
+        hb1, hb2, tms = ctx.autotune(steps=5)
+        tiles = {"hb1": hb1, "hb2": hb2}
+        log("autotune: row blocks u/v/p %d, T/C %d (0 = built-in), %.4f ms/step" % (hb1, hb2, tms))
     ctx.pseudo_steps(max(args.warmup, 1))
     log("warmup done (%d steps), timing %d steps" % (args.warmup, args.steps))
 
@@ -113,14 +132,35 @@ def main():
     log("timed: %.3f s wall, %.4f ms/step (events), residuals %s" % (wall, ms_event, res))
 
     value = cells * steps / wall
-    B = BYTES_PER_CELL.get((args.config, prec), 128)
-    achieved = B * cells / (ms_event * 1e-3) / 1e9
+    Bstep = BYTES_PER_CELL.get(("step", prec), 128)
+    step_gbs = Bstep * cells / (ms_event * 1e-3) / 1e9
+    kern = None
+    if args.kernel == 1 and args.kernel_steps > 0:
+        _, kms = ctx.pseudo_steps_timed(args.kernel_steps)
+        kern = {"uvp_ms": kms[0], "tc_ms": kms[1], "bc_ms": kms[2], "step_ms": kms[3]}
+        log("per kernel (events, mean of %d): u/v/p %.4f ms, T/C %.4f ms, boundary %.4f ms"
+            % (args.kernel_steps, kms[0], kms[1], kms[2]))
+    if kern is not None and prec == PREC_FP64:
+        # dominant kernel: the fused u/v/p kernel
+        B1 = BYTES_PER_CELL[("k1", prec)]
+        achieved = B1 * cells / (kern["uvp_ms"] * 1e-3) / 1e9
+        rname, rB, rms = "k_uvp (fused u/v/p pseudo-step kernel)", B1, kern["uvp_ms"]
+    else:
+        achieved, rname, rB, rms = step_gbs, "whole pseudo-step", Bstep, ms_event
     traffic = None
     if os.path.exists(args.traffic):
         with open(args.traffic) as f:
             t = json.load(f)
-        if t.get("imax") == prm.imax and t.get("jmax") == prm.jmax and t.get("kernel") == args.kernel:
-            traffic = t.get("bytes_per_step")
+        if (t.get("imax") == prm.imax and t.get("jmax") == prm.jmax and t.get("kernel") == args.kernel
+                and t.get("tiles") == tiles and prec == PREC_FP64):
+            traffic = t.get("uvp_bytes_per_launch")
+    if args.copy_probe:
+        # plain copy kernels of known size (physical level -> pseudo-time level of u, v, T,
+        # C), so that memory counters can be scaled against the bytes they must move
+        ctx.sync_levels()
+        nu, nv, npc = (prm.imax + 2) * (prm.jmax + 1), (prm.imax + 1) * (prm.jmax + 2), (prm.imax + 1) * (prm.jmax + 1)
+        log("copy probe: k_copy launches of %d, %d, %d, %d doubles (read once, written once)" % (nu, nv, npc, npc))
+
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
@@ -129,11 +169,14 @@ def main():
         "data": "synthetic (seeded SplitMix64 inputs of configuration 3: mesh, porosity, state)",
         "config": {"workload": "%s fused pseudo-step, %s" % (cfg["name"], args.precision),
                    "imax": prm.imax, "jmax": prm.jmax, "kernel": args.kernel,
-                   "graph_steps": args.graph, "bytes_per_cell": B,
+                   "graph_steps": args.graph, "tiles": tiles,
                    "steps_fused": n_fused, "steps_exact": n_exact},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "kernel_ms_per_step": ms_event,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": rname, "bytes_per_cell": rB, "cells_per_launch": cells, "kernel_ms": rms},
+        "step_roofline": {"bytes_per_cell": Bstep, "ms": ms_event, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "kernel_ms": kern,
     }
     ctx.close()
     if not args.no_cpu:

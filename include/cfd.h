@@ -11,7 +11,7 @@
  *   c_cuda/functions.h:7-21,36 (RESU..bcC, max_reduce) -> include/cfd_legacy.h
  *   c_cuda/main.cu:203-277 (pseudo-time and physical-time loops) -> cfd_pseudo_steps,
  *     cfd_run_to_tol, cfd_physical_step below
- *   OpenMp_Multicore/main.f90:263-333 (same loops, Fortran)
+ *   OpenMp_Multicore/main.f90:116-157 (same loops, Fortran)
  *
  * Array layout (identical to the reference C layout, c_cuda/main.cu:59-89):
  *   1-based, index 0 is padding, j (y direction) is the contiguous axis.
@@ -124,7 +124,7 @@ typedef struct cfd_params {
     double mu_coef[5];          /* CFD_MU_POLY4 coefficients c0..c4 */
 } cfd_params;
 
-/* Reference defaults (OpenMp_Multicore/comum.f90:29-115, main.f90:252-253, input/iterations.dat):
+/* Reference defaults (OpenMp_Multicore/comum.f90:29-115, main.f90:105-106, input/iterations.dat):
  * Re=Pe=Sc=1, Da=1e-2, Cf=1.75/sqrt(150*0.5^3), g=9.80665, Fr=1, v_i=0.5, Tinf=1,
  * dtau=0.05, dt=0.005, beta=1300, b_art=0, QUICK, reference BCs. */
 void cfd_params_default(cfd_params *p, int32_t imax, int32_t jmax);
@@ -167,7 +167,7 @@ int  cfd_ctx_download(cfd_ctx *ctx, int32_t field, double *host, int32_t layout)
  * because the iterate and the new level swap by pointer). */
 int  cfd_ctx_device_ptr(cfd_ctx *ctx, int32_t field, void **out);
 /* Reset pn := p, T_tau := T, C_tau := C, um_tau := um, vm_tau := vm on the device (the
- * reference leaves pn, T_tau, C_tau unset before the first pseudo-step, main.f90:252-256). */
+ * reference leaves pn, T_tau, C_tau unset before the first pseudo-step, main.f90:105-109). */
 int  cfd_ctx_sync_levels(cfd_ctx *ctx);
 
 /* Run `nsteps` pseudo-steps (U, V, P, T, C, then the tau-level swap), recording the
@@ -177,15 +177,15 @@ int  cfd_ctx_sync_levels(cfd_ctx *ctx);
 int  cfd_pseudo_steps(cfd_ctx *ctx, int32_t nsteps, double res[3], double *history);
 
 /* Iterate until max(res_u,res_v,res_p) < tol (test skipped on the first step, as
- * main.f90:289), checking every `check_every` steps, at most itc_max steps. A residual that
+ * main.f90:142), checking every `check_every` steps, at most itc_max steps. A residual that
  * exceeds `diverge_factor` times the smallest residual seen returns CFD_ERR_DIVERGED
  * (diverge_factor <= 0 disables the guard). */
 int  cfd_run_to_tol(cfd_ctx *ctx, double tol, int64_t itc_max, int32_t check_every,
                     double diverge_factor, int64_t *iters, double res[3]);
 
 /* One physical time step: pseudo-iterate to `eps` (at most itc_max steps, first-step
- * quirk of main.f90:224,267 reproduced when first != 0), then um=um_n_tau, vm=vm_n_tau,
- * T=T_n_tau, C=C_n_tau (main.f90:301-304). */
+ * quirk of main.f90:77,182 reproduced when first != 0), then um=um_n_tau, vm=vm_n_tau,
+ * T=T_n_tau, C=C_n_tau (main.f90:154-157). */
 int  cfd_physical_step(cfd_ctx *ctx, double eps, int64_t itc_max, int32_t first,
                        int64_t *iters, double res[3]);
 
@@ -201,10 +201,28 @@ int  cfd_ctx_last_timing(const cfd_ctx *ctx, double *ms_per_step);
  * single-pass kernel). Mainly for tests that pin both against each other. */
 int  cfd_ctx_set_kernel(cfd_ctx *ctx, int32_t kernel);
 
+/* Tile shapes of the fused kernels: row-block heights of the u/v/p kernel (hb1) and of the
+ * T/C kernel (hb2); 0 selects the built-in choice (one block per CU). */
+int  cfd_ctx_set_tiles(cfd_ctx *ctx, int32_t hb1, int32_t hb2);
+int  cfd_ctx_get_tiles(const cfd_ctx *ctx, int32_t *hb1, int32_t *hb2);
+
+/* Autotuner: times `steps` fused pseudo-steps for each candidate pair of row-block heights
+ * on a snapshot of the current state, restores the state, and keeps the fastest pair. The
+ * result is cached per (imax, jmax, precision, model switches) for the process, so later
+ * contexts of the same grid class start tuned. out_ms (may be NULL) receives the best mean
+ * time per step. */
+int  cfd_ctx_autotune(cfd_ctx *ctx, int32_t steps, int32_t *hb1, int32_t *hb2, double *out_ms);
+
 /* Pseudo-steps taken by each path since the context was created: fused kernels and the
  * exact per-operator path (the latter runs the first step after new inputs, and every step
  * with b_art != 0 or CFD_COMPAT_CUDA). */
 int  cfd_ctx_step_counts(const cfd_ctx *ctx, int64_t *fused, int64_t *exact);
+
+/* `nsteps` pseudo-steps like cfd_pseudo_steps, with events around each launch of the fused
+ * path: ms[0] = u/v/p kernel, ms[1] = T/C kernel, ms[2] = boundary pass, ms[3] = whole step,
+ * each the mean over the fused steps (steps that take the exact path are not timed).
+ * The host waits after every step, so use it for measurement, not production runs. */
+int  cfd_pseudo_steps_timed(cfd_ctx *ctx, int32_t nsteps, double res[3], double ms[4]);
 
 const char *cfd_last_error(void);
 const char *cfd_version(void);

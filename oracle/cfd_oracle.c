@@ -616,7 +616,7 @@ void orc_solve_c(orc_ctx *o, const double *um_n, const double *vm_n, const doubl
 /* ------------------------------------------------------------------------------------ */
 static void copy_(double *dst, const double *src, size_t n) { memcpy(dst, src, n * sizeof(double)); }
 
-/* main.f90:267-297. With steady params the physical level is the iterate itself. */
+/* main.f90:120-151. With steady params the physical level is the iterate itself. */
 double orc_pseudo_step(orc_ctx *o, orc_state *s, double res[3], int do_swap)
 {
     const int st = o->P.steady;
@@ -652,7 +652,7 @@ void orc_pseudo_steps(orc_ctx *o, orc_state *s, int n, double *history)
     }
 }
 
-/* main.f90:263-305: one pass of the physical-time loop body. */
+/* main.f90:116-157: one pass of the physical-time loop body. */
 long orc_physical_step(orc_ctx *o, orc_state *s, double eps, long itc_max, int first, double res[3])
 {
     long itc = first ? 1 : 0, taken = 0;

@@ -68,19 +68,19 @@ void   orc_solve_z(orc_ctx *o, const double *um_n, const double *vm_n, const dou
 void   orc_solve_c(orc_ctx *o, const double *um_n, const double *vm_n, const double *c,
                    double *c_n_tau, const double *c_tau);
 
-/* Full state of the pseudo-time loop (main.f90:162-167), C layout. */
+/* Full state of the pseudo-time loop (main.f90:15-20), C layout. */
 typedef struct orc_state {
     double *um, *vm, *um_tau, *vm_tau, *um_n_tau, *vm_n_tau;
     double *p, *pn, *T, *T_tau, *T_n_tau, *C, *C_tau, *C_n_tau;
 } orc_state;
 
-/* One pseudo-step of main.f90:267-297 (U, V, P, Z(T), C, residual, tau swap unless
+/* One pseudo-step of main.f90:120-151 (U, V, P, Z(T), C, residual, tau swap unless
  * `converged_exit` is returned). `steady` params alias the physical level to the iterate.
  * res[3] = (res_u, res_v, res_p). Returns the error max(res_u,res_v,res_p). */
 double orc_pseudo_step(orc_ctx *o, orc_state *s, double res[3], int do_swap);
 /* n pseudo-steps with swaps; history (3n doubles) optional. */
 void   orc_pseudo_steps(orc_ctx *o, orc_state *s, int n, double *history);
-/* One physical step (main.f90:263-305); first != 0 reproduces the itc=1 start. Returns
+/* One physical step (main.f90:116-157); first != 0 reproduces the itc=1 start. Returns
  * the pseudo-steps taken. */
 long   orc_physical_step(orc_ctx *o, orc_state *s, double eps, long itc_max, int first, double res[3]);
 

@@ -15,7 +15,7 @@
 !   state  (modes 1-3): um(imax+1,jmax), vm(imax,jmax+1), p(imax,jmax), T(imax,jmax), C(imax,jmax)
 ! Output (stream): harness_out.bin, layout per mode (see make_golden.py).
 !
-! The driver loops restate OpenMp_Multicore/main.f90:263-305 with the unset levels pinned:
+! The driver loops restate OpenMp_Multicore/main.f90:116-157 with the unset levels pinned:
 ! pn = p, T_tau = T, C_tau = C before the first pseudo-step.
 program ref_harness
     use comum

@@ -24,7 +24,7 @@ EXPORTS_CFD = [
     "cfd_ctx_get_params", "cfd_ctx_set_params", "cfd_ctx_set_mesh", "cfd_ctx_upload", "cfd_ctx_download",
     "cfd_ctx_device_ptr", "cfd_ctx_sync_levels", "cfd_pseudo_steps", "cfd_run_to_tol", "cfd_physical_step",
     "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_ctx_step_counts",
-    "cfd_last_error", "cfd_version",
+    "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed", "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
     "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
@@ -75,6 +75,10 @@ def load():
         "cfd_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
         "cfd_ctx_set_kernel": (ctypes.c_int, [_vp, i32]),
         "cfd_ctx_step_counts": (ctypes.c_int, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "cfd_ctx_set_tiles": (ctypes.c_int, [_vp, i32, i32]),
+        "cfd_pseudo_steps_timed": (ctypes.c_int, [_vp, i32, _dp, _dp]),
+        "cfd_ctx_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "cfd_ctx_autotune": (ctypes.c_int, [_vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), _dp]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -217,6 +221,35 @@ class Context:
         f, e = ctypes.c_int64(0), ctypes.c_int64(0)
         check(load().cfd_ctx_step_counts(self.h, ctypes.byref(f), ctypes.byref(e)))
         return f.value, e.value
+
+    def pseudo_steps_timed(self, n):
+        """n pseudo-steps with per-launch events: returns (res, ms) where ms = mean ms of the
+        u/v/p kernel, the T/C kernel, the boundary pass and the whole fused step."""
+        res, ms = np.zeros(3), np.zeros(4)
+        check(load().cfd_pseudo_steps_timed(self.h, int(n), _ptr(res), _ptr(ms)))
+        return res, ms
+
+    def sync_levels(self):
+        """Copy the physical levels into the pseudo-time iterates (um_tau := um, vm_tau := vm,
+        T_tau := T, C_tau := C); the next pseudo-step starts like the first one."""
+        check(load().cfd_ctx_sync_levels(self.h))
+
+    def set_tiles(self, hb1, hb2):
+        """Row-block heights of the fused u/v/p and T/C kernels (0: built-in choice)."""
+        check(load().cfd_ctx_set_tiles(self.h, int(hb1), int(hb2)))
+
+    def get_tiles(self):
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(load().cfd_ctx_get_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, steps=10):
+        """Time candidate row-block heights on a snapshot of the state (restored afterwards);
+        returns (hb1, hb2, best ms per step)."""
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        ms = np.zeros(1)
+        check(load().cfd_ctx_autotune(self.h, int(steps), ctypes.byref(a), ctypes.byref(b), _ptr(ms)))
+        return a.value, b.value, float(ms[0])
 
     def device_ptr(self, name):
         p = _vp()

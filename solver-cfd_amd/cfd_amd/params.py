@@ -1,7 +1,7 @@
 """ctypes mirror of ``cfd_params`` (include/cfd.h) and its reference defaults.
 
 The defaults restate OpenMp_Multicore/comum.f90:29-115 (Re, Pe, Sc, Darcy number, Cf,
-g, Fr, v_i), main.f90:185 (Tinf), main.f90:252-253 (dtau, dt) and
+g, Fr, v_i), main.f90:38 (Tinf), main.f90:105-106 (dtau, dt) and
 OpenMp_Multicore/input/iterations.dat (beta, b_art).
 """
 import ctypes

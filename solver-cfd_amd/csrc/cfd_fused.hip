@@ -1,6 +1,6 @@
 // cfd_fused.hip — the fused pseudo-step: two row-streaming kernels and their support.
 //
-// One pseudo-step (main.f90:269-279: solve_U, solve_V, solve_P, solve_Z, solve_C) runs as
+// One pseudo-step (main.f90:122-132: solve_U, solve_V, solve_P, solve_Z, solve_C) runs as
 //   K1  k_uvp    the three SSP-RK3 stages of u and v and the pressure update,
 //   K2  k_scal   the three stages of T and of C,
 //   k_materialize  the boundary cells of the five outputs (the kernels themselves read
@@ -89,28 +89,19 @@ constexpr int W1 = NT1 - HL1 - HR1;             // output columns per workgroup
 constexpr int PAD = 3;                          // LDS padding for j-3 .. j+2 reads
 constexpr int LROW1 = NT1 + 2 * PAD;
 
+// Scalars are precomputed on the host so that few values occupy scalar registers.
 struct K1Args {
-    const double *um, *vm, *ut, *vt, *p, *T;    // inputs
-    double *un, *vn, *pn;                       // outputs (interior cells)
-    unsigned long long *res;                    // max |r3| bit patterns: u, v, p
-    const double *eps, *lp;
-    const double *aun, *aue, *avn, *ave;        // face areas
-    const double *rdx, *rdy, *rdxm, *rdym;      // reciprocal spacings
-    int imax, jmax, nbi, nbj, hb;
-    cfd_params P;
+    const double *um, *vm, *ut, *vt, *p, *T, *eps, *lp;   // inputs
+    double *un, *vn, *pn;                                  // outputs (interior cells)
+    unsigned long long *res;                               // max |r3| bit patterns: u, v, p
+    const double *pk;                                      // packed geometry (cfd_devfactors.pack)
+    int imax, jmax, nbi, nbj, hb, chan, upwind_all, bouss;
+    double dt, dtau, beta, inv_re, inv_reda, cfsd, g, invfr2, ri, t_ref, v_i, u_in;
+    double mu[5];
 };
 
-// u of the centre row at column j+dj, the reference BC formula on boundary columns
-// (boundary.f90:14-16,24-26 / channel: no-slip walls). Centre rows are interior rows.
-__device__ __forceinline__ double lds_u(const double *row, int x, int j, int dj, int jm, bool chan)
-{
-    const int jj = j + dj;
-    if (jj == 1) return 0.0;
-    if (jj == jm) return chan ? 0.0 : row[PAD + x + dj - 1];
-    return row[PAD + x + dj];
-}
-
-// v of the centre row at column j+dj (boundary.f90:9-12,19-22 / channel walls).
+// v of the centre row at column j+dj with the BC formulas (boundary.f90:9-12,19-22 /
+// channel walls): used where the ring's boundary columns are not yet fixed up.
 __device__ __forceinline__ double lds_v(const double *row, int x, int j, int dj, int jm, bool chan, double vi)
 {
     const int jj = j + dj;
@@ -124,8 +115,25 @@ __device__ __forceinline__ double lds_v(const double *row, int x, int j, int dj,
     return row[PAD + x + dj];
 }
 
-template <bool STEADY>
-__global__ __launch_bounds__(NT1, 1) void k_uvp(K1Args a)
+// Reference-set boundary columns of a u ring row (u(jmax) = u(jmax-1), boundary.f90:24-26)
+// and of a v ring row (top extrapolation, boundary.f90:19-22), written by the lanes that
+// own those columns one iteration after the row was produced and before it is read.
+__device__ __forceinline__ void fix_u(double *row, int x, int j, int jm)
+{
+    if (j == jm) row[PAD + x] = row[PAD + x - 1];
+}
+__device__ __forceinline__ void fix_v(double *row, int x, int j, int jm)
+{
+    if (j == jm) {
+        row[PAD + x] = 2.0 * row[PAD + x - 2] - row[PAD + x - 1];
+    } else if (j == jm + 1) {
+        const double a = row[PAD + x - 3], b = row[PAD + x - 2];
+        row[PAD + x] = 2.0 * b - (2.0 * a - b);
+    }
+}
+
+template <bool STEADY, bool MUPOLY>
+__global__ __launch_bounds__(NT1, 1) void k_uvp(const K1Args a)
 {
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
@@ -136,13 +144,10 @@ __global__ __launch_bounds__(NT1, 1) void k_uvp(K1Args a)
     const int jlo = 2 + bj * W1, jhi = min(jlo + W1, jm);            // output cols [jlo, jhi)
     const int x = threadIdx.x;
     const int j = jlo - HL1 + x;
-    const cfd_params &P = a.P;
-    const bool chan = P.bc == CFD_BC_CHANNEL;
-    const bool upwind_all = P.scheme == CFD_SCHEME_UPWIND;
-    const bool mu_poly = P.mu_model == CFD_MU_POLY4;
-    const bool bouss = P.buoyancy == CFD_BUOY_BOUSSINESQ;
-    const double dt = P.dt, dtau = P.dtau, vi = P.v_i;
-    const double inv_re = 1.0 / P.re, inv_reda = 1.0 / (P.re * P.darcy);
+    const bool chan = a.chan != 0, upwind_all = a.upwind_all != 0, bouss = a.bouss != 0;
+    const double dt = a.dt, dtau = a.dtau, vi = a.v_i;
+    const int R = im + 2, Cc = jm + 2;
+    const double *pr = a.pk, *pc = a.pk + 4 * R;    // rows [aun|avn|rdx|rdxm], cols [aue|ave|rdy|rdym]
 
     __shared__ double sUT[4][LROW1], sVT[4][LROW1], sP[2][LROW1], sT[2][LROW1];
     __shared__ double sU1[3][LROW1], sU2[3][LROW1], sV1[3][LROW1], sV2[3][LROW1], sV3[2][LROW1];
@@ -155,10 +160,11 @@ __global__ __launch_bounds__(NT1, 1) void k_uvp(K1Args a)
     // per-column geometry (clamped indices; values outside the ranges are never used)
     const bool jvalid = j >= 1 && j <= jm + 1;
     const int jc = min(max(j, 2), jm);
-    const double aue_j = a.aue[min(max(j, 1), jm)];
-    const double ave_j = a.ave[jc];
-    const double rdy_j = a.rdy[jc], rdy_n = a.rdy[min(jc + 1, jm)];
-    const double rdym_j = a.rdym[jc], rdym_n = a.rdym[min(jc + 1, jm + 1)];
+    const double haue_j = 0.5 * pc[min(max(j, 1), jm)];
+    const double aue_j = pc[min(max(j, 1), jm)];
+    const double ave_j = pc[Cc + jc], have_j = 0.5 * ave_j;
+    const double rdy_j = pc[2 * Cc + jc], rdy_n = pc[2 * Cc + min(jc + 1, jm)];
+    const double rdym_j = pc[3 * Cc + jc], rdym_n = pc[3 * Cc + min(jc + 1, jm + 1)];
     const bool ucol = j >= 2 && j <= jm - 1;           // u / p interior column
     const bool vcol = j >= 3 && j <= jm - 1;           // v interior column
     const bool u_upw_col = upwind_all || j == 2 || j == jm - 1;
@@ -172,30 +178,40 @@ __global__ __launch_bounds__(NT1, 1) void k_uvp(K1Args a)
     Win<5> w1, w2, x1, x2;                            // u1, u2, v1, v2 own windows
     Win<2> w3, x3;                                    // u3, v3
     ut.fill(0.0); Su.fill(0.0); vt.fill(0.0); Svn.fill(0.0); pw.fill(0.0);
-    ie.fill(1.0); ew.fill(1.0); nuw.fill(0.0); k1w.fill(0.0); k2w.fill(0.0); dpj.fill(0.0); bw.fill(0.0);
-    du.fill(0.0); dv.fill(0.0);
+    ie.fill(1.0); ew.fill(1.0); k1w.fill(0.0); k2w.fill(0.0); dpj.fill(0.0); bw.fill(0.0);
+    if (MUPOLY) nuw.fill(0.0);
+    if (!STEADY) { du.fill(0.0); dv.fill(0.0); }
     w1.fill(0.0); w2.fill(0.0); x1.fill(0.0); x2.fill(0.0); w3.fill(0.0); x3.fill(0.0);
     unsigned long long mu_ = 0, mv_ = 0, mp_ = 0;
-    __syncthreads();
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
-    for (int t = ilo - 6; t <= ihi + 6; ++t) {
-        // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
-        double ut_new = 0.0, vt_new = 0.0;
+    // next-row registers: the loads for row t+1 are issued before row t is computed
+    double q_ut, q_vt, q_p, q_T, q_e, q_L, q_um, q_vm;
+    auto fetch = [&](int t) {
+        q_ut = 0.0; q_vt = 0.0; q_p = 0.0; q_T = 1.0; q_e = 1.0; q_L = 0.0; q_um = 0.0; q_vm = 0.0;
         if (jvalid && t >= 1) {
-            if (t <= im + 1 && j <= jm) ut_new = a.ut[(size_t)t * su + j];
-            if (t <= im) vt_new = a.vt[(size_t)t * sv + j];
+            if (t <= im + 1 && j <= jm) q_ut = a.ut[(size_t)t * su + j];
+            if (t <= im) q_vt = a.vt[(size_t)t * sv + j];
         }
         const int rp = t - 2;
-        double p_rp = 0.0, T_rp = 1.0, e = 1.0, Lp = 0.0, um_rp = 0.0, vm_rp = 0.0;
         if (jvalid && rp >= 1 && rp <= im) {
             if (j <= jm) {
                 const size_t k = (size_t)rp * su + j;
-                p_rp = a.p[k]; T_rp = a.T[k]; e = a.eps[k]; Lp = a.lp[k];
-                if (!STEADY) um_rp = a.um[k];
+                q_p = a.p[k]; q_T = a.T[k]; q_e = a.eps[k]; q_L = a.lp[k];
+                if (!STEADY) q_um = a.um[k];
             }
-            if (!STEADY) vm_rp = a.vm[(size_t)rp * sv + j];
+            if (!STEADY) q_vm = a.vm[(size_t)rp * sv + j];
         }
+    };
+    __syncthreads();
+    fetch(ilo - 6);
+
+    for (int t = ilo - 6; t <= ihi + 6; ++t) {
+        // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
+        const double ut_new = q_ut, vt_new = q_vt, p_rp = q_p, T_rp = q_T, e = q_e, Lp = q_L;
+        const double um_rp = q_um, vm_rp = q_vm;
+        fetch(t + 1);
+        const int rp = t - 2;
         ut.push(ut_new);
         vt.push(vt_new);
         sUT[s4(t)][PAD + x] = ut_new;
@@ -204,143 +220,154 @@ __global__ __launch_bounds__(NT1, 1) void k_uvp(K1Args a)
         sT[s2(rp)][PAD + x] = T_rp;
         __syncthreads();
 
-        // ---------------- phase B: per-cell data of row t-2, neighbour sums
+        // ---------------- phase B
+        if (!chan) {   // boundary columns of the ring rows produced in the previous iteration
+            fix_u(sU1[s3(t - 3)], x, j, jm);
+            fix_u(sU2[s3(t - 5)], x, j, jm);
+            fix_v(sV1[s3(t - 3)], x, j, jm);
+            fix_v(sV2[s3(t - 5)], x, j, jm);
+        }
         Su.push(ut_new + sUT[s4(t)][PAD + x - 1]);
         Svn.push(sVT[s4(t - 1)][PAD + x + 1] + sVT[s4(t - 2)][PAD + x + 1]);
         pw.push(p_rp);
         {
-            const double iev = 1.0 / e;
-            double nu = e * inv_re;
-            if (mu_poly) {
+            double iev = 1.0, k1 = 0.0, k2 = 0.0;
+            if (rp >= ilo - 4 && rp <= ihi + 4) {
+                iev = e == 1.0 ? 1.0 : 1.0 / e;
+                if (Lp != 0.0) {
+                    k1 = e * Lp * a.inv_reda;
+                    k2 = Lp * a.cfsd * sqrt(e);
+                }
+            }
+            ie.push(iev); ew.push(e); k1w.push(k1); k2w.push(k2);
+            if (MUPOLY) {
                 const double tt = T_rp;
-                nu *= (((P.mu_coef[4] * tt + P.mu_coef[3]) * tt + P.mu_coef[2]) * tt + P.mu_coef[1]) * tt + P.mu_coef[0];
+                nuw.push(e * a.inv_re * ((((a.mu[4] * tt + a.mu[3]) * tt + a.mu[2]) * tt + a.mu[1]) * tt + a.mu[0]));
             }
-            double k1 = 0.0, k2 = 0.0;
-            if (Lp != 0.0) {
-                k1 = e * Lp * inv_reda;
-                k2 = e * Lp * P.cf / sqrt(e * P.darcy);
-            }
-            ie.push(iev); ew.push(e); nuw.push(nu); k1w.push(k1); k2w.push(k2);
             dpj.push(p_rp - sP[s2(rp)][PAD + x - 1]);
             const double Tm = sT[s2(rp)][PAD + x - 1];
-            bw.push(bouss ? P.ri * ((T_rp + Tm) * 0.5 - P.t_ref) : P.invfr2 * (1.0 - 1.0 / ((T_rp + Tm) * 0.5)));
+            bw.push(bouss ? a.ri * ((T_rp + Tm) * 0.5 - a.t_ref) : a.invfr2 * (1.0 - 2.0 / (T_rp + Tm)));
             if (!STEADY) { du.push(um_rp - ut[2]); dv.push(vm_rp - vt[2]); }
         }
 
-        // ---------------- U stage k at row r = t - 2k (u update: i = 3..imax-1, j = 2..jmax-1)
+        // ---------------- U stage K at row r = t - 2K (u update: i = 3..imax-1, j = 2..jmax-1)
 #define U_STAGE(K, WP, RING_C, WOUT, RING_OUT)                                                   \
         {                                                                                         \
             const int r = t - 2 * K;                                                              \
             const int bk = 2 * K - 2; /* back index of row r in the rows t-6..t-2 windows */      \
             double val = 0.0, rs = 0.0;                                                           \
             bool interior = false;                                                                \
-            if (r >= 3 && r <= im - 1) {                                                          \
-                if (ucol) {                                                                       \
-                    interior = true;                                                              \
-                    const double uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0];    \
-                    const double *c = RING_C;                                                     \
-                    const double uS = lds_u(c, x, j, -1, jm, chan), uN = lds_u(c, x, j, 1, jm, chan); \
-                    const double iev = ie[bk], nu = nuw[bk], ev = ew[bk];                         \
-                    const double aun_r = a.aun[r], rdx_r = a.rdx[r];                              \
-                    const double cns = 0.5 * aun_r * iev, cew = 0.5 * aue_j * iev;                \
-                    const double sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;          \
-                    const double fn = Svn[2 * K - 1] * cns;                                       \
-                    const double fs = (vt[2 * K] + vt[2 * K + 1]) * cns;                          \
-                    const double fe = sE * cew, fw = sW * cew;                                    \
-                    const double nA = nu * aun_r, nB = nu * aue_j;                                \
-                    const double diff = nB * a.rdxm[r] * (uW - uP) + nB * a.rdxm[r + 1] * (uE - uP) \
-                                      + nA * rdy_j * (uS - uP) + nA * rdy_n * (uN - uP);          \
-                    double pw_, pe_, ps_, pn_;                                                    \
-                    if (u_upw_col) {                                                              \
-                        pw_ = fw > 0.0 ? uW : uP; pe_ = fe > 0.0 ? uP : uE;                       \
-                        ps_ = fs > 0.0 ? uS : uP; pn_ = fn > 0.0 ? uP : uN;                       \
-                    } else {                                                                      \
-                        const double uSS = lds_u(c, x, j, -2, jm, chan), uNN = lds_u(c, x, j, 2, jm, chan); \
-                        const double cW = uWW - 2.0 * uW + uP, cPi = uW - 2.0 * uP + uE, cE = uP - 2.0 * uE + uEE; \
-                        const double cS = uSS - 2.0 * uS + uP, cPj = uS - 2.0 * uP + uN, cN = uP - 2.0 * uN + uNN; \
-                        pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE);     \
-                        ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN);     \
+            if (r >= ilo - 2 * (3 - K) && r <= ihi + 2 * (3 - K)) {                               \
+                if (r >= 3 && r <= im - 1) {                                                      \
+                    if (ucol) {                                                                   \
+                        interior = true;                                                          \
+                        const double uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0]; \
+                        const double *c = &RING_C[PAD + x];                                       \
+                        const double uS = c[-1], uN = c[1];                                       \
+                        const double iev = ie[bk], ev = ew[bk];                                   \
+                        const double nu = MUPOLY ? nuw[bk] : ev * a.inv_re;                       \
+                        const double aun_r = pr[r], rdx_r = pr[2 * R + r];                        \
+                        const double cns = 0.5 * aun_r * iev, cew = haue_j * iev;                 \
+                        const double sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;      \
+                        const double fn = Svn[2 * K - 1] * cns;                                   \
+                        const double fs = (vt[2 * K] + vt[2 * K + 1]) * cns;                      \
+                        const double fe = sE * cew, fw = sW * cew;                                \
+                        const double diff = (nu * aue_j) * (pr[3 * R + r] * (uW - uP) + pr[3 * R + r + 1] * (uE - uP)) \
+                                          + (nu * aun_r) * (rdy_j * (uS - uP) + rdy_n * (uN - uP)); \
+                        double pw_, pe_, ps_, pn_;                                                \
+                        if (u_upw_col) {                                                          \
+                            pw_ = fw > 0.0 ? uW : uP; pe_ = fe > 0.0 ? uP : uE;                   \
+                            ps_ = fs > 0.0 ? uS : uP; pn_ = fn > 0.0 ? uP : uN;                   \
+                        } else {                                                                  \
+                            const double uSS = c[-2], uNN = c[2];                                 \
+                            const double cW = uWW - 2.0 * uW + uP, cPi = uW - 2.0 * uP + uE, cE = uP - 2.0 * uE + uEE; \
+                            const double cS = uSS - 2.0 * uS + uP, cPj = uS - 2.0 * uP + uN, cN = uP - 2.0 * uN + uNN; \
+                            pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE); \
+                            ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
+                        }                                                                         \
+                        const double conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;            \
+                        double Rr = rdx_r * (rdy_j * (diff + conv) - ev * (pw[bk] - pw[bk + 1])) - a.g * ev; \
+                        const double k2 = k2w[bk];                                                \
+                        if (k2 != 0.0) {                                                          \
+                            const double vP = vt[2 * K];                                          \
+                            Rr -= uP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                  \
+                        }                                                                         \
+                        rs = ((STEADY ? 0.0 : du[bk]) + Rr * dt) * dtau;                          \
+                        const double tau = ut[2 * K];                                             \
+                        if (K == 1) val = tau + rs;                                               \
+                        else if (K == 2) val = 0.75 * tau + 0.25 * (uP + rs);                     \
+                        else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (uP + rs);                      \
                     }                                                                             \
-                    const double conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                \
-                    double R = rdx_r * rdy_j * (diff + conv) - ev * (pw[bk] - pw[bk + 1]) * rdx_r - P.g * ev; \
-                    const double k2 = k2w[bk];                                                    \
-                    if (k2 != 0.0) {                                                              \
-                        const double vP = vt[2 * K];                                              \
-                        R -= uP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                       \
-                    }                                                                             \
-                    rs = ((STEADY ? 0.0 : du[bk]) + R * dt) * dtau;                               \
-                    const double tau = ut[2 * K];                                                 \
-                    if (K == 1) val = tau + rs;                                                   \
-                    else if (K == 2) val = 0.75 * tau + 0.25 * (uP + rs);                         \
-                    else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (uP + rs);                          \
+                } else if (r <= 2) {                                                              \
+                    val = chan ? a.u_in : 0.0;                                                    \
+                } else {                                                                          \
+                    val = (chan && ucol) ? WOUT[0] : 0.0;                                         \
                 }                                                                                 \
-            } else if (r <= 2) {                                                                  \
-                val = chan ? P.u_in : 0.0;                                                        \
-            } else {                                                                              \
-                val = (chan && ucol) ? WOUT[0] : 0.0;                                             \
+                RING_OUT                                                                          \
             }                                                                                     \
             WOUT.push(val);                                                                       \
-            RING_OUT                                                                              \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.un[(size_t)r * su + j] = val;                                                   \
                 const unsigned long long bb = dbits(rs); mu_ = bb > mu_ ? bb : mu_;               \
             }                                                                                     \
         }
 
-        // ---------------- V stage k at row r = t - 2k (v update: i = 2..imax-1, j = 3..jmax-1)
+        // ---------------- V stage K at row r = t - 2K (v update: i = 2..imax-1, j = 3..jmax-1)
 #define V_STAGE(K, WP, RING_C, WOUT, RING_OUT)                                                   \
         {                                                                                         \
             const int r = t - 2 * K;                                                              \
             const int bk = 2 * K - 2;                                                             \
             double val = 0.0, rs = 0.0;                                                           \
             bool interior = false;                                                                \
-            if (j == 2) {                                                                         \
-                val = chan ? 0.0 : vi;                                                            \
-            } else if (r >= 2 && r <= im - 1) {                                                   \
-                if (vcol) {                                                                       \
-                    interior = true;                                                              \
-                    const double vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0];    \
-                    const double *c = RING_C;                                                     \
-                    const double vS = lds_v(c, x, j, -1, jm, chan, vi), vN = lds_v(c, x, j, 1, jm, chan, vi); \
-                    const double iev = ie[bk], nu = nuw[bk], ev = ew[bk];                         \
-                    const double avn_r = a.avn[r], rdx_r = a.rdx[r];                              \
-                    const double cns = 0.5 * avn_r * iev, cew = 0.5 * ave_j * iev;                \
-                    const double sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;          \
-                    const double fn = sN * cns, fs = sS * cns;                                    \
-                    const double fe = Su[2 * K - 1] * cew, fw = Su[2 * K] * cew;                  \
-                    const double nA = nu * avn_r, nB = nu * ave_j;                                \
-                    const double diff = nB * rdx_r * (vW - vP) + nB * a.rdx[r + 1] * (vE - vP)    \
-                                      + nA * rdym_j * (vS - vP) + nA * rdym_n * (vN - vP);        \
-                    double pw_, pe_, ps_, pn_;                                                    \
-                    if (upwind_all || r == 2 || r == im - 1) {                                    \
-                        pw_ = fw > 0.0 ? vW : vP; pe_ = fe > 0.0 ? vP : vE;                       \
-                        ps_ = fs > 0.0 ? vS : vP; pn_ = fn > 0.0 ? vP : vN;                       \
-                    } else {                                                                      \
-                        const double vSS = lds_v(c, x, j, -2, jm, chan, vi), vNN = lds_v(c, x, j, 2, jm, chan, vi); \
-                        const double cW = vWW - 2.0 * vW + vP, cPi = vW - 2.0 * vP + vE, cE = vP - 2.0 * vE + vEE; \
-                        const double cS = vSS - 2.0 * vS + vP, cPj = vS - 2.0 * vP + vN, cN = vP - 2.0 * vN + vNN; \
-                        pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE);     \
-                        ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN);     \
+            if (r >= ilo - 2 * (3 - K) && r <= ihi - 1 + 2 * (3 - K)) {                           \
+                if (j <= 2) {                                                                     \
+                    val = chan ? 0.0 : vi;                                                        \
+                } else if (r >= 2 && r <= im - 1) {                                               \
+                    if (vcol) {                                                                   \
+                        interior = true;                                                          \
+                        const double vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0]; \
+                        const double *c = &RING_C[PAD + x];                                       \
+                        const double vS = c[-1], vN = c[1];                                       \
+                        const double iev = ie[bk], ev = ew[bk];                                   \
+                        const double nu = MUPOLY ? nuw[bk] : ev * a.inv_re;                       \
+                        const double avn_r = pr[R + r], rdx_r = pr[2 * R + r];                    \
+                        const double cns = 0.5 * avn_r * iev, cew = have_j * iev;                 \
+                        const double sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;      \
+                        const double fn = sN * cns, fs = sS * cns;                                \
+                        const double fe = Su[2 * K - 1] * cew, fw = Su[2 * K] * cew;              \
+                        const double diff = (nu * ave_j) * (rdx_r * (vW - vP) + pr[2 * R + r + 1] * (vE - vP)) \
+                                          + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP)); \
+                        double pw_, pe_, ps_, pn_;                                                \
+                        if (upwind_all || r == 2 || r == im - 1) {                                \
+                            pw_ = fw > 0.0 ? vW : vP; pe_ = fe > 0.0 ? vP : vE;                   \
+                            ps_ = fs > 0.0 ? vS : vP; pn_ = fn > 0.0 ? vP : vN;                   \
+                        } else {                                                                  \
+                            const double vSS = c[-2], vNN = c[2];                                 \
+                            const double cW = vWW - 2.0 * vW + vP, cPi = vW - 2.0 * vP + vE, cE = vP - 2.0 * vE + vEE; \
+                            const double cS = vSS - 2.0 * vS + vP, cPj = vS - 2.0 * vP + vN, cN = vP - 2.0 * vN + vNN; \
+                            pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE); \
+                            ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
+                        }                                                                         \
+                        const double conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;            \
+                        double Rr = rdy_j * (rdx_r * (diff + conv) - ev * dpj[bk]) + bw[bk];      \
+                        const double k2 = k2w[bk];                                                \
+                        if (k2 != 0.0) {                                                          \
+                            const double uP = ut[2 * K];                                          \
+                            Rr -= vP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                  \
+                        }                                                                         \
+                        rs = ((STEADY ? 0.0 : dv[bk]) + Rr * dt) * dtau;                          \
+                        const double tau = vt[2 * K];                                             \
+                        if (K == 1) val = tau + rs;                                               \
+                        else if (K == 2) val = 0.75 * tau + 0.25 * (vP + rs);                     \
+                        else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (vP + rs);                       \
+                        if (r == 2 && !chan) WOUT.set(0, val);   /* row 1 = row 2 (boundary.f90:34-36) */ \
                     }                                                                             \
-                    const double conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                \
-                    double R = rdx_r * rdy_j * (diff + conv) - ev * dpj[bk] * rdy_j + bw[bk];     \
-                    const double k2 = k2w[bk];                                                    \
-                    if (k2 != 0.0) {                                                              \
-                        const double uP = ut[2 * K];                                              \
-                        R -= vP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                       \
-                    }                                                                             \
-                    rs = ((STEADY ? 0.0 : dv[bk]) + R * dt) * dtau;                               \
-                    const double tau = vt[2 * K];                                                 \
-                    if (K == 1) val = tau + rs;                                                   \
-                    else if (K == 2) val = 0.75 * tau + 0.25 * (vP + rs);                         \
-                    else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (vP + rs);                           \
-                    if (r == 2 && !chan) WOUT.set(0, val);   /* row 1 = row 2 (boundary.f90:34-36) */ \
+                } else if (r == im) {                                                             \
+                    val = vcol ? WOUT[0] : 0.0;                                                   \
                 }                                                                                 \
-            } else if (r == im) {                                                                 \
-                val = vcol ? WOUT[0] : 0.0;                                                       \
+                RING_OUT                                                                          \
             }                                                                                     \
             WOUT.push(val);                                                                       \
-            RING_OUT                                                                              \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.vn[(size_t)r * sv + j] = val;                                                   \
                 const unsigned long long bb = dbits(rs); mv_ = bb > mv_ ? bb : mv_;               \
@@ -363,13 +390,13 @@ __global__ __launch_bounds__(NT1, 1) void k_uvp(K1Args a)
                 const double u3p = w3[1], u3e = w3[0];
                 const double v3p = x3[1];
                 const double v3n = lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
-                const double avn_r = a.avn[r];
+                const double avn_r = pr[R + r];
                 const double rpv = -((u3e * aue_j - u3p * aue_j) + (v3n * avn_r - v3p * avn_r));
-                const double dd = dtau * rpv * P.beta;
-                const double pr = pw[5];
-                double pi = pr + dd;
-                pi = 0.75 * pr + 0.25 * (pi + dd);
-                a.pn[(size_t)r * su + j] = 1.0 / 3.0 * pr + 2.0 / 3.0 * (pi + dd);
+                const double dd = dtau * rpv * a.beta;
+                const double pp = pw[5];
+                double pi = pp + dd;
+                pi = 0.75 * pp + 0.25 * (pi + dd);
+                a.pn[(size_t)r * su + j] = 1.0 / 3.0 * pp + 2.0 / 3.0 * (pi + dd);
                 const unsigned long long bb = dbits(dd); mp_ = bb > mp_ ? bb : mp_;
             }
         }
@@ -388,21 +415,20 @@ constexpr int W2 = NT2 - HL2 - HR2;
 constexpr int LROW2 = NT2 + 2 * PAD;
 
 struct K2Args {
-    const double *un, *vn, *T, *Tt, *C, *Ct;
+    const double *un, *vn, *T, *Tt, *C, *Ct, *eps, *lp, *y;
     double *Tn, *Cn;
-    const double *eps, *lp, *y;
-    const double *aue, *avn, *rdx, *rdy, *rdxm, *rdym;
-    int imax, jmax, nbi, nbj, hb;
-    cfd_params P;
+    const double *pk;
+    int imax, jmax, nbi, nbj, hb, chan;
+    double dt, dtau, kt, kc, tinf, t_bottom, t_top, v_i, u_in, slot_lo, slot_hi, slot_value;
 };
 
 // T of the centre row at column j+dj (boundary.f90:85-88 / channel wall temperatures).
-__device__ __forceinline__ double lds_t(const double *row, int x, int j, int dj, int jm, bool chan, const cfd_params &P)
+__device__ __forceinline__ double lds_t(const double *row, int x, int j, int dj, int jm, bool chan, const K2Args &a)
 {
     const int jj = j + dj;
-    if (jj == 1) return chan ? P.t_bottom : P.tinf;
+    if (jj == 1) return chan ? a.t_bottom : a.tinf;
     if (jj == jm) {
-        if (chan) return P.t_top;
+        if (chan) return a.t_top;
         const double a1 = row[PAD + x + dj - 1], a2 = row[PAD + x + dj - 2];
         return a1 + 1.0 * (a1 - a2);
     }
@@ -423,7 +449,7 @@ __device__ __forceinline__ double lds_c(const double *row, int x, int j, int dj,
 }
 
 template <bool STEADY>
-__global__ __launch_bounds__(NT2, 1) void k_scal(K2Args a)
+__global__ __launch_bounds__(NT2, 1) void k_scal(const K2Args a)
 {
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
@@ -434,10 +460,10 @@ __global__ __launch_bounds__(NT2, 1) void k_scal(K2Args a)
     const int jlo = 2 + bj * W2, jhi = min(jlo + W2, jm);
     const int x = threadIdx.x;
     const int j = jlo - HL2 + x;
-    const cfd_params &P = a.P;
-    const bool chan = P.bc == CFD_BC_CHANNEL;
-    const double dt = P.dt, dtau = P.dtau;
-    const double kt = 1.0 / P.pe, kc = 1.0 / P.re / P.sc;
+    const bool chan = a.chan != 0;
+    const double dt = a.dt, dtau = a.dtau, kt = a.kt, kc = a.kc;
+    const int R = im + 2, Cc = jm + 2;
+    const double *pr = a.pk, *pc = a.pk + 4 * R;
 
     __shared__ double sTT[3][LROW2], sCT[3][LROW2], sV[2][LROW2];
     __shared__ double sZ1[2][LROW2], sZ2[2][LROW2], sC1[2][LROW2], sC2[2][LROW2];
@@ -447,37 +473,46 @@ __global__ __launch_bounds__(NT2, 1) void k_scal(K2Args a)
     }
     const bool jvalid = j >= 1 && j <= jm;
     const int jc = min(max(j, 2), jm - 1);
-    const double aue_j = a.aue[min(max(j, 1), jm)];
-    const double rdy_j = a.rdy[jc], rdy_n = a.rdy[jc + 1];
-    const double rdym_n = a.rdym[jc + 1];
+    const double aue_j = pc[min(max(j, 1), jm)];
+    const double rdy_j = pc[2 * Cc + jc], rdy_n = pc[2 * Cc + jc + 1];
+    const double rdym_n = pc[3 * Cc + jc + 1];
     const bool scol = j >= 2 && j <= jm - 1;
     const bool out_col = j >= jlo && j < jhi;
-    const double cslot = chan ? slot_value(P, a.y[min(max(j, 1), jm)]) : 0.0;
+    const double yj = a.y[min(max(j, 1), jm)];
+    const double cslot = chan ? ((yj >= a.slot_lo && yj <= a.slot_hi) ? a.slot_value : 0.0) : 0.0;
 
     Win<4> tt, ct, uw, vw, vnw, id, oml, dT, dC;
-    Win<3> z1, z2, c1, c2;
+    Win<3> z1, z2, c1, c2, z3, c3;
     tt.fill(0.0); ct.fill(0.0); uw.fill(0.0); vw.fill(0.0); vnw.fill(0.0); id.fill(1.0); oml.fill(1.0);
-    dT.fill(0.0); dC.fill(0.0); z1.fill(0.0); z2.fill(0.0); c1.fill(0.0); c2.fill(0.0);
-    __syncthreads();
+    if (!STEADY) { dT.fill(0.0); dC.fill(0.0); }
+    z1.fill(0.0); z2.fill(0.0); c1.fill(0.0); c2.fill(0.0); z3.fill(0.0); c3.fill(0.0);
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
-    for (int t = ilo - 3; t <= ihi + 2; ++t) {
-        // ---------------- phase A: row t
-        double tnew = 0.0, cnew = 0.0, unew = 0.0, vnew = 0.0, Tp = 0.0, Cp = 0.0, e = 1.0, Lp = 0.0;
+    double q_t, q_c, q_u, q_v, q_T, q_C, q_e, q_L;
+    // next row: u3/v3 from K1's interior cells, boundary rows/columns from the BC formulas
+    auto fetch = [&](int t) {
+        q_t = 0.0; q_c = 0.0; q_u = 0.0; q_v = 0.0; q_T = 0.0; q_C = 0.0; q_e = 1.0; q_L = 0.0;
         if (jvalid && t >= 1 && t <= im + 1) {
             if (t <= im) {
                 const size_t k = (size_t)t * su + j;
-                tnew = a.Tt[k]; cnew = a.Ct[k]; e = a.eps[k]; Lp = a.lp[k];
-                if (!STEADY) { Tp = a.T[k]; Cp = a.C[k]; }
-                // v3: interior rows/columns as written by K1, boundary column 2 from its BC
-                if (j == 2) vnew = chan ? 0.0 : P.v_i;
-                else if (t >= 2 && t <= im - 1 && j <= jm - 1) vnew = a.vn[(size_t)t * sv + j];
+                q_t = a.Tt[k]; q_c = a.Ct[k]; q_e = a.eps[k]; q_L = a.lp[k];
+                if (!STEADY) { q_T = a.T[k]; q_C = a.C[k]; }
+                if (j == 2) q_v = chan ? 0.0 : a.v_i;
+                else if (t >= 2 && t <= im - 1 && j <= jm - 1) q_v = a.vn[(size_t)t * sv + j];
             }
-            // u3: interior rows as written by K1, boundary rows from the BC formulas
-            if (t <= 2) unew = chan ? P.u_in : 0.0;
-            else if (t >= im) unew = chan ? uw[0] : 0.0;
-            else unew = a.un[(size_t)t * su + j];
+            if (t > 2 && t < im) q_u = a.un[(size_t)t * su + j];
         }
+    };
+    __syncthreads();
+    fetch(ilo - 3);
+
+    for (int t = ilo - 3; t <= ihi + 2; ++t) {
+        // ---------------- phase A: row t
+        const double tnew = q_t, cnew = q_c, vnew = q_v, Tp = q_T, Cp = q_C, e = q_e, Lp = q_L;
+        double unew = q_u;
+        fetch(t + 1);
+        if (t <= 2) unew = chan ? a.u_in : 0.0;             // u3 boundary rows (boundary.f90:29-32,39-42)
+        else if (t >= im) unew = chan ? uw[0] : 0.0;
         tt.push(tnew); ct.push(cnew); uw.push(unew); vw.push(vnew);
         sTT[s3(t)][PAD + x] = tnew;
         sCT[s3(t)][PAD + x] = cnew;
@@ -486,54 +521,53 @@ __global__ __launch_bounds__(NT2, 1) void k_scal(K2Args a)
 
         // ---------------- phase B: per-cell data of row t
         {
-            const int jn = j + 1;
             double vn_ = sV[s2(t)][PAD + x + 1];
-            if (jn == jm) vn_ = chan ? 0.0 : 2.0 * sV[s2(t)][PAD + x - 1] - sV[s2(t)][PAD + x];
+            if (j + 1 == jm) vn_ = chan ? 0.0 : 2.0 * sV[s2(t)][PAD + x - 1] - sV[s2(t)][PAD + x];
             vnw.push(vn_);
             id.push(Lp != 0.0 ? 1.0 / (Lp * (e - 1.0) + 1.0) : 1.0);
             oml.push(1.0 - Lp);
             if (!STEADY) { dT.push(Tp - tnew); dC.push(Cp - cnew); }
         }
 
-
-        // stage k of a scalar at row r = t - k; newest own row of WP is t-k+1
+        // stage K of a scalar at row r = t - K; newest own row of WP is t-K+1
 #define SCAL_STAGE(K, WP, CROW, ISC, WOUT, RINGW, OUTP, KAPPA, DW, TAUW)                         \
         {                                                                                         \
             const int r = t - K;                                                                  \
             const int bq = K;            /* back index of row r in windows whose newest is t */   \
             double val = 0.0;                                                                     \
             bool interior = false;                                                                \
-            if (r >= 2 && r <= im - 1) {                                                          \
-                if (scol) {                                                                       \
-                    interior = true;                                                              \
-                    const double zE = WP[0], zP = WP[1], zW = WP[2];                              \
-                    const double *c = CROW;                                                       \
-                    const double zS = ISC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, P); \
-                    const double zN = ISC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, P);   \
-                    const double avn_r = a.avn[r];                                                \
-                    const double kA = KAPPA * aue_j, kB = KAPPA * avn_r;                          \
-                    const double diff = kA * a.rdx[r + 1] * (zE - zP) + kA * a.rdx[r] * (zW - zP) \
-                                      + kB * rdy_n * (zN - zP) + kB * rdy_j * (zS - zP);          \
-                    const double uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];         \
-                    const double conv = 0.5 * ((zE + zP) * uE - (zW + zP) * uP) * aue_j           \
-                                      + 0.5 * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;          \
-                    const double vols = a.rdxm[r + 1] * rdym_n;                                   \
-                    const double R = ISC ? vols * (diff - conv * id[bq])                          \
-                                         : vols * (diff - oml[bq] * conv) * id[bq];               \
-                    const double rs = ((STEADY ? 0.0 : DW[bq]) + R * dt) * dtau;                  \
-                    const double tau = TAUW[bq];                                                  \
-                    if (K == 1) val = tau + rs;                                                   \
-                    else if (K == 2) val = 0.75 * tau + 0.25 * (zP + rs);                         \
-                    else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (zP + rs);                          \
-                    if (r == 2) WOUT.set(0, (ISC && chan) ? cslot : val);  /* row 1 */            \
+            if (r >= ilo - (3 - K) && r <= ihi - 1 + (3 - K)) {                                   \
+                if (r >= 2 && r <= im - 1) {                                                      \
+                    if (scol) {                                                                   \
+                        interior = true;                                                          \
+                        const double zE = WP[0], zP = WP[1], zW = WP[2];                          \
+                        const double *c = CROW;                                                   \
+                        const double zS = ISC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, a); \
+                        const double zN = ISC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, a);   \
+                        const double avn_r = pr[R + r];                                           \
+                        const double diff = (KAPPA * aue_j) * (pr[2 * R + r + 1] * (zE - zP) + pr[2 * R + r] * (zW - zP)) \
+                                          + (KAPPA * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP)); \
+                        const double uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];     \
+                        const double conv = 0.5 * ((zE + zP) * uE - (zW + zP) * uP) * aue_j       \
+                                          + 0.5 * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;      \
+                        const double vols = pr[3 * R + r + 1] * rdym_n;                           \
+                        const double Rr = ISC ? vols * (diff - conv * id[bq])                     \
+                                              : vols * (diff - oml[bq] * conv) * id[bq];          \
+                        const double rs = ((STEADY ? 0.0 : DW[bq]) + Rr * dt) * dtau;             \
+                        const double tau = TAUW[bq];                                              \
+                        if (K == 1) val = tau + rs;                                               \
+                        else if (K == 2) val = 0.75 * tau + 0.25 * (zP + rs);                     \
+                        else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (zP + rs);                       \
+                        if (r == 2) WOUT.set(0, (ISC && chan) ? cslot : val);  /* row 1 */        \
+                    }                                                                             \
+                } else if (r == im) {                                                             \
+                    val = scol ? WOUT[0] : 0.0;                                                   \
+                } else if (r == 1) {                                                              \
+                    val = (ISC && chan) ? cslot : 0.0;                                            \
                 }                                                                                 \
-            } else if (r == im) {                                                                 \
-                val = scol ? WOUT[0] : 0.0;                                                       \
-            } else if (r == 1) {                                                                  \
-                val = (ISC && chan) ? cslot : 0.0;                                                \
+                RINGW                                                                             \
             }                                                                                     \
             WOUT.push(val);                                                                       \
-            RINGW                                                                                 \
             if (OUTP != nullptr && interior && out_col && r >= ilo && r < ihi)                    \
                 OUTP[(size_t)r * su + j] = val;                                                   \
         }
@@ -543,12 +577,8 @@ __global__ __launch_bounds__(NT2, 1) void k_scal(K2Args a)
         SCAL_STAGE(1, ct, sCT[s3(t - 1)], true, c1, { sC1[s2(t - 1)][PAD + x] = val; }, noout, kc, dC, ct)
         SCAL_STAGE(2, z1, sZ1[s2(t - 2)], false, z2, { sZ2[s2(t - 2)][PAD + x] = val; }, noout, kt, dT, tt)
         SCAL_STAGE(2, c1, sC1[s2(t - 2)], true, c2, { sC2[s2(t - 2)][PAD + x] = val; }, noout, kc, dC, ct)
-        {
-            Win<3> z3dummy, c3dummy;
-            z3dummy.fill(0.0); c3dummy.fill(0.0);
-            SCAL_STAGE(3, z2, sZ2[s2(t - 3)], false, z3dummy, {}, a.Tn, kt, dT, tt)
-            SCAL_STAGE(3, c2, sC2[s2(t - 3)], true, c3dummy, {}, a.Cn, kc, dC, ct)
-        }
+        SCAL_STAGE(3, z2, sZ2[s2(t - 3)], false, z3, {}, a.Tn, kt, dT, tt)
+        SCAL_STAGE(3, c2, sC2[s2(t - 3)], true, c3, {}, a.Cn, kc, dC, ct)
 #undef SCAL_STAGE
     }
 }
@@ -558,7 +588,19 @@ extern "C" {
 
 int cfdk_fused_available(void) { return 1; }
 
-int cfdk_factors(const cfd_devgeo *g, cfd_devfactors *f, hipStream_t s);
+// Row-block height: `want` > 0 (from the autotuner) or one block per CU, at least min_rows.
+static int pick_rows(int rows, int nbj, int min_rows, int want, int *hb)
+{
+    if (want > 0) {
+        *hb = want < rows ? want : rows;
+        return cdiv(rows, *hb);
+    }
+    int nbi = 256 / nbj;
+    if (nbi < 1) nbi = 1;
+    if (rows / nbi < min_rows) nbi = rows / min_rows > 0 ? rows / min_rows : 1;
+    *hb = cdiv(rows, nbi);
+    return cdiv(rows, *hb);
+}
 
 int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P,
                     const cfd_fused_args *a, int kernel, hipStream_t s)
@@ -569,47 +611,53 @@ int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_para
     // K1
     K1Args k1;
     k1.um = a->um; k1.vm = a->vm; k1.ut = a->um_tau; k1.vt = a->vm_tau; k1.p = a->p; k1.T = a->T;
-    k1.un = a->um_n; k1.vn = a->vm_n; k1.pn = a->pn; k1.res = (unsigned long long *)a->res_max;
     k1.eps = g->eps; k1.lp = g->lp;
-    k1.aun = g->aun; k1.aue = g->aue; k1.avn = g->avn; k1.ave = g->ave;
-    k1.rdx = f->rdx; k1.rdy = f->rdy; k1.rdxm = f->rdxm; k1.rdym = f->rdym;
-    k1.imax = im; k1.jmax = jm; k1.P = *P;
+    k1.un = a->um_n; k1.vn = a->vm_n; k1.pn = a->pn; k1.res = (unsigned long long *)a->res_max;
+    k1.pk = f->pack;
+    k1.imax = im; k1.jmax = jm;
+    k1.chan = P->bc == CFD_BC_CHANNEL; k1.upwind_all = P->scheme == CFD_SCHEME_UPWIND;
+    k1.bouss = P->buoyancy == CFD_BUOY_BOUSSINESQ;
+    k1.dt = P->dt; k1.dtau = P->dtau; k1.beta = P->beta; k1.inv_re = 1.0 / P->re;
+    k1.inv_reda = 1.0 / (P->re * P->darcy); k1.cfsd = P->cf / sqrt(P->darcy); k1.g = P->g;
+    k1.invfr2 = P->invfr2; k1.ri = P->ri; k1.t_ref = P->t_ref; k1.v_i = P->v_i; k1.u_in = P->u_in;
+    for (int q = 0; q < 5; ++q) k1.mu[q] = P->mu_coef[q];
     k1.nbj = cdiv(jm - 2, W1);
-    {
-        int nbi = 256 / k1.nbj;
-        const int rows = im - 2;
-        if (nbi < 1) nbi = 1;
-        if (rows / nbi < 24) nbi = rows / 24 > 0 ? rows / 24 : 1;
-        k1.hb = cdiv(rows, nbi);
-        k1.nbi = cdiv(rows, k1.hb);
-    }
+    k1.nbi = pick_rows(im - 2, k1.nbj, 24, a->hb1, &k1.hb);
     const int G1 = k1.nbi * k1.nbj, grid1 = ((G1 + 7) / 8) * 8;
-    if (P->steady) hipLaunchKernelGGL(k_uvp<true>, dim3(grid1), dim3(NT1), 0, s, k1);
-    else hipLaunchKernelGGL(k_uvp<false>, dim3(grid1), dim3(NT1), 0, s, k1);
+    const bool mp = P->mu_model == CFD_MU_POLY4;
+    const bool timed = a->ev[0] != nullptr;
+    if (timed) hipEventRecord(a->ev[0], s);
+    if (P->steady) {
+        if (mp) hipLaunchKernelGGL((k_uvp<true, true>), dim3(grid1), dim3(NT1), 0, s, k1);
+        else hipLaunchKernelGGL((k_uvp<true, false>), dim3(grid1), dim3(NT1), 0, s, k1);
+    } else {
+        if (mp) hipLaunchKernelGGL((k_uvp<false, true>), dim3(grid1), dim3(NT1), 0, s, k1);
+        else hipLaunchKernelGGL((k_uvp<false, false>), dim3(grid1), dim3(NT1), 0, s, k1);
+    }
     int r = launched();
     if (r != CFD_OK) return r;
+    if (timed) hipEventRecord(a->ev[1], s);
     // K2
     K2Args k2;
     k2.un = a->um_n; k2.vn = a->vm_n; k2.T = a->T; k2.Tt = a->T_tau; k2.C = a->C; k2.Ct = a->C_tau;
-    k2.Tn = a->T_n; k2.Cn = a->C_n;
     k2.eps = g->eps; k2.lp = g->lp; k2.y = g->y;
-    k2.aue = g->aue; k2.avn = g->avn; k2.rdx = f->rdx; k2.rdy = f->rdy; k2.rdxm = f->rdxm; k2.rdym = f->rdym;
-    k2.imax = im; k2.jmax = jm; k2.P = *P;
+    k2.Tn = a->T_n; k2.Cn = a->C_n;
+    k2.pk = f->pack;
+    k2.imax = im; k2.jmax = jm; k2.chan = P->bc == CFD_BC_CHANNEL;
+    k2.dt = P->dt; k2.dtau = P->dtau; k2.kt = 1.0 / P->pe; k2.kc = 1.0 / P->re / P->sc;
+    k2.tinf = P->tinf; k2.t_bottom = P->t_bottom; k2.t_top = P->t_top; k2.v_i = P->v_i; k2.u_in = P->u_in;
+    k2.slot_lo = P->slot_lo; k2.slot_hi = P->slot_hi; k2.slot_value = P->slot_value;
     k2.nbj = cdiv(jm - 2, W2);
-    {
-        int nbi = 256 / k2.nbj;
-        const int rows = im - 2;
-        if (nbi < 1) nbi = 1;
-        if (rows / nbi < 16) nbi = rows / 16 > 0 ? rows / 16 : 1;
-        k2.hb = cdiv(rows, nbi);
-        k2.nbi = cdiv(rows, k2.hb);
-    }
+    k2.nbi = pick_rows(im - 2, k2.nbj, 16, a->hb2, &k2.hb);
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
     if (P->steady) hipLaunchKernelGGL(k_scal<true>, dim3(grid2), dim3(NT2), 0, s, k2);
     else hipLaunchKernelGGL(k_scal<false>, dim3(grid2), dim3(NT2), 0, s, k2);
     r = launched();
     if (r != CFD_OK) return r;
-    return cfdk_materialize_bc(g, P, a->um_n, a->vm_n, a->pn, a->T_n, a->C_n, a->vm_tau, s);
+    if (timed) hipEventRecord(a->ev[2], s);
+    r = cfdk_materialize_bc(g, P, a->um_n, a->vm_n, a->pn, a->T_n, a->C_n, a->vm_tau, s);
+    if (timed) hipEventRecord(a->ev[3], s);
+    return r;
 }
 
 }  // extern "C"
@@ -618,13 +666,31 @@ int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_para
 // support launchers
 // ---------------------------------------------------------------------------------------
 namespace {
-__global__ void k_factors(Geo g, double *rdx, double *rdy, double *rdxm, double *rdym)
+__global__ void k_factors(Geo g, double *rdx, double *rdy, double *rdxm, double *rdym, double *pack)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 && t <= g.imax) rdx[t] = 1.0 / (g.x[t] - g.x[t - 1]);
-    if (t >= 2 && t <= g.jmax) rdy[t] = 1.0 / (g.y[t] - g.y[t - 1]);
-    if (t >= 2 && t <= g.imax + 1) rdxm[t] = 1.0 / (g.xm[t] - g.xm[t - 1]);
-    if (t >= 2 && t <= g.jmax + 1) rdym[t] = 1.0 / (g.ym[t] - g.ym[t - 1]);
+    const int R = g.imax + 2, Cc = g.jmax + 2;
+    double *pr = pack, *pc = pack + 4 * R;
+    if (t < R) {
+        const double a = (t >= 2 && t <= g.imax) ? 1.0 / (g.x[t] - g.x[t - 1]) : 0.0;
+        const double b = (t >= 2 && t <= g.imax + 1) ? 1.0 / (g.xm[t] - g.xm[t - 1]) : 0.0;
+        if (t >= 2 && t <= g.imax) rdx[t] = a;
+        if (t >= 2 && t <= g.imax + 1) rdxm[t] = b;
+        pr[t] = (t >= 2 && t <= g.imax) ? g.aun[t] : 0.0;
+        pr[R + t] = (t >= 1 && t <= g.imax) ? g.avn[t] : 0.0;
+        pr[2 * R + t] = a;
+        pr[3 * R + t] = b;
+    }
+    if (t < Cc) {
+        const double a = (t >= 2 && t <= g.jmax) ? 1.0 / (g.y[t] - g.y[t - 1]) : 0.0;
+        const double b = (t >= 2 && t <= g.jmax + 1) ? 1.0 / (g.ym[t] - g.ym[t - 1]) : 0.0;
+        if (t >= 2 && t <= g.jmax) rdy[t] = a;
+        if (t >= 2 && t <= g.jmax + 1) rdym[t] = b;
+        pc[t] = (t >= 1 && t <= g.jmax) ? g.aue[t] : 0.0;
+        pc[Cc + t] = (t >= 2 && t <= g.jmax) ? g.ave[t] : 0.0;
+        pc[2 * Cc + t] = a;
+        pc[3 * Cc + t] = b;
+    }
 }
 
 // Boundary cells of the five outputs of a fused step, from their interior values. The
@@ -679,7 +745,7 @@ extern "C" {
 int cfdk_factors(const cfd_devgeo *g, cfd_devfactors *f, hipStream_t s)
 {
     const int n = (g->imax > g->jmax ? g->imax : g->jmax) + 2;
-    hipLaunchKernelGGL(k_factors, dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), f->rdx, f->rdy, f->rdxm, f->rdym);
+    hipLaunchKernelGGL(k_factors, dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), f->rdx, f->rdy, f->rdxm, f->rdym, f->pack);
     return launched();
 }
 

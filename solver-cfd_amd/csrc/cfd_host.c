@@ -10,7 +10,7 @@
  *            one kernel per operator; used for the first step after new inputs (when the
  *            iterates may violate the boundary conditions) and by the legacy names;
  *   fused  — cfdk_fused_step on boundary-consistent iterates.
- * The tau-level update of main.f90:292-296 is a pointer swap.
+ * The tau-level update of main.f90:145-149 is a pointer swap.
  */
 #include "cfd_internal.h"
 #include "../../include/cfd_legacy.h"
@@ -152,9 +152,14 @@ struct cfd_ctx {
     int64_t hist_cap;
     uint64_t *h_hist;      /* pinned mirror */
     hipEvent_t ev0, ev1;
+    hipEvent_t evk[4];     /* per-kernel timing events, created on first use */
     double last_ms_per_step;
     int64_t n_fused, n_exact;
+    int32_t hb1, hb2;      /* fused row-block heights (0: built-in choice) */
+    int tiles_set;         /* hb1/hb2 chosen by the caller or the autotuner */
 };
+
+static void tiles_from_cache(cfd_ctx *c);
 
 static size_t kind_count(const cfd_ctx *c, char kind)
 {
@@ -197,6 +202,7 @@ static int ctx_alloc(cfd_ctx *c)
     CHK(dalloc(&g->eps, c->np)); CHK(dalloc(&g->lp, c->np));
     CHK(dalloc(&c->fac.rdx, im + 2)); CHK(dalloc(&c->fac.rdy, jm + 2));
     CHK(dalloc(&c->fac.rdxm, im + 2)); CHK(dalloc(&c->fac.rdym, jm + 2));
+    CHK(dalloc(&c->fac.pack, (size_t)4 * (im + 2) + (size_t)4 * (jm + 2)));
     static const char kinds[S_NSTATE] = {'u', 'v', 'u', 'v', 'u', 'v', 'p', 'p', 'p', 'p', 'p', 'p', 'p', 'p'};
     for (int k = 0; k < S_NSTATE; ++k) CHK(dalloc(&c->st[k], kind_count(c, kinds[k])));
     static const char skinds[X_NSCR] = {'u', 'v', 'p', 'p', 'u', 'v', 'p', 'p', 'p', 'p', 'u', 'v', 'p',
@@ -214,7 +220,8 @@ static void ctx_free(cfd_ctx *c)
 {
     cfd_devgeo *g = &c->g;
     double *geo[] = {g->x, g->y, g->xm, g->ym, g->aun, g->aus, g->aue, g->auw, g->avn, g->avs,
-                     g->ave, g->avw, g->eps, g->lp, c->fac.rdx, c->fac.rdy, c->fac.rdxm, c->fac.rdym};
+                     g->ave, g->avw, g->eps, g->lp, c->fac.rdx, c->fac.rdy, c->fac.rdxm, c->fac.rdym,
+                     c->fac.pack};
     for (size_t k = 0; k < sizeof geo / sizeof geo[0]; ++k) if (geo[k]) hipFree(geo[k]);
     for (int k = 0; k < S_NSTATE; ++k) if (c->st[k]) hipFree(c->st[k]);
     for (int k = 0; k < X_NSCR; ++k) if (c->scr[k]) hipFree(c->scr[k]);
@@ -222,6 +229,7 @@ static void ctx_free(cfd_ctx *c)
     if (c->h_hist) hipHostFree(c->h_hist);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    for (int k = 0; k < 4; ++k) if (c->evk[k]) hipEventDestroy(c->evk[k]);
     if (c->owns_stream && c->s) hipStreamDestroy(c->s);
 }
 
@@ -242,6 +250,7 @@ static int ctx_init(cfd_ctx *c, const cfd_params *params, int32_t precision, int
     c->first_step = 1;
     c->have_fused = cfdk_fused_available();
     c->kernel = 1;
+    tiles_from_cache(c);
     return ctx_alloc(c);
 }
 
@@ -540,7 +549,7 @@ static void tau_swap(cfd_ctx *c)
     c->pending_in_tau = 1;
 }
 
-/* One pseudo-step (main.f90:269-280) through the per-operator kernels; residuals into slot[0..2]. */
+/* One pseudo-step (main.f90:122-142) through the per-operator kernels; residuals into slot[0..2]. */
 static int step_exact(cfd_ctx *c, uint64_t *slot)
 {
     xenv e = {&c->g, &c->P, c->s};
@@ -577,8 +586,9 @@ static int step_exact(cfd_ctx *c, uint64_t *slot)
     return CFD_OK;
 }
 
-/* Fused step: boundary-consistent iterates in, outputs with unmaterialised boundaries. */
-static int step_fused(cfd_ctx *c, uint64_t *slot)
+/* Fused step: boundary-consistent iterates in, outputs with boundaries written last.
+ * With `timed`, the context's per-kernel events bracket each launch. */
+static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
 {
     const int st = c->P.steady;
     double **S = c->st, **X = c->scr;
@@ -595,6 +605,8 @@ static int step_fused(cfd_ctx *c, uint64_t *slot)
     a.u1 = X[X_U1]; a.u2 = X[X_U2]; a.v1 = X[X_V1]; a.v2 = X[X_V2];
     a.z1 = X[X_Z1]; a.z2 = X[X_Z2]; a.c1 = X[X_C1]; a.c2 = X[X_C2];
     a.res_max = slot;
+    a.hb1 = c->hb1; a.hb2 = c->hb2;
+    if (timed) for (int k = 0; k < 4; ++k) a.ev[k] = c->evk[k];
     CHK(cfdk_zero_u64(slot, 3, c->s));
     CHK(cfdk_fused_step(&c->g, &c->fac, &c->P, &a, c->kernel, c->s));
     c->bc_clean = 1;          /* cfdk_fused_step writes the boundary cells last */
@@ -602,6 +614,8 @@ static int step_fused(cfd_ctx *c, uint64_t *slot)
     c->n_fused++;
     return CFD_OK;
 }
+
+static int step_fused(cfd_ctx *c, uint64_t *slot) { return step_fused_ev(c, slot, 0); }
 
 static int materialize(cfd_ctx *c)
 {
@@ -735,7 +749,7 @@ int cfd_run_to_tol(cfd_ctx *c, double tol, int64_t itc_max, int32_t check_every,
                 (long long)done, fmax(fmax(r[0], r[1]), r[2]));
 }
 
-/* main.f90:263-305, one pass of the physical-time loop body. */
+/* main.f90:116-157, one pass of the physical-time loop body. */
 int cfd_physical_step(cfd_ctx *c, double eps, int64_t itc_max, int32_t first, int64_t *iters, double res[3])
 {
     CHK(check_ready(c));
@@ -825,6 +839,178 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
     if (res) memcpy(res, rr, sizeof rr);
     for (int q = 0; q < 3; ++q)
         if (!isfinite(rr[q])) return fail(CFD_ERR_NONFINITE, "residual %d is not finite", q);
+    return CFD_OK;
+}
+
+/* ---- tile shapes and the autotuner ------------------------------------------------- */
+/* Tuned row-block heights, kept per grid class for the life of the process. */
+typedef struct tile_entry {
+    int32_t imax, jmax, prec, steady, mu_model, scheme, bc, buoyancy;
+    int32_t hb1, hb2;
+} tile_entry;
+enum { TILE_CACHE = 32 };
+static tile_entry g_tiles[TILE_CACHE];
+static int g_ntiles;
+
+static int tile_key_eq(const tile_entry *e, const cfd_ctx *c)
+{
+    const cfd_params *P = &c->P;
+    return e->imax == P->imax && e->jmax == P->jmax && e->prec == c->prec && e->steady == P->steady
+        && e->mu_model == P->mu_model && e->scheme == P->scheme && e->bc == P->bc && e->buoyancy == P->buoyancy;
+}
+
+static void tiles_from_cache(cfd_ctx *c)
+{
+    for (int k = 0; k < g_ntiles; ++k)
+        if (tile_key_eq(&g_tiles[k], c)) { c->hb1 = g_tiles[k].hb1; c->hb2 = g_tiles[k].hb2; c->tiles_set = 1; return; }
+}
+
+static void tiles_to_cache(const cfd_ctx *c)
+{
+    int k = 0;
+    while (k < g_ntiles && !tile_key_eq(&g_tiles[k], c)) ++k;
+    if (k == g_ntiles) {
+        if (g_ntiles < TILE_CACHE) ++g_ntiles;
+        else k = TILE_CACHE - 1;          /* full: overwrite the last entry */
+    }
+    const cfd_params *P = &c->P;
+    tile_entry e = {P->imax, P->jmax, c->prec, P->steady, P->mu_model, P->scheme, P->bc, P->buoyancy, c->hb1, c->hb2};
+    g_tiles[k] = e;
+}
+
+int cfd_ctx_set_tiles(cfd_ctx *c, int32_t hb1, int32_t hb2)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (hb1 < 0 || hb2 < 0 || hb1 > c->P.imax || hb2 > c->P.imax)
+        return fail(CFD_ERR_ARG, "row-block heights must lie in [0, imax], got %d, %d", hb1, hb2);
+    c->hb1 = hb1;
+    c->hb2 = hb2;
+    c->tiles_set = 1;
+    return CFD_OK;
+}
+
+int cfd_ctx_get_tiles(const cfd_ctx *c, int32_t *hb1, int32_t *hb2)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (hb1) *hb1 = c->hb1;
+    if (hb2) *hb2 = c->hb2;
+    return CFD_OK;
+}
+
+/* Mean ms per fused step over `steps` steps (after one untimed step) with the given tiles. */
+static int time_tiles(cfd_ctx *c, int32_t hb1, int32_t hb2, int32_t steps, double *ms_out)
+{
+    c->hb1 = hb1;
+    c->hb2 = hb2;
+    CHK(step_fused(c, c->d_hist));
+    tau_swap(c);
+    HIPCHK(hipEventRecord(c->ev0, c->s));
+    for (int32_t k = 0; k < steps; ++k) {
+        CHK(step_fused(c, c->d_hist));
+        tau_swap(c);
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->s));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_out = (double)ms / steps;
+    return CFD_OK;
+}
+
+int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, int32_t *hb1_out, int32_t *hb2_out, double *out_ms)
+{
+    CHK(check_ready(c));
+    if (steps < 1) steps = 1;
+    if (!(c->kernel == 1 && c->have_fused && c->prec == CFD_PREC_FP64 && c->P.b_art == 0.0
+          && c->P.compat == CFD_COMPAT_FORTRAN))
+        return fail(CFD_ERR_UNSUPPORTED, "autotune: this context does not run the fused kernels");
+    CHK(materialize(c));
+    /* snapshot: every state buffer's contents, the buffer roles and the step flags */
+    static const char kinds[S_NSTATE] = {'u', 'v', 'u', 'v', 'u', 'v', 'p', 'p', 'p', 'p', 'p', 'p', 'p', 'p'};
+    size_t total = 0, off[S_NSTATE];
+    for (int k = 0; k < S_NSTATE; ++k) { off[k] = total; total += kind_count(c, kinds[k]); }
+    double *snap = NULL;
+    HIPCHK(hipMalloc((void **)&snap, total * sizeof(double)));
+    double *roles[S_NSTATE];
+    memcpy(roles, c->st, sizeof roles);
+    const int f_bc = c->bc_clean, f_uv = c->uv_reads_pn, f_first = c->first_step, f_pend = c->pending_in_tau;
+    const int64_t n_f = c->n_fused, n_e = c->n_exact;
+    const double last_ms = c->last_ms_per_step;
+    int r = CFD_OK;
+    for (int k = 0; k < S_NSTATE && r == CFD_OK; ++k)
+        r = cfdk_copy(snap + off[k], c->st[k], (long)kind_count(c, kinds[k]), c->s);
+    /* the fused kernels need boundary-consistent iterates: one exact step if not there yet */
+    if (r == CFD_OK && !fused_ok(c)) {
+        r = step_exact(c, c->d_hist);
+        if (r == CFD_OK) tau_swap(c);
+    }
+    const int rows = c->P.imax - 2;
+    static const int32_t cand1[] = {0, 16, 24, 32, 48, 64, 96, 128};
+    static const int32_t cand2[] = {0, 8, 12, 16, 24, 32, 48, 64};
+    int32_t best1 = 0, best2 = 0;
+    double best = INFINITY, t = 0.0;
+    /* the two kernels are independent: tune K1's height with K2's default, then K2's */
+    for (size_t q = 0; q < sizeof cand1 / sizeof cand1[0] && r == CFD_OK; ++q) {
+        if (cand1[q] > rows) continue;
+        r = time_tiles(c, cand1[q], 0, steps, &t);
+        if (r == CFD_OK && t < best) { best = t; best1 = cand1[q]; }
+    }
+    for (size_t q = 0; q < sizeof cand2 / sizeof cand2[0] && r == CFD_OK; ++q) {
+        if (cand2[q] > rows) continue;
+        r = time_tiles(c, best1, cand2[q], steps, &t);
+        if (r == CFD_OK && t < best) { best = t; best2 = cand2[q]; }
+    }
+    /* restore the state exactly as it was */
+    memcpy(c->st, roles, sizeof roles);
+    for (int k = 0; k < S_NSTATE; ++k) {
+        const int rr = cfdk_copy(c->st[k], snap + off[k], (long)kind_count(c, kinds[k]), c->s);
+        if (r == CFD_OK) r = rr;
+    }
+    hipStreamSynchronize(c->s);
+    hipFree(snap);
+    c->bc_clean = f_bc; c->uv_reads_pn = f_uv; c->first_step = f_first; c->pending_in_tau = f_pend;
+    c->n_fused = n_f; c->n_exact = n_e;
+    c->last_ms_per_step = last_ms;
+    if (r != CFD_OK) return r;
+    c->hb1 = best1;
+    c->hb2 = best2;
+    c->tiles_set = 1;
+    tiles_to_cache(c);
+    if (hb1_out) *hb1_out = best1;
+    if (hb2_out) *hb2_out = best2;
+    if (out_ms) *out_ms = best;
+    return CFD_OK;
+}
+
+int cfd_pseudo_steps_timed(cfd_ctx *c, int32_t nsteps, double res[3], double ms[4])
+{
+    CHK(check_ready(c));
+    if (nsteps < 1) return fail(CFD_ERR_ARG, "nsteps < 1");
+    if (!ms) return fail(CFD_ERR_ARG, "ms is NULL");
+    for (int k = 0; k < 4; ++k)
+        if (!c->evk[k]) HIPCHK(hipEventCreate(&c->evk[k]));
+    CHK(ensure_hist(c, 1));
+    double acc[4] = {0, 0, 0, 0};
+    int32_t nf = 0;
+    for (int32_t k = 0; k < nsteps; ++k) {
+        if (fused_ok(c)) {
+            CHK(step_fused_ev(c, c->d_hist, 1));
+            tau_swap(c);
+            HIPCHK(hipEventSynchronize(c->evk[3]));
+            float t[3];
+            for (int q = 0; q < 3; ++q) HIPCHK(hipEventElapsedTime(&t[q], c->evk[q], c->evk[q + 1]));
+            acc[0] += t[0]; acc[1] += t[1]; acc[2] += t[2]; acc[3] += (double)t[0] + t[1] + t[2];
+            ++nf;
+        } else {
+            CHK(one_step(c, c->d_hist));
+            tau_swap(c);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int q = 0; q < 4; ++q) ms[q] = nf ? acc[q] / nf : 0.0;
+    if (res) for (int q = 0; q < 3; ++q) res[q] = u64_to_res(c->h_hist[q]);
+    if (!nf) return fail(CFD_ERR_UNSUPPORTED, "no step took the fused path");
     return CFD_OK;
 }
 

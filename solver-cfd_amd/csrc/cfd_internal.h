@@ -63,6 +63,10 @@ typedef struct cfd_fused_args {
     double *u1, *u2, *v1, *v2, *z1, *z2, *c1, *c2;
     /* residual accumulators: res_max[0..2] (uint64 bit patterns of max|r3| for u, v, p) */
     uint64_t *res_max;
+    /* row-block heights of K1 / K2 (0: built-in heuristic); chosen by the autotuner */
+    int32_t hb1, hb2;
+    /* optional events recorded before K1, after K1, after K2 and after the boundary pass */
+    hipEvent_t ev[4];
 } cfd_fused_args;
 
 /* Precomputed 1-D geometry factors for the fused kernels (allocated by the host). */
@@ -71,6 +75,9 @@ typedef struct cfd_devfactors {
     double *rdy;     /* 1/(y_j - y_{j-1})          j = 2..jmax   */
     double *rdxm;    /* 1/(xm_i - xm_{i-1})        i = 2..imax+1 */
     double *rdym;    /* 1/(ym_j - ym_{j-1})        j = 2..jmax+1 */
+    /* packed copy for the fused kernels, 4*(imax+2) + 4*(jmax+2) doubles:
+     * rows [aun | avn | rdx | rdxm] then columns [aue | ave | rdy | rdym] */
+    double *pack;
 } cfd_devfactors;
 
 /* 1 when this build has the fused kernels. */

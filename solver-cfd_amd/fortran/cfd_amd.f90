@@ -1,7 +1,7 @@
 ! cfd_amd.f90 -- iso_c_binding interface to libcfd_amd.so for a Fortran driver.
 !
 ! A Fortran main program like OpenMp_Multicore/main.f90 replaces its calls to
-! solve_U/solve_V/solve_P/solve_Z/solve_C (main.f90:269-279) by one context and
+! solve_U/solve_V/solve_P/solve_Z/solve_C (main.f90:122-132) by one context and
 ! cfd_pseudo_steps / cfd_physical_step, passing its own column-major arrays with
 ! CFD_LAYOUT_FORTRAN: um(1:imax+1,1:jmax), vm(1:imax,1:jmax+1), p/T/C(1:imax,1:jmax).
 ! The derived type mirrors struct cfd_params of include/cfd.h field by field.

@@ -145,3 +145,52 @@ def test_graph_replays_match_eager():
         assert np.array_equal(a, b), k
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("tiles", [(16, 8), (37, 11), (64, 64)])
+def test_fused_explicit_tiles_match_oracle(tiles):
+    """Row-block heights other than the built-in choice (including ones that do not divide
+    the row count) give the same trajectory."""
+    say("case: explicit tiles", tiles)
+    prm, x, y, eps, lp, st = _multiblock_case(180, 300, bc=0, invfr2=0.7, g=1.5)
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    h_ref = o.pseudo_steps(s, cs, 30)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_tiles(*tiles)
+    assert ctx.get_tiles() == tiles
+    ctx.set_state(st)
+    res, hist = ctx.pseudo_steps(30, history=True)
+    errs = {k: max_rel(ctx.download(k), s[k]) for k in FIELDS}
+    say(tiles, max_rel(hist, h_ref), errs)
+    assert ctx.step_counts() == (29, 1)
+    assert max_rel(hist, h_ref) <= TOL and max(errs.values()) <= TOL
+    ctx.close()
+
+
+def test_autotune_restores_state_and_keeps_parity():
+    """The autotuner leaves the state, the step counters and the first-step semantics as they
+    were, picks heights from its candidate set, and later contexts of the grid reuse them."""
+    say("case: autotune")
+    prm, x, y, eps, lp, st = _multiblock_case(200, 280, bc=BC_CHANNEL, invfr2=0.0)
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    h_ref = o.pseudo_steps(s, cs, 20)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_state(st)
+    before = {k: ctx.download(k) for k in FIELDS}
+    hb1, hb2, ms = ctx.autotune(steps=3)
+    say("tuned", hb1, hb2, ms)
+    assert hb1 in (0, 16, 24, 32, 48, 64, 96, 128) and hb2 in (0, 8, 12, 16, 24, 32, 48, 64)
+    assert ms > 0 and ctx.get_tiles() == (hb1, hb2)
+    assert ctx.step_counts() == (0, 0)
+    for k in FIELDS:
+        assert np.array_equal(ctx.download(k), before[k]), k
+    res, hist = ctx.pseudo_steps(20, history=True)
+    errs = {k: max_rel(ctx.download(k), s[k]) for k in FIELDS}
+    assert ctx.step_counts() == (19, 1)
+    assert max_rel(hist, h_ref) <= TOL and max(errs.values()) <= TOL
+    ctx.close()
+    ctx2 = L.Context(prm, x, y, eps, lp)
+    assert ctx2.get_tiles() == (hb1, hb2)
+    ctx2.close()

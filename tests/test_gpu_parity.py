@@ -230,7 +230,7 @@ def test_itc_max_exhaustion_and_divergence_guard():
 
 def test_run_to_tol_matches_reference_exit_step():
     """cfd_run_to_tol with check_every=1 stops at the first step with err < tol (and
-    itc != 1), as main.f90:289 does; the oracle gives the same step."""
+    itc != 1), as main.f90:142 does; the oracle gives the same step."""
     prm, x, y, eps, lp, st, out, meta = load_golden("steps24")
     tol = float(np.max(out["hist"], axis=1)[150])
     o = O.Oracle(prm, x, y, eps, lp)
