@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: pseudo-time iteration throughput (cell-updates/s) on one MI355X.
 
-Workload (default): configuration 3a — 2048 x 2048 tanh-clustered mesh, QUICK, porous
+Workload (default): full physics on a 2048 x 2048 tanh-clustered mesh — QUICK, porous
 middle-third block, Boussinesq Ri=2, species slot, Sc=0.7, mu(T) polynomial; FP64 drop-in.
 A "step" is one pseudo-step (three SSP-RK3 stages of u, v, p, T, C with their boundary
 conditions and the residual maxima) over the whole grid; one cell-update is one pseudo-step
@@ -72,7 +72,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c3", choices=["c3", "c2"])
+    ap.add_argument("--case", default="full", choices=["full", "porous"],
+                    help="full: full physics on a tanh mesh; porous: uniform mesh, porous block, buoyancy")
     ap.add_argument("--imax", type=int, default=2048)
     ap.add_argument("--jmax", type=int, default=2048)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
@@ -80,7 +81,7 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: eager)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_c3_2048x2048_fp64.json"))
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_full_physics_2048x2048_fp64.json"))
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in tile shapes")
     ap.add_argument("--tiles", default="", help="hb1,hb2: set the row-block heights instead of autotuning")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
@@ -94,10 +95,10 @@ def main():
         sys.exit(2)
 
     prec = PREC_FP64 if args.precision == "fp64" else PREC_FP32
-    if args.config == "c3":
-        cfg = C.config_c3(args.imax, args.jmax)
+    if args.case == "full":
+        cfg = C.case_full_physics(args.imax, args.jmax)
     else:
-        cfg = C.config_c2(args.imax)
+        cfg = C.case_porous_buoyant(args.imax)
     prm = cfg["params"]
     cells = prm.imax * prm.jmax
     log("workload %s %dx%d %s, building context" % (cfg["name"], prm.imax, prm.jmax, args.precision))
@@ -166,7 +167,7 @@ def main():
         "steps": steps, "warmup": args.warmup, "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == PREC_FP64 else "f32+f64",
-        "data": "synthetic (seeded SplitMix64 inputs of configuration 3: mesh, porosity, state)",
+        "data": "synthetic (seeded SplitMix64 inputs: tanh mesh, porous block, perturbed state)",
         "config": {"workload": "%s fused pseudo-step, %s" % (cfg["name"], args.precision),
                    "imax": prm.imax, "jmax": prm.jmax, "kernel": args.kernel,
                    "graph_steps": args.graph, "tiles": tiles,

@@ -21,8 +21,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("CFD_OFFLOAD_ARCH", "gfx950")
 
-# packed FP32 instructions are kept out of every kernel (they have given
-# wrong results on these boxes); the host pass warns that it does not know the feature.
+# packed FP32 instructions (v_pk_*_f32) are kept out of every kernel: some of their forms
+# have given silently wrong results on MI355X. The host pass warns that it does not know
+# the feature; that is harmless.
 NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wno-unused-command-line-argument",
               "-I" + os.path.join(REPO, "include")] + NO_PACKED

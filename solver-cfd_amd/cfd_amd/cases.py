@@ -168,8 +168,8 @@ def stable_dt(x, y, nu, kappa, umax, dtau, safety=0.4):
     return (safety * 2.5 / dtau - 1.0) / lam
 
 
-# ---- BASELINE configurations ---------------------------------------------------------
-def config_c1():
+# ---- benchmark configurations ---------------------------------------------------------
+def case_channel_upwind():
     """51x51 unit square, left inflow u=1, right outflow, no-slip walls, Re=100, beta=1,
     global first-order upwind, no porosity/buoyancy/species; seeded +/-1e-3 perturbation."""
     n = 51
@@ -180,10 +180,10 @@ def config_c1():
     eps = zeros("p", n, n); eps[1:, 1:] = 1.0
     lp = zeros("p", n, n)
     st = perturbed_state(n, n, seed=1, amp=1e-3)
-    return dict(name="c1", params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
+    return dict(name="channel_upwind_51x51", params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
 
 
-def config_c2(n=204):
+def case_porous_buoyant(n=204):
     """204x204 uniform, QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2),
     Da=1e-3, Cf=0.55, Boussinesq Ri=1 with hot bottom (1) and cold top (0)."""
     x = uniform_nodes(n); y = uniform_nodes(n)
@@ -192,7 +192,7 @@ def config_c2(n=204):
                       u_in=1.0, t_bottom=1.0, t_top=0.0, slot_lo=1.0, slot_hi=0.0)
     eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
     st = perturbed_state(n, n, seed=2, amp=1e-3, T0=0.5)
-    return dict(name="c2", params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
+    return dict(name="porous_buoyant_%dx%d" % (n, n), params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
 
 
 def mu_poly_coefficients(seed=3):
@@ -202,7 +202,7 @@ def mu_poly_coefficients(seed=3):
     return tuple(base * (1.0 + uniform(seed, 5, -0.02, 0.02)))
 
 
-def config_c3(imax=2048, jmax=2048):
+def case_full_physics(imax=2048, jmax=2048):
     """Full physics at scale: tanh-clustered mesh (Q=2) toward the walls, QUICK, porous
     middle-third block (eps~U[0.4,0.9]), Boussinesq Ri=2, species slot Z=1 of width 0.1 on
     the left wall, Sc=0.7, mu(T) polynomial (seed 3)."""
@@ -218,4 +218,4 @@ def config_c3(imax=2048, jmax=2048):
                       mu_model=MU_POLY4, mu_coef=mu)
     eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
     st = perturbed_state(imax, jmax, seed=3, amp=1e-3, T0=0.5)
-    return dict(name="c3_%dx%d" % (imax, jmax), params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
+    return dict(name="full_physics_tanh_%dx%d" % (imax, jmax), params=prm, x=x, y=y, eps=eps, lp=lp, state=st)

@@ -200,7 +200,7 @@ def case_phys24():
     save("phys24", prm, x, y, eps, lp, st, dict(nphys=nphys, itc_max=itc_max, eps_conv=eps_conv), out)
 
 
-def case_c1ref51():
+def case_upwind_pe100_51():
     """Configuration 1's grid, Re, beta and seeded IC with the reference BC set and QUICK
     (the Fortran reference has no other BC set or global-upwind solve)."""
     n = 51
@@ -211,10 +211,10 @@ def case_c1ref51():
     lp = C.zeros("p", n, n)
     st = C.perturbed_state(n, n, seed=1, amp=1e-3, T0=1.0, C0=0.0)
     hd = dict(dtau=0.05, dt=0.005, beta=1.0, b_art=0.0, tinf=1.0)
-    case_steps("c1ref51", "c51", ov, x, y, eps, lp, st, 2000, hd)
+    case_steps("upwind_pe100_51", "c51", ov, x, y, eps, lp, st, 2000, hd)
 
 
-def case_c2ref48():
+def case_quick_porous48():
     """Configuration 2's physics (QUICK, Re=200, porous middle third eps~U[0.4,0.9] seed 2,
     Da=1e-3, Cf=0.55, buoyancy on) at 48x48 with the reference BC set and buoyancy form."""
     n = 48
@@ -224,7 +224,7 @@ def case_c2ref48():
     eps, lp = C.porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
     st = C.perturbed_state(n, n, seed=2, amp=1e-3, T0=1.5, C0=0.0)
     hd = dict(dtau=0.05, dt=0.005, beta=1.0, b_art=0.0, tinf=1.0)
-    case_steps("c2ref48", "c48", ov, x, y, eps, lp, st, 2000, hd)
+    case_steps("quick_porous48", "c48", ov, x, y, eps, lp, st, 2000, hd)
 
 
 def case_mesh51():
@@ -253,7 +253,7 @@ def case_mesh51():
 
 
 CASES = dict(ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
-             phys24=case_phys24, c1ref51=case_c1ref51, c2ref48=case_c2ref48, mesh51=case_mesh51)
+             phys24=case_phys24, upwind_pe100_51=case_upwind_pe100_51, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)

@@ -2,7 +2,7 @@
 golden vectors, the CPU oracle and the exact per-operator path.
 
 The fused kernels evaluate QUICK in flux form with precomputed reciprocals and FMA, so they
-differ from the literal reference arithmetic by rounding: the bar is the north-star 1e-10
+differ from the literal reference arithmetic by rounding: the bar is 1e-10
 max relative difference per field (after up to 2000 pseudo-steps).
 """
 import sys
@@ -17,7 +17,7 @@ from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSIN
 from helpers import load_golden, max_rel, field_rel
 
 pytestmark = pytest.mark.gpu
-TOL = 1e-10   # per field: max |a - b| / max |b|
+TOL = 1e-10   # per field: max |a - b| / max |b| (the FP64 drop-in bar)
 FIELDS = ("um_tau", "vm_tau", "p", "T_tau", "C_tau")
 
 
@@ -32,7 +32,7 @@ def _setup():
     L.load()
 
 
-@pytest.mark.parametrize("name", ["c1ref51", "c2ref48", "steady24"])
+@pytest.mark.parametrize("name", ["upwind_pe100_51", "quick_porous48", "steady24"])
 def test_fused_trajectory_matches_reference(name):
     say("case: fused trajectory vs Fortran golden", name)
     prm, x, y, eps, lp, st, out, meta = load_golden(name)
@@ -104,10 +104,10 @@ def test_fused_multiblock_channel_variants(variant):
 
 
 def test_fused_matches_exact_path_at_scale():
-    """Configuration 3 physics at 1024 x 512 (the second configuration-3 mesh): 40 steps of
+    """Full physics on the 1024 x 512 tanh mesh: 40 steps of
     the fused path against the exact per-operator path on the same GPU."""
-    say("case: c3 1024x512 fused vs exact, 40 steps")
-    cfg = C.config_c3(1024, 512)
+    say("case: full physics 1024x512 fused vs exact, 40 steps")
+    cfg = C.case_full_physics(1024, 512)
     outs = []
     for kernel in (0, 1):
         ctx = L.Context(cfg["params"], cfg["x"], cfg["y"], cfg["eps"], cfg["lp"])
@@ -119,7 +119,7 @@ def test_fused_matches_exact_path_at_scale():
     (fe, he, ce), (ff, hf, cf) = outs
     assert ce == (0, 40) and cf == (39, 1)
     errs = {k: max_rel(ff[k], fe[k]) for k in FIELDS}
-    say("c3 1024x512", max_rel(hf, he), errs)
+    say("full physics 1024x512", max_rel(hf, he), errs)
     assert max_rel(hf, he) <= TOL and max(errs.values()) <= TOL
 
 

@@ -2,7 +2,7 @@
 
 Bar: the drop-in (per-operator, reference call structure) path computes with one IEEE
 rounding per operation in the reference's order, so it is held to 1e-13 relative per array
-(bit equality is reported); the north-star bar for the FP64 drop-in is 1e-10 max relative
+(bit equality is reported); the bar for the FP64 drop-in is 1e-10 max relative
 difference per field after 2000 pseudo-steps.
 """
 import ctypes
@@ -117,7 +117,7 @@ def _ctx_from_golden(name, kernel=0):
     return ctx, prm, out, meta
 
 
-@pytest.mark.parametrize("name", ["steps24", "steady24", "c1ref51", "c2ref48"])
+@pytest.mark.parametrize("name", ["steps24", "steady24", "upwind_pe100_51", "quick_porous48"])
 def test_exact_path_trajectory_matches_reference(name):
     say("case: exact-path trajectory", name)
     ctx, prm, out, meta = _ctx_from_golden(name, kernel=0)
@@ -143,7 +143,7 @@ def test_physical_steps_match_reference():
     ctx.close()
 
 
-# ---- configurations beyond the reference's BC set: against the CPU oracle -------------------
+# ---- cases beyond the reference's BC set: against the CPU oracle -------------------
 def _compare_with_oracle(prm, x, y, eps, lp, st, nsteps, kernel=0, tol=1e-10):
     o = O.Oracle(prm, x, y, eps, lp)
     s, cs = o.make_state(st)
@@ -159,8 +159,8 @@ def _compare_with_oracle(prm, x, y, eps, lp, st, nsteps, kernel=0, tol=1e-10):
 
 
 def test_config1_channel_upwind_matches_oracle():
-    say("case: configuration 1 (channel BCs, global upwind), 2000 steps")
-    cfg = C.config_c1()
+    say("case: 51x51 channel BCs, global upwind, 2000 steps")
+    cfg = C.case_channel_upwind()
     h_err, errs = _compare_with_oracle(cfg["params"], cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], cfg["state"], 2000)
     say("c1", h_err, errs)
     assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
@@ -176,7 +176,7 @@ def test_boussinesq_mu_species_slot_match_oracle():
                       slot_lo=0.45, slot_hi=0.55, mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3))
     st = C.perturbed_state(n, m, seed=3, amp=1e-3, T0=0.5)
     h_err, errs = _compare_with_oracle(prm, x, y, eps, lp, st, 300)
-    say("c3-physics", h_err, errs)
+    say("full-physics", h_err, errs)
     assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
 
 
