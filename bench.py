@@ -83,7 +83,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_full_physics_2048x2048_fp64.json"))
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in tile shapes")
-    ap.add_argument("--tiles", default="", help="hb1,hb2: set the row-block heights instead of autotuning")
+    ap.add_argument("--tiles", default="",
+                    help="rows_uvp,rows_tc[,waves_uvp,waves_tc]: set the kernel shapes instead of autotuning")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
@@ -107,13 +108,17 @@ def main():
     ctx.set_state(cfg["state"])
     tiles = None
     if args.tiles:
-        hb1, hb2 = (int(v) for v in args.tiles.split(","))
-        ctx.set_tiles(hb1, hb2)
-        tiles = {"hb1": hb1, "hb2": hb2}
+        tt = tuple(int(v) for v in args.tiles.split(","))
+        ctx.set_tiles(*tt)
+        tiles = ctx.get_tiles()
     elif args.kernel == 1 and not args.no_autotune:
-        hb1, hb2, tms = ctx.autotune(steps=5)
-        tiles = {"hb1": hb1, "hb2": hb2}
-        log("autotune: row blocks u/v/p %d, T/C %d (0 = built-in), %.4f ms/step" % (hb1, hb2, tms))
+        tiles, tms = ctx.autotune(steps=5)
+        log("autotune: rows u/v/p %d, rows T/C %d, waves u/v/p %d, waves T/C %d (0 = built-in), %.4f ms/step"
+            % (tiles + (tms,)))
+    if tiles is not None:
+        # waves 0 means the built-in budget, which is the one-wave / as-compiled variant
+        tiles = dict(zip(("rows_uvp", "rows_tc", "waves_uvp", "waves_tc"),
+                         (tiles[0], tiles[1], tiles[2] or 1, tiles[3] or 1)))
     ctx.pseudo_steps(max(args.warmup, 1))
     log("warmup done (%d steps), timing %d steps" % (args.warmup, args.steps))
 

@@ -201,17 +201,26 @@ int  cfd_ctx_last_timing(const cfd_ctx *ctx, double *ms_per_step);
  * single-pass kernel). Mainly for tests that pin both against each other. */
 int  cfd_ctx_set_kernel(cfd_ctx *ctx, int32_t kernel);
 
-/* Tile shapes of the fused kernels: row-block heights of the u/v/p kernel (hb1) and of the
- * T/C kernel (hb2); 0 selects the built-in choice (one block per CU). */
-int  cfd_ctx_set_tiles(cfd_ctx *ctx, int32_t hb1, int32_t hb2);
-int  cfd_ctx_get_tiles(const cfd_ctx *ctx, int32_t *hb1, int32_t *hb2);
+/* Shapes of the fused kernels. Zeros select the built-in choice (one row block per CU,
+ * no register-budget override). */
+typedef struct cfd_tiles {
+    int32_t rows_uvp;    /* row-block height of the u/v/p kernel, 0 or 1..imax            */
+    int32_t rows_tc;     /* row-block height of the T/C kernel, 0 or 1..imax              */
+    int32_t waves_uvp;   /* register budget of the u/v/p kernel: 0|1 = one wave per SIMD,
+                            2 = two waves per SIMD (excess registers spilled to scratch) */
+    int32_t waves_tc;    /* register budget of the T/C kernel: 0|1 = as compiled (3 waves
+                            per SIMD), 4 = four waves per SIMD (with spills)              */
+} cfd_tiles;
 
-/* Autotuner: times `steps` fused pseudo-steps for each candidate pair of row-block heights
- * on a snapshot of the current state, restores the state, and keeps the fastest pair. The
- * result is cached per (imax, jmax, precision, model switches) for the process, so later
- * contexts of the same grid class start tuned. out_ms (may be NULL) receives the best mean
- * time per step. */
-int  cfd_ctx_autotune(cfd_ctx *ctx, int32_t steps, int32_t *hb1, int32_t *hb2, double *out_ms);
+int  cfd_ctx_set_tiles(cfd_ctx *ctx, const cfd_tiles *tiles);
+int  cfd_ctx_get_tiles(const cfd_ctx *ctx, cfd_tiles *tiles);
+
+/* Autotuner: times `steps` fused pseudo-steps for each candidate shape on a snapshot of the
+ * current state, restores the state (contents, step counters and first-step semantics),
+ * and keeps the fastest shape. The result is cached per (imax, jmax, precision, model
+ * switches) for the process, so later contexts of the same grid class start tuned.
+ * out (may be NULL) receives the shape, out_ms (may be NULL) its mean time per step. */
+int  cfd_ctx_autotune(cfd_ctx *ctx, int32_t steps, cfd_tiles *out, double *out_ms);
 
 /* Pseudo-steps taken by each path since the context was created: fused kernels and the
  * exact per-operator path (the latter runs the first step after new inputs, and every step

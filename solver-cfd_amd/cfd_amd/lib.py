@@ -33,6 +33,15 @@ EXPORTS_LEGACY = [
 ]
 
 
+class Tiles(ctypes.Structure):
+    """cfd_tiles of include/cfd.h."""
+    _fields_ = [("rows_uvp", ctypes.c_int32), ("rows_tc", ctypes.c_int32),
+                ("waves_uvp", ctypes.c_int32), ("waves_tc", ctypes.c_int32)]
+
+    def as_tuple(self):
+        return (self.rows_uvp, self.rows_tc, self.waves_uvp, self.waves_tc)
+
+
 class CfdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s: %s" % (ERRORS.get(code, code), msg))
@@ -75,10 +84,10 @@ def load():
         "cfd_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
         "cfd_ctx_set_kernel": (ctypes.c_int, [_vp, i32]),
         "cfd_ctx_step_counts": (ctypes.c_int, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
-        "cfd_ctx_set_tiles": (ctypes.c_int, [_vp, i32, i32]),
+        "cfd_ctx_set_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(Tiles)]),
         "cfd_pseudo_steps_timed": (ctypes.c_int, [_vp, i32, _dp, _dp]),
-        "cfd_ctx_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
-        "cfd_ctx_autotune": (ctypes.c_int, [_vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), _dp]),
+        "cfd_ctx_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(Tiles)]),
+        "cfd_ctx_autotune": (ctypes.c_int, [_vp, i32, ctypes.POINTER(Tiles), _dp]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -234,22 +243,23 @@ class Context:
         T_tau := T, C_tau := C); the next pseudo-step starts like the first one."""
         check(load().cfd_ctx_sync_levels(self.h))
 
-    def set_tiles(self, hb1, hb2):
-        """Row-block heights of the fused u/v/p and T/C kernels (0: built-in choice)."""
-        check(load().cfd_ctx_set_tiles(self.h, int(hb1), int(hb2)))
+    def set_tiles(self, rows_uvp=0, rows_tc=0, waves_uvp=0, waves_tc=0):
+        """Shapes of the fused u/v/p and T/C kernels (zeros: built-in choice)."""
+        t = Tiles(int(rows_uvp), int(rows_tc), int(waves_uvp), int(waves_tc))
+        check(load().cfd_ctx_set_tiles(self.h, ctypes.byref(t)))
 
     def get_tiles(self):
-        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
-        check(load().cfd_ctx_get_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        t = Tiles()
+        check(load().cfd_ctx_get_tiles(self.h, ctypes.byref(t)))
+        return t.as_tuple()
 
     def autotune(self, steps=10):
-        """Time candidate row-block heights on a snapshot of the state (restored afterwards);
-        returns (hb1, hb2, best ms per step)."""
-        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        """Time candidate shapes on a snapshot of the state (restored afterwards); returns
+        ((rows_uvp, rows_tc, waves_uvp, waves_tc), best ms per step)."""
+        t = Tiles()
         ms = np.zeros(1)
-        check(load().cfd_ctx_autotune(self.h, int(steps), ctypes.byref(a), ctypes.byref(b), _ptr(ms)))
-        return a.value, b.value, float(ms[0])
+        check(load().cfd_ctx_autotune(self.h, int(steps), ctypes.byref(t), _ptr(ms)))
+        return t.as_tuple(), float(ms[0])
 
     def device_ptr(self, name):
         p = _vp()

@@ -132,8 +132,10 @@ __device__ __forceinline__ void fix_v(double *row, int x, int j, int jm)
     }
 }
 
-template <bool STEADY, bool MUPOLY>
-__global__ __launch_bounds__(NT1, 1) void k_uvp(const K1Args a)
+// OCC: waves per SIMD the register budget is sized for (1: no spills; 2: 256 registers, the
+// compiler spills the rest to scratch). The autotuner picks the faster one per grid.
+template <bool STEADY, bool MUPOLY, int OCC>
+__global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
 {
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
@@ -448,8 +450,8 @@ __device__ __forceinline__ double lds_c(const double *row, int x, int j, int dj,
     return row[PAD + x + dj];
 }
 
-template <bool STEADY>
-__global__ __launch_bounds__(NT2, 1) void k_scal(const K2Args a)
+template <bool STEADY, int OCC>
+__global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
 {
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
@@ -627,12 +629,24 @@ int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_para
     const bool mp = P->mu_model == CFD_MU_POLY4;
     const bool timed = a->ev[0] != nullptr;
     if (timed) hipEventRecord(a->ev[0], s);
+    const int o1 = a->occ1 == 2 ? 2 : 1;
+    const dim3 g1(grid1), b1(NT1);
     if (P->steady) {
-        if (mp) hipLaunchKernelGGL((k_uvp<true, true>), dim3(grid1), dim3(NT1), 0, s, k1);
-        else hipLaunchKernelGGL((k_uvp<true, false>), dim3(grid1), dim3(NT1), 0, s, k1);
+        if (mp) {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<true, true, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<true, true, 1>), g1, b1, 0, s, k1);
+        } else {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<true, false, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<true, false, 1>), g1, b1, 0, s, k1);
+        }
     } else {
-        if (mp) hipLaunchKernelGGL((k_uvp<false, true>), dim3(grid1), dim3(NT1), 0, s, k1);
-        else hipLaunchKernelGGL((k_uvp<false, false>), dim3(grid1), dim3(NT1), 0, s, k1);
+        if (mp) {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<false, true, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<false, true, 1>), g1, b1, 0, s, k1);
+        } else {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<false, false, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<false, false, 1>), g1, b1, 0, s, k1);
+        }
     }
     int r = launched();
     if (r != CFD_OK) return r;
@@ -650,8 +664,14 @@ int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_para
     k2.nbj = cdiv(jm - 2, W2);
     k2.nbi = pick_rows(im - 2, k2.nbj, 16, a->hb2, &k2.hb);
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
-    if (P->steady) hipLaunchKernelGGL(k_scal<true>, dim3(grid2), dim3(NT2), 0, s, k2);
-    else hipLaunchKernelGGL(k_scal<false>, dim3(grid2), dim3(NT2), 0, s, k2);
+    const dim3 g2(grid2), b2(NT2);
+    if (a->occ2 == 4) {
+        if (P->steady) hipLaunchKernelGGL((k_scal<true, 4>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<false, 4>), g2, b2, 0, s, k2);
+    } else {
+        if (P->steady) hipLaunchKernelGGL((k_scal<true, 1>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<false, 1>), g2, b2, 0, s, k2);
+    }
     r = launched();
     if (r != CFD_OK) return r;
     if (timed) hipEventRecord(a->ev[2], s);

@@ -155,8 +155,8 @@ struct cfd_ctx {
     hipEvent_t evk[4];     /* per-kernel timing events, created on first use */
     double last_ms_per_step;
     int64_t n_fused, n_exact;
-    int32_t hb1, hb2;      /* fused row-block heights (0: built-in choice) */
-    int tiles_set;         /* hb1/hb2 chosen by the caller or the autotuner */
+    cfd_tiles tiles;       /* fused kernel shapes (zeros: built-in choice) */
+    int tiles_set;         /* chosen by the caller or the autotuner */
 };
 
 static void tiles_from_cache(cfd_ctx *c);
@@ -605,7 +605,8 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     a.u1 = X[X_U1]; a.u2 = X[X_U2]; a.v1 = X[X_V1]; a.v2 = X[X_V2];
     a.z1 = X[X_Z1]; a.z2 = X[X_Z2]; a.c1 = X[X_C1]; a.c2 = X[X_C2];
     a.res_max = slot;
-    a.hb1 = c->hb1; a.hb2 = c->hb2;
+    a.hb1 = c->tiles.rows_uvp; a.hb2 = c->tiles.rows_tc;
+    a.occ1 = c->tiles.waves_uvp; a.occ2 = c->tiles.waves_tc;
     if (timed) for (int k = 0; k < 4; ++k) a.ev[k] = c->evk[k];
     CHK(cfdk_zero_u64(slot, 3, c->s));
     CHK(cfdk_fused_step(&c->g, &c->fac, &c->P, &a, c->kernel, c->s));
@@ -843,10 +844,10 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
 }
 
 /* ---- tile shapes and the autotuner ------------------------------------------------- */
-/* Tuned row-block heights, kept per grid class for the life of the process. */
+/* Tuned shapes, kept per grid class for the life of the process. */
 typedef struct tile_entry {
     int32_t imax, jmax, prec, steady, mu_model, scheme, bc, buoyancy;
-    int32_t hb1, hb2;
+    cfd_tiles t;
 } tile_entry;
 enum { TILE_CACHE = 32 };
 static tile_entry g_tiles[TILE_CACHE];
@@ -862,7 +863,7 @@ static int tile_key_eq(const tile_entry *e, const cfd_ctx *c)
 static void tiles_from_cache(cfd_ctx *c)
 {
     for (int k = 0; k < g_ntiles; ++k)
-        if (tile_key_eq(&g_tiles[k], c)) { c->hb1 = g_tiles[k].hb1; c->hb2 = g_tiles[k].hb2; c->tiles_set = 1; return; }
+        if (tile_key_eq(&g_tiles[k], c)) { c->tiles = g_tiles[k].t; c->tiles_set = 1; return; }
 }
 
 static void tiles_to_cache(const cfd_ctx *c)
@@ -874,34 +875,43 @@ static void tiles_to_cache(const cfd_ctx *c)
         else k = TILE_CACHE - 1;          /* full: overwrite the last entry */
     }
     const cfd_params *P = &c->P;
-    tile_entry e = {P->imax, P->jmax, c->prec, P->steady, P->mu_model, P->scheme, P->bc, P->buoyancy, c->hb1, c->hb2};
+    tile_entry e = {P->imax, P->jmax, c->prec, P->steady, P->mu_model, P->scheme, P->bc, P->buoyancy, c->tiles};
     g_tiles[k] = e;
 }
 
-int cfd_ctx_set_tiles(cfd_ctx *c, int32_t hb1, int32_t hb2)
+static int tiles_valid(const cfd_ctx *c, const cfd_tiles *t)
+{
+    if (t->rows_uvp < 0 || t->rows_tc < 0 || t->rows_uvp > c->P.imax || t->rows_tc > c->P.imax)
+        return fail(CFD_ERR_ARG, "row-block heights must lie in [0, imax], got %d, %d", t->rows_uvp, t->rows_tc);
+    if (!(t->waves_uvp == 0 || t->waves_uvp == 1 || t->waves_uvp == 2))
+        return fail(CFD_ERR_ARG, "waves_uvp must be 0, 1 or 2, got %d", t->waves_uvp);
+    if (!(t->waves_tc == 0 || t->waves_tc == 1 || t->waves_tc == 4))
+        return fail(CFD_ERR_ARG, "waves_tc must be 0, 1 or 4, got %d", t->waves_tc);
+    return CFD_OK;
+}
+
+int cfd_ctx_set_tiles(cfd_ctx *c, const cfd_tiles *t)
 {
     if (!c) return fail(CFD_ERR_STATE, "no context");
-    if (hb1 < 0 || hb2 < 0 || hb1 > c->P.imax || hb2 > c->P.imax)
-        return fail(CFD_ERR_ARG, "row-block heights must lie in [0, imax], got %d, %d", hb1, hb2);
-    c->hb1 = hb1;
-    c->hb2 = hb2;
+    if (!t) return fail(CFD_ERR_ARG, "tiles is NULL");
+    CHK(tiles_valid(c, t));
+    c->tiles = *t;
     c->tiles_set = 1;
     return CFD_OK;
 }
 
-int cfd_ctx_get_tiles(const cfd_ctx *c, int32_t *hb1, int32_t *hb2)
+int cfd_ctx_get_tiles(const cfd_ctx *c, cfd_tiles *t)
 {
     if (!c) return fail(CFD_ERR_STATE, "no context");
-    if (hb1) *hb1 = c->hb1;
-    if (hb2) *hb2 = c->hb2;
+    if (!t) return fail(CFD_ERR_ARG, "tiles is NULL");
+    *t = c->tiles;
     return CFD_OK;
 }
 
 /* Mean ms per fused step over `steps` steps (after one untimed step) with the given tiles. */
-static int time_tiles(cfd_ctx *c, int32_t hb1, int32_t hb2, int32_t steps, double *ms_out)
+static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_out)
 {
-    c->hb1 = hb1;
-    c->hb2 = hb2;
+    c->tiles = *t;
     CHK(step_fused(c, c->d_hist));
     tau_swap(c);
     HIPCHK(hipEventRecord(c->ev0, c->s));
@@ -917,7 +927,20 @@ static int time_tiles(cfd_ctx *c, int32_t hb1, int32_t hb2, int32_t steps, doubl
     return CFD_OK;
 }
 
-int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, int32_t *hb1_out, int32_t *hb2_out, double *out_ms)
+/* Candidate shapes. The u/v/p kernel holds ~300 registers per lane: one wave per SIMD
+ * without spills, or two with the excess spilled to scratch (then more, shorter row blocks
+ * fill the 2 x 256 wave slots). The T/C kernel runs 3 waves per SIMD as compiled, or 4 with
+ * spills. */
+static const cfd_tiles CAND_UVP[] = {
+    {0, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0}, {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
+    {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
+};
+static const cfd_tiles CAND_TC[] = {
+    {0, 0, 0, 1}, {0, 8, 0, 1}, {0, 12, 0, 1}, {0, 16, 0, 1}, {0, 24, 0, 1}, {0, 32, 0, 1}, {0, 64, 0, 1},
+    {0, 8, 0, 4}, {0, 12, 0, 4}, {0, 16, 0, 4}, {0, 24, 0, 4}, {0, 32, 0, 4},
+};
+
+int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)
 {
     CHK(check_ready(c));
     if (steps < 1) steps = 1;
@@ -936,6 +959,7 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, int32_t *hb1_out, int32_t *hb2_o
     const int f_bc = c->bc_clean, f_uv = c->uv_reads_pn, f_first = c->first_step, f_pend = c->pending_in_tau;
     const int64_t n_f = c->n_fused, n_e = c->n_exact;
     const double last_ms = c->last_ms_per_step;
+    const cfd_tiles prev = c->tiles;
     int r = CFD_OK;
     for (int k = 0; k < S_NSTATE && r == CFD_OK; ++k)
         r = cfdk_copy(snap + off[k], c->st[k], (long)kind_count(c, kinds[k]), c->s);
@@ -945,20 +969,23 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, int32_t *hb1_out, int32_t *hb2_o
         if (r == CFD_OK) tau_swap(c);
     }
     const int rows = c->P.imax - 2;
-    static const int32_t cand1[] = {0, 16, 24, 32, 48, 64, 96, 128};
-    static const int32_t cand2[] = {0, 8, 12, 16, 24, 32, 48, 64};
-    int32_t best1 = 0, best2 = 0;
+    cfd_tiles best_t = {0, 0, 1, 1};
     double best = INFINITY, t = 0.0;
-    /* the two kernels are independent: tune K1's height with K2's default, then K2's */
-    for (size_t q = 0; q < sizeof cand1 / sizeof cand1[0] && r == CFD_OK; ++q) {
-        if (cand1[q] > rows) continue;
-        r = time_tiles(c, cand1[q], 0, steps, &t);
-        if (r == CFD_OK && t < best) { best = t; best1 = cand1[q]; }
+    /* the two kernels are independent: tune the u/v/p shape with the T/C default, then the
+     * T/C shape with the best u/v/p one */
+    for (size_t q = 0; q < sizeof CAND_UVP / sizeof CAND_UVP[0] && r == CFD_OK; ++q) {
+        if (CAND_UVP[q].rows_uvp > rows) continue;
+        cfd_tiles cand = CAND_UVP[q];
+        cand.rows_tc = 0; cand.waves_tc = 1;
+        r = time_tiles(c, &cand, steps, &t);
+        if (r == CFD_OK && t < best) { best = t; best_t.rows_uvp = cand.rows_uvp; best_t.waves_uvp = cand.waves_uvp; }
     }
-    for (size_t q = 0; q < sizeof cand2 / sizeof cand2[0] && r == CFD_OK; ++q) {
-        if (cand2[q] > rows) continue;
-        r = time_tiles(c, best1, cand2[q], steps, &t);
-        if (r == CFD_OK && t < best) { best = t; best2 = cand2[q]; }
+    for (size_t q = 0; q < sizeof CAND_TC / sizeof CAND_TC[0] && r == CFD_OK; ++q) {
+        if (CAND_TC[q].rows_tc > rows) continue;
+        cfd_tiles cand = best_t;
+        cand.rows_tc = CAND_TC[q].rows_tc; cand.waves_tc = CAND_TC[q].waves_tc;
+        r = time_tiles(c, &cand, steps, &t);
+        if (r == CFD_OK && t < best) { best = t; best_t = cand; }
     }
     /* restore the state exactly as it was */
     memcpy(c->st, roles, sizeof roles);
@@ -971,13 +998,11 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, int32_t *hb1_out, int32_t *hb2_o
     c->bc_clean = f_bc; c->uv_reads_pn = f_uv; c->first_step = f_first; c->pending_in_tau = f_pend;
     c->n_fused = n_f; c->n_exact = n_e;
     c->last_ms_per_step = last_ms;
-    if (r != CFD_OK) return r;
-    c->hb1 = best1;
-    c->hb2 = best2;
+    if (r != CFD_OK) { c->tiles = prev; return r; }
+    c->tiles = best_t;
     c->tiles_set = 1;
     tiles_to_cache(c);
-    if (hb1_out) *hb1_out = best1;
-    if (hb2_out) *hb2_out = best2;
+    if (out) *out = best_t;
     if (out_ms) *out_ms = best;
     return CFD_OK;
 }

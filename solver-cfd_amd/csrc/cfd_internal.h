@@ -63,8 +63,9 @@ typedef struct cfd_fused_args {
     double *u1, *u2, *v1, *v2, *z1, *z2, *c1, *c2;
     /* residual accumulators: res_max[0..2] (uint64 bit patterns of max|r3| for u, v, p) */
     uint64_t *res_max;
-    /* row-block heights of K1 / K2 (0: built-in heuristic); chosen by the autotuner */
-    int32_t hb1, hb2;
+    /* row-block heights of K1 / K2 (0: built-in heuristic) and the register-budget variants
+     * (occ1 in {1, 2}, occ2 in {1, 4} waves per SIMD); chosen by the autotuner */
+    int32_t hb1, hb2, occ1, occ2;
     /* optional events recorded before K1, after K1, after K2 and after the boundary pass */
     hipEvent_t ev[4];
 } cfd_fused_args;

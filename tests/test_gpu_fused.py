@@ -147,10 +147,11 @@ def test_graph_replays_match_eager():
         c.close()
 
 
-@pytest.mark.parametrize("tiles", [(16, 8), (37, 11), (64, 64)])
+@pytest.mark.parametrize("tiles", [(16, 8, 1, 1), (37, 11, 2, 4), (64, 64, 2, 1), (24, 5, 1, 4)])
 def test_fused_explicit_tiles_match_oracle(tiles):
     """Row-block heights other than the built-in choice (including ones that do not divide
-    the row count) give the same trajectory."""
+    the row count) and both register-budget variants of each kernel give the same
+    trajectory."""
     say("case: explicit tiles", tiles)
     prm, x, y, eps, lp, st = _multiblock_case(180, 300, bc=0, invfr2=0.7, g=1.5)
     o = O.Oracle(prm, x, y, eps, lp)
@@ -168,9 +169,29 @@ def test_fused_explicit_tiles_match_oracle(tiles):
     ctx.close()
 
 
+def test_fused_two_wave_variants_channel_mu():
+    """The two-wave (spilling) variants on the channel set with mu(T) and Boussinesq
+    buoyancy, the template instances the full-physics workload runs."""
+    say("case: two-wave variants, channel + mu + Boussinesq")
+    prm, x, y, eps, lp, st = _multiblock_case(
+        140, 520, bc=BC_CHANNEL, invfr2=0.0, buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, slot_lo=0.4,
+        slot_hi=0.6, slot_value=1.0, mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3), t_bottom=1.0, t_top=0.0)
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    h_ref = o.pseudo_steps(s, cs, 40)
+    ctx = L.Context(prm, x, y, eps, lp)
+    ctx.set_tiles(24, 12, 2, 4)
+    ctx.set_state(st)
+    res, hist = ctx.pseudo_steps(40, history=True)
+    errs = {k: max_rel(ctx.download(k), s[k]) for k in FIELDS}
+    say(max_rel(hist, h_ref), errs)
+    assert max_rel(hist, h_ref) <= TOL and max(errs.values()) <= TOL
+    ctx.close()
+
+
 def test_autotune_restores_state_and_keeps_parity():
     """The autotuner leaves the state, the step counters and the first-step semantics as they
-    were, picks heights from its candidate set, and later contexts of the grid reuse them."""
+    were, picks a shape from its candidate set, and later contexts of the grid reuse it."""
     say("case: autotune")
     prm, x, y, eps, lp, st = _multiblock_case(200, 280, bc=BC_CHANNEL, invfr2=0.0)
     o = O.Oracle(prm, x, y, eps, lp)
@@ -179,10 +200,12 @@ def test_autotune_restores_state_and_keeps_parity():
     ctx = L.Context(prm, x, y, eps, lp)
     ctx.set_state(st)
     before = {k: ctx.download(k) for k in FIELDS}
-    hb1, hb2, ms = ctx.autotune(steps=3)
-    say("tuned", hb1, hb2, ms)
-    assert hb1 in (0, 16, 24, 32, 48, 64, 96, 128) and hb2 in (0, 8, 12, 16, 24, 32, 48, 64)
-    assert ms > 0 and ctx.get_tiles() == (hb1, hb2)
+    tiles, ms = ctx.autotune(steps=3)
+    say("tuned", tiles, ms)
+    rows_uvp, rows_tc, waves_uvp, waves_tc = tiles
+    assert rows_uvp in (0, 16, 24, 32, 40, 48, 64, 96, 128) and rows_tc in (0, 8, 12, 16, 24, 32, 64)
+    assert waves_uvp in (1, 2) and waves_tc in (1, 4)
+    assert ms > 0 and ctx.get_tiles() == tiles
     assert ctx.step_counts() == (0, 0)
     for k in FIELDS:
         assert np.array_equal(ctx.download(k), before[k]), k
@@ -192,5 +215,5 @@ def test_autotune_restores_state_and_keeps_parity():
     assert max_rel(hist, h_ref) <= TOL and max(errs.values()) <= TOL
     ctx.close()
     ctx2 = L.Context(prm, x, y, eps, lp)
-    assert ctx2.get_tiles() == (hb1, hb2)
+    assert ctx2.get_tiles() == tiles
     ctx2.close()
