@@ -200,9 +200,9 @@ def case_phys24():
     save("phys24", prm, x, y, eps, lp, st, dict(nphys=nphys, itc_max=itc_max, eps_conv=eps_conv), out)
 
 
-def case_upwind_pe100_51():
-    """Configuration 1's grid, Re, beta and seeded IC with the reference BC set and QUICK
-    (the Fortran reference has no other BC set or global-upwind solve)."""
+def case_uniform51_pe100():
+    """51x51 uniform grid, Re=100, beta=1, seeded +/-1e-3 perturbation, with the reference BC
+    set and QUICK (the Fortran reference has no other BC set or global-upwind solve)."""
     n = 51
     ov = dict(imax=n, jmax=n, Re=100.0, Pe=100.0, Sc=1.0, Darcy_number=1.0e-2, Cf=0.4,
               g=0.0, Fr=1.0, v_i=0.5)
@@ -211,12 +211,12 @@ def case_upwind_pe100_51():
     lp = C.zeros("p", n, n)
     st = C.perturbed_state(n, n, seed=1, amp=1e-3, T0=1.0, C0=0.0)
     hd = dict(dtau=0.05, dt=0.005, beta=1.0, b_art=0.0, tinf=1.0)
-    case_steps("upwind_pe100_51", "c51", ov, x, y, eps, lp, st, 2000, hd)
+    case_steps("uniform51_pe100", "c51", ov, x, y, eps, lp, st, 2000, hd)
 
 
 def case_quick_porous48():
-    """Configuration 2's physics (QUICK, Re=200, porous middle third eps~U[0.4,0.9] seed 2,
-    Da=1e-3, Cf=0.55, buoyancy on) at 48x48 with the reference BC set and buoyancy form."""
+    """QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2), Da=1e-3, Cf=0.55,
+    buoyancy on, at 48x48 with the reference BC set and buoyancy form."""
     n = 48
     ov = dict(imax=n, jmax=n, Re=200.0, Pe=200.0, Sc=1.0, Darcy_number=1.0e-3, Cf=0.55,
               g=0.0, Fr=1.0, v_i=0.5)
@@ -253,7 +253,7 @@ def case_mesh51():
 
 
 CASES = dict(ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
-             phys24=case_phys24, upwind_pe100_51=case_upwind_pe100_51, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
+             phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)
