@@ -233,6 +233,48 @@ int  cfd_ctx_step_counts(const cfd_ctx *ctx, int64_t *fused, int64_t *exact);
  * The host waits after every step, so use it for measurement, not production runs. */
 int  cfd_pseudo_steps_timed(cfd_ctx *ctx, int32_t nsteps, double res[3], double ms[4]);
 
+/* ---- batched solves ------------------------------------------------------------------
+ * Many independent cases (each its own grid size, parameters, mesh and fields) advance
+ * together: one persistent launch per kernel walks the tiles of every case, the per-case
+ * exit test of the pseudo-time loop runs on the device, and a run replays a captured
+ * hipGraph of `steps_per_graph` pseudo-steps until every case has stopped. Fields are
+ * stored in the batch precision (CFD_PREC_FP64, or CFD_PREC_FP32 with FP64 pressure and
+ * residual maxima). Every case must share `steady` and `mu_model`; b_art must be 0 and
+ * compat CFD_COMPAT_FORTRAN (batches run the fused kernels only). Arrays use the C layout
+ * of one case (see above); uploads and downloads are host arrays of doubles. */
+typedef struct cfd_batch cfd_batch;
+
+int  cfd_batch_create(int32_t ncases, const cfd_params *params, int32_t precision, cfd_batch **out);
+int  cfd_batch_destroy(cfd_batch *b);
+/* node coordinates x[1..imax], y[1..jmax] of one case (index 0 unused) */
+int  cfd_batch_set_mesh(cfd_batch *b, int32_t c, const double *x, const double *y);
+/* Fields of one case. Uploads write the current pseudo-time iterate (um_tau, ...) or the
+ * physical level (um, ...; for steady batches the two are the same array), eps and L.
+ * Downloads of um_tau/vm_tau/p/T_tau/C_tau return the case's latest iterate (the step's
+ * output when it stopped on convergence, as the reference exits without the update). */
+int  cfd_batch_upload(cfd_batch *b, int32_t c, int32_t field, const double *host);
+int  cfd_batch_download(cfd_batch *b, int32_t c, int32_t field, double *host);
+/* Writes the boundary cells of every case's current iterates from their interior values
+ * (channel BC set), so that a raw initial state (zeros, a warm start) is BC-consistent. */
+int  cfd_batch_apply_bc(cfd_batch *b);
+int  cfd_batch_set_tiles(cfd_batch *b, const cfd_tiles *tiles);
+int  cfd_batch_get_tiles(const cfd_batch *b, cfd_tiles *tiles);
+/* Times candidate tile shapes (row heights and register variants of both kernels) with
+ * `steps`-step runs on the current state, restores it, and keeps the fastest. */
+int  cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *ms_per_step);
+/* Runs every case until it stops: converged (itc != 1 and max(res_u, res_v, res_p) < tol,
+ * status 1), non-finite residual (2), residual above diverge_factor x its best so far (3,
+ * diverge_factor <= 0 disables), or itc_max pseudo-steps (4). tol <= 0 runs exactly
+ * itc_max steps. iters[c], res[3c..3c+2] and status[c] (each may be NULL) receive the
+ * per-case outcome. Returns CFD_ERR_NONFINITE if a case failed, CFD_ERR_NOCONV if a case
+ * did not converge (tol > 0), CFD_OK otherwise. A later run continues from the stopped
+ * states. */
+int  cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
+                   int64_t *iters, double *res, int32_t *status);
+/* Wall time of the last run (HIP events on the batch stream), graph steps replayed, and
+ * cell-updates done (sum over cases of pseudo-steps x imax x jmax). */
+int  cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates);
+
 const char *cfd_last_error(void);
 const char *cfd_version(void);
 

@@ -24,7 +24,10 @@ EXPORTS_CFD = [
     "cfd_ctx_get_params", "cfd_ctx_set_params", "cfd_ctx_set_mesh", "cfd_ctx_upload", "cfd_ctx_download",
     "cfd_ctx_device_ptr", "cfd_ctx_sync_levels", "cfd_pseudo_steps", "cfd_run_to_tol", "cfd_physical_step",
     "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_ctx_step_counts",
-    "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed", "cfd_last_error", "cfd_version",
+    "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
+    "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
+    "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
+    "cfd_batch_last_timing", "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
     "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
@@ -88,6 +91,17 @@ def load():
         "cfd_pseudo_steps_timed": (ctypes.c_int, [_vp, i32, _dp, _dp]),
         "cfd_ctx_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(Tiles)]),
         "cfd_ctx_autotune": (ctypes.c_int, [_vp, i32, ctypes.POINTER(Tiles), _dp]),
+        "cfd_batch_create": (ctypes.c_int, [i32, P, i32, ctypes.POINTER(_vp)]),
+        "cfd_batch_destroy": (ctypes.c_int, [_vp]),
+        "cfd_batch_set_mesh": (ctypes.c_int, [_vp, i32, _dp, _dp]),
+        "cfd_batch_upload": (ctypes.c_int, [_vp, i32, i32, _dp]),
+        "cfd_batch_download": (ctypes.c_int, [_vp, i32, i32, _dp]),
+        "cfd_batch_apply_bc": (ctypes.c_int, [_vp]),
+        "cfd_batch_set_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(Tiles)]),
+        "cfd_batch_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(Tiles)]),
+        "cfd_batch_autotune": (ctypes.c_int, [_vp, i32, ctypes.POINTER(Tiles), _dp]),
+        "cfd_batch_run": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
+        "cfd_batch_last_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -265,3 +279,93 @@ class Context:
         p = _vp()
         check(load().cfd_ctx_device_ptr(self.h, FIELD_NAMES.index(name), ctypes.byref(p)))
         return p.value
+
+
+from .cases import shape_of, FIELD_KIND  # noqa: E402
+
+
+class Batch:
+    """Many independent cases advanced together (include/cfd.h, cfd_batch_*)."""
+
+    def __init__(self, params_list, precision=PREC_FP64):
+        L = load()
+        n = len(params_list)
+        arr = (CfdParams * n)(*params_list)
+        h = _vp()
+        check(L.cfd_batch_create(n, arr, precision, ctypes.byref(h)))
+        self.h = h
+        self.params = list(params_list)
+        self.n = n
+
+    def set_mesh(self, c, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+        check(load().cfd_batch_set_mesh(self.h, int(c), _ptr(x), _ptr(y)))
+
+    def upload(self, c, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        P = self.params[c]
+        want = shape_of(FIELD_KIND[name], P.imax, P.jmax)
+        if a.shape != want:
+            raise ValueError("%s of case %d: shape %s, expected %s" % (name, c, a.shape, want))
+        check(load().cfd_batch_upload(self.h, int(c), FIELD_NAMES.index(name), _ptr(a)))
+
+    def download(self, c, name):
+        P = self.params[c]
+        a = np.zeros(shape_of(FIELD_KIND[name], P.imax, P.jmax))
+        check(load().cfd_batch_download(self.h, int(c), FIELD_NAMES.index(name), _ptr(a)))
+        return a
+
+    def set_state(self, c, st):
+        """Current iterates of case c from a dict (um, vm, p, T, C and/or the *_tau names)."""
+        for k in ("um", "vm", "p", "T", "C"):
+            if k in st:
+                self.upload(c, k, st[k])
+        for k, base in (("um_tau", "um"), ("vm_tau", "vm"), ("T_tau", "T"), ("C_tau", "C")):
+            src = st.get(k, st.get(base))
+            if src is not None:
+                self.upload(c, k, src)
+
+    def apply_bc(self):
+        check(load().cfd_batch_apply_bc(self.h))
+
+    def set_tiles(self, rows_uvp=0, rows_tc=0, waves_uvp=0, waves_tc=0):
+        t = Tiles(int(rows_uvp), int(rows_tc), int(waves_uvp), int(waves_tc))
+        check(load().cfd_batch_set_tiles(self.h, ctypes.byref(t)))
+
+    def get_tiles(self):
+        t = Tiles()
+        check(load().cfd_batch_get_tiles(self.h, ctypes.byref(t)))
+        return t.as_tuple()
+
+    def autotune(self, steps=10):
+        t = Tiles()
+        ms = np.zeros(1)
+        check(load().cfd_batch_autotune(self.h, int(steps), ctypes.byref(t), _ptr(ms)))
+        return t.as_tuple(), float(ms[0])
+
+    def run(self, tol, itc_max, steps_per_graph=50, diverge_factor=0.0, raise_on_fail=False):
+        """Returns (code, iters[n], res[n, 3], status[n])."""
+        it = (ctypes.c_int64 * self.n)()
+        st = (ctypes.c_int32 * self.n)()
+        res = np.zeros((self.n, 3))
+        code = load().cfd_batch_run(self.h, float(tol), int(itc_max), int(steps_per_graph), float(diverge_factor),
+                                    it, _ptr(res), st)
+        if raise_on_fail:
+            check(code)
+        return code, np.array(it[:], dtype=np.int64), res, np.array(st[:], dtype=np.int32)
+
+    def last_timing(self):
+        ms = ctypes.c_double(0); steps = ctypes.c_int64(0); cells = ctypes.c_int64(0)
+        check(load().cfd_batch_last_timing(self.h, ctypes.byref(ms), ctypes.byref(steps), ctypes.byref(cells)))
+        return ms.value, steps.value, cells.value
+
+    def close(self):
+        if self.h:
+            load().cfd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

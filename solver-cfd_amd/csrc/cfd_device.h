@@ -177,7 +177,8 @@ __device__ __forceinline__ double ressc_lit(const Geo &g, double k, bool species
 // ---------------------------------------------------------------------------------------
 
 // u, reference set (boundary.f90:14-16, 24-26, 29-32, 39-42)
-__device__ __forceinline__ double bc_u_ref(const Geo &g, const double *u, int i, int j)
+template <typename F>
+__device__ __forceinline__ double bc_u_ref(const Geo &g, const F *u, int i, int j)
 {
     const int im = g.imax, jm = g.jmax;
     if (i <= 2 || i >= im) return 0.0;
@@ -185,7 +186,8 @@ __device__ __forceinline__ double bc_u_ref(const Geo &g, const double *u, int i,
     return u[iu(g, i, jm - 1)];                 // j == jmax
 }
 // u, channel set: left inflow u_in, right zero-gradient, no-slip walls
-__device__ __forceinline__ double bc_u_chan(const Geo &g, const cfd_params &P, const double *u, int i, int j)
+template <typename F>
+__device__ __forceinline__ double bc_u_chan(const Geo &g, const cfd_params &P, const F *u, int i, int j)
 {
     const int im = g.imax, jm = g.jmax;
     if (i <= 2) return P.u_in;
@@ -193,11 +195,13 @@ __device__ __forceinline__ double bc_u_chan(const Geo &g, const cfd_params &P, c
     return 0.0;                                  // j == 1 or j == jmax, 3 <= i <= imax-1
 }
 // v, reference set (boundary.f90:9-12, 19-22, 34-36, 44-46); (imax, jmax+1) excluded.
-__device__ __forceinline__ double bc_v_top_ref(const Geo &g, const double *v, int i)
+template <typename F>
+__device__ __forceinline__ double bc_v_top_ref(const Geo &g, const F *v, int i)
 {
-    return 2.0 * v[iv(g, i, g.jmax - 2)] - v[iv(g, i, g.jmax - 1)];
+    return 2.0 * (double)v[iv(g, i, g.jmax - 2)] - (double)v[iv(g, i, g.jmax - 1)];
 }
-__device__ __forceinline__ double bc_v_ref(const Geo &g, const cfd_params &P, const double *v, int i, int j)
+template <typename F>
+__device__ __forceinline__ double bc_v_ref(const Geo &g, const cfd_params &P, const F *v, int i, int j)
 {
     const int im = g.imax, jm = g.jmax;
     if (j <= 2) return P.v_i;
@@ -207,30 +211,33 @@ __device__ __forceinline__ double bc_v_ref(const Geo &g, const cfd_params &P, co
     }
     if (j == jm + 1) {                           // i <= imax-1 here
         const int ii = i == 1 ? 2 : i;
-        return 2.0 * v[iv(g, ii, jm - 1)] - bc_v_top_ref(g, v, ii);
+        return 2.0 * (double)v[iv(g, ii, jm - 1)] - bc_v_top_ref(g, v, ii);
     }
     return i == 1 ? v[iv(g, 2, j)] : v[iv(g, im - 1, j)];
 }
-__device__ __forceinline__ double bc_v_chan(const Geo &g, const double *v, int i, int j)
+template <typename F>
+__device__ __forceinline__ double bc_v_chan(const Geo &g, const F *v, int i, int j)
 {
     const int im = g.imax, jm = g.jmax;
     if (j <= 2 || j >= jm || i == 1) return 0.0;
     return v[iv(g, im - 1, j)];                  // i == imax, 3 <= j <= jmax-1
 }
 // p (kind 0), T (kind 1), C (kind 2); reference set boundary.f90:52-159, channel set as the oracle.
-__device__ __forceinline__ double bc_s_ref(const Geo &g, const cfd_params &P, int kind, const double *a, int i, int j)
+template <typename F>
+__device__ __forceinline__ double bc_s_ref(const Geo &g, const cfd_params &P, int kind, const F *a, int i, int j)
 {
     const int im = g.imax, jm = g.jmax;
     const int ii = i == 1 ? 2 : (i == im ? im - 1 : i);
     if (j == 1) return kind == 0 ? a[iu(g, ii, 2)] : (kind == 1 ? P.tinf : 0.0);
-    if (j == jm) return a[iu(g, ii, jm - 1)] + 1.0 * (a[iu(g, ii, jm - 1)] - a[iu(g, ii, jm - 2)]);
+    if (j == jm) return (double)a[iu(g, ii, jm - 1)] + 1.0 * ((double)a[iu(g, ii, jm - 1)] - (double)a[iu(g, ii, jm - 2)]);
     return a[iu(g, ii, j)];
 }
 __device__ __forceinline__ double slot_value(const cfd_params &P, double yj)
 {
     return (yj >= P.slot_lo && yj <= P.slot_hi) ? P.slot_value : 0.0;
 }
-__device__ __forceinline__ double bc_s_chan(const Geo &g, const cfd_params &P, int kind, const double *a, int i, int j)
+template <typename F>
+__device__ __forceinline__ double bc_s_chan(const Geo &g, const cfd_params &P, int kind, const F *a, int i, int j)
 {
     const int im = g.imax, jm = g.jmax;
     if (kind == 0) {

@@ -7,17 +7,27 @@
 //                  boundary values of in-flight stages through the BC formulas).
 //
 // Decomposition. Lanes walk j (the contiguous axis): every global access of a wavefront is
-// one coalesced row segment. A workgroup owns a column strip and a block of rows and marches
-// down its rows; stage k of a chain is computed a fixed number of rows behind stage k-1
-// (QUICK reaches two rows), so all three RK stages and the pressure update of a row happen
-// in one pass over the inputs. i-direction neighbours come from per-lane register windows
-// (Win<N>), j-direction neighbours of the centre row from small LDS rings. Each input row is
-// read from HBM once per pass (halo columns/rows of neighbouring blocks hit L2/MALL).
+// one coalesced row segment. A workgroup owns a column strip and a block of rows (a tile)
+// and marches down its rows; stage k of a chain is computed a fixed number of rows behind
+// stage k-1 (QUICK reaches two rows), so all three RK stages and the pressure update of a
+// row happen in one pass over the inputs. i-direction neighbours come from per-lane
+// register windows (Win<N>), j-direction neighbours of the centre row from small LDS rings.
+// Each input row is read from HBM once per pass (halo columns/rows of neighbouring tiles
+// hit L2/MALL).
+//
+// Precision. The tile routines are templated on the field type R: double (the FP64
+// drop-in) or float (FP32 storage and stencil arithmetic). Pressure (p, pn), the
+// continuity update and the residual maxima stay FP64 in both.
+//
+// Two launch forms share the tile routines: one grid per launch (k_uvp / k_scal, tile
+// from blockIdx with an XCD-aware order) and a persistent batch (k_uvp_batch /
+// k_scal_batch: a fixed grid walks a tile list that spans many independent cases, each
+// with its own sizes, parameters and buffers, skipping cases that have converged).
 //
 // Numerics: flux-form QUICK (face value 1/2 (U+D) - 1/8 curvature at the upstream cell,
 // which equals the reference's coefficient assembly of equations.f90:51-72), reciprocals of
-// the 1-D spacings precomputed, FMA contraction allowed: results differ from the literal
-// reference arithmetic by rounding only (tested at 1e-10 relative).
+// the 1-D spacings precomputed, FMA contraction allowed: FP64 results differ from the
+// literal reference arithmetic by rounding only (tested at 1e-10 relative).
 #include "cfd_device.h"
 #include "cfd_internal.h"
 
@@ -38,22 +48,22 @@ inline Geo mk(const cfd_devgeo *g)
 }
 
 // Sliding window of the last N rows of one lane's column; [0] is the newest row.
-template <int N>
+template <int N, typename T>
 struct Win {
-    double v[N];
-    __device__ __forceinline__ void fill(double x)
+    T v[N];
+    __device__ __forceinline__ void fill(T x)
     {
 #pragma unroll
         for (int k = 0; k < N; ++k) v[k] = x;
     }
-    __device__ __forceinline__ void push(double x)
+    __device__ __forceinline__ void push(T x)
     {
 #pragma unroll
         for (int k = 0; k < N - 1; ++k) v[k] = v[k + 1];
         v[N - 1] = x;
     }
-    __device__ __forceinline__ double operator[](int back) const { return v[N - 1 - back]; }
-    __device__ __forceinline__ void set(int back, double x) { v[N - 1 - back] = x; }
+    __device__ __forceinline__ T operator[](int back) const { return v[N - 1 - back]; }
+    __device__ __forceinline__ void set(int back, T x) { v[N - 1 - back] = x; }
 };
 
 __device__ __forceinline__ int s2(int r) { return (r + 24) & 1; }
@@ -75,10 +85,16 @@ __device__ __forceinline__ void fold_max(unsigned long long *out, unsigned long 
 }
 
 // QUICK face value in flux form: f > 0 takes the side-a upstream curvature.
-__device__ __forceinline__ double quick_face(double f, double s, double ca, double cb)
+template <typename R>
+__device__ __forceinline__ R quick_face(R f, R s, R ca, R cb)
 {
-    return 0.5 * s - 0.125 * (f > 0.0 ? ca : cb);
+    return R(0.5) * s - R(0.125) * (f > R(0) ? ca : cb);
 }
+
+// One tile of a persistent batch: case index, row block, column strip.
+struct BatchTile {
+    int c, bi, bj, pad;
+};
 
 // ---------------------------------------------------------------------------------------
 // K1: u, v (three RK stages each) and p.
@@ -90,9 +106,13 @@ constexpr int PAD = 3;                          // LDS padding for j-3 .. j+2 re
 constexpr int LROW1 = NT1 + 2 * PAD;
 
 // Scalars are precomputed on the host so that few values occupy scalar registers.
+template <typename R>
 struct K1Args {
-    const double *um, *vm, *ut, *vt, *p, *T, *eps, *lp;   // inputs
-    double *un, *vn, *pn;                                  // outputs (interior cells)
+    const R *um, *vm, *ut, *vt;                            // inputs
+    const double *p;
+    const R *T, *eps, *lp;
+    R *un, *vn;                                            // outputs (interior cells)
+    double *pn;
     unsigned long long *res;                               // max |r3| bit patterns: u, v, p
     const double *pk;                                      // packed geometry (cfd_devfactors.pack)
     int imax, jmax, nbi, nbj, hb, chan, upwind_all, bouss;
@@ -100,17 +120,25 @@ struct K1Args {
     double mu[5];
 };
 
+template <typename R>
+struct Smem1 {
+    R sUT[4][LROW1], sVT[4][LROW1], sT[2][LROW1];
+    R sU1[3][LROW1], sU2[3][LROW1], sV1[3][LROW1], sV2[3][LROW1], sV3[2][LROW1];
+    double sP[2][LROW1];
+};
+
 // v of the centre row at column j+dj with the BC formulas (boundary.f90:9-12,19-22 /
 // channel walls): used where the ring's boundary columns are not yet fixed up.
-__device__ __forceinline__ double lds_v(const double *row, int x, int j, int dj, int jm, bool chan, double vi)
+template <typename R>
+__device__ __forceinline__ R lds_v(const R *row, int x, int j, int dj, int jm, bool chan, R vi)
 {
     const int jj = j + dj;
-    if (jj <= 2) return chan ? 0.0 : vi;
-    if (jj == jm) return chan ? 0.0 : 2.0 * row[PAD + x + dj - 2] - row[PAD + x + dj - 1];
+    if (jj <= 2) return chan ? R(0) : vi;
+    if (jj == jm) return chan ? R(0) : R(2) * row[PAD + x + dj - 2] - row[PAD + x + dj - 1];
     if (jj == jm + 1) {
-        if (chan) return 0.0;
-        const double a = row[PAD + x + dj - 3], b = row[PAD + x + dj - 2];
-        return 2.0 * b - (2.0 * a - b);
+        if (chan) return R(0);
+        const R a = row[PAD + x + dj - 3], b = row[PAD + x + dj - 2];
+        return R(2) * b - (R(2) * a - b);
     }
     return row[PAD + x + dj];
 }
@@ -118,79 +146,77 @@ __device__ __forceinline__ double lds_v(const double *row, int x, int j, int dj,
 // Reference-set boundary columns of a u ring row (u(jmax) = u(jmax-1), boundary.f90:24-26)
 // and of a v ring row (top extrapolation, boundary.f90:19-22), written by the lanes that
 // own those columns one iteration after the row was produced and before it is read.
-__device__ __forceinline__ void fix_u(double *row, int x, int j, int jm)
+template <typename R>
+__device__ __forceinline__ void fix_u(R *row, int x, int j, int jm)
 {
     if (j == jm) row[PAD + x] = row[PAD + x - 1];
 }
-__device__ __forceinline__ void fix_v(double *row, int x, int j, int jm)
+template <typename R>
+__device__ __forceinline__ void fix_v(R *row, int x, int j, int jm)
 {
     if (j == jm) {
-        row[PAD + x] = 2.0 * row[PAD + x - 2] - row[PAD + x - 1];
+        row[PAD + x] = R(2) * row[PAD + x - 2] - row[PAD + x - 1];
     } else if (j == jm + 1) {
-        const double a = row[PAD + x - 3], b = row[PAD + x - 2];
-        row[PAD + x] = 2.0 * b - (2.0 * a - b);
+        const R a = row[PAD + x - 3], b = row[PAD + x - 2];
+        row[PAD + x] = R(2) * b - (R(2) * a - b);
     }
 }
 
-// OCC: waves per SIMD the register budget is sized for (1: no spills; 2: 256 registers, the
-// compiler spills the rest to scratch). The autotuner picks the faster one per grid.
-template <bool STEADY, bool MUPOLY, int OCC>
-__global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
+template <typename R, bool STEADY, bool MUPOLY>
+__device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Smem1<R> &sm)
 {
-    const int G = a.nbi * a.nbj, per = (G + 7) / 8;
-    const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
-    if (L >= G) return;
-    const int bi = L / a.nbj, bj = L % a.nbj;
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);        // output rows [ilo, ihi)
     const int jlo = 2 + bj * W1, jhi = min(jlo + W1, jm);            // output cols [jlo, jhi)
     const int x = threadIdx.x;
     const int j = jlo - HL1 + x;
     const bool chan = a.chan != 0, upwind_all = a.upwind_all != 0, bouss = a.bouss != 0;
-    const double dt = a.dt, dtau = a.dtau, vi = a.v_i;
-    const int R = im + 2, Cc = jm + 2;
-    const double *pr = a.pk, *pc = a.pk + 4 * R;    // rows [aun|avn|rdx|rdxm], cols [aue|ave|rdy|rdym]
+    const R dt = R(a.dt), dtau = R(a.dtau), vi = R(a.v_i), inv_re = R(a.inv_re), gg = R(a.g);
+    const int RR = im + 2, Cc = jm + 2;
+    const double *pr = a.pk, *pc = a.pk + 4 * RR;    // rows [aun|avn|rdx|rdxm], cols [aue|ave|rdy|rdym]
+    auto sUT = sm.sUT; auto sVT = sm.sVT; auto sT = sm.sT; auto sP = sm.sP;
+    auto sU1 = sm.sU1; auto sU2 = sm.sU2; auto sV1 = sm.sV1; auto sV2 = sm.sV2; auto sV3 = sm.sV3;
 
-    __shared__ double sUT[4][LROW1], sVT[4][LROW1], sP[2][LROW1], sT[2][LROW1];
-    __shared__ double sU1[3][LROW1], sU2[3][LROW1], sV1[3][LROW1], sV2[3][LROW1], sV3[2][LROW1];
     for (int k = x; k < LROW1; k += NT1) {
-        for (int q = 0; q < 4; ++q) { sUT[q][k] = 0.0; sVT[q][k] = 0.0; }
-        for (int q = 0; q < 3; ++q) { sU1[q][k] = 0.0; sU2[q][k] = 0.0; sV1[q][k] = 0.0; sV2[q][k] = 0.0; }
-        for (int q = 0; q < 2; ++q) { sP[q][k] = 0.0; sT[q][k] = 1.0; sV3[q][k] = 0.0; }
+        for (int q = 0; q < 4; ++q) { sUT[q][k] = R(0); sVT[q][k] = R(0); }
+        for (int q = 0; q < 3; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
+        for (int q = 0; q < 2; ++q) { sP[q][k] = 0.0; sT[q][k] = R(1); sV3[q][k] = R(0); }
     }
 
     // per-column geometry (clamped indices; values outside the ranges are never used)
     const bool jvalid = j >= 1 && j <= jm + 1;
     const int jc = min(max(j, 2), jm);
-    const double haue_j = 0.5 * pc[min(max(j, 1), jm)];
-    const double aue_j = pc[min(max(j, 1), jm)];
-    const double ave_j = pc[Cc + jc], have_j = 0.5 * ave_j;
-    const double rdy_j = pc[2 * Cc + jc], rdy_n = pc[2 * Cc + min(jc + 1, jm)];
-    const double rdym_j = pc[3 * Cc + jc], rdym_n = pc[3 * Cc + min(jc + 1, jm + 1)];
+    const R haue_j = R(0.5 * pc[min(max(j, 1), jm)]);
+    const double aue_jd = pc[min(max(j, 1), jm)];
+    const R aue_j = R(aue_jd);
+    const R ave_j = R(pc[Cc + jc]), have_j = R(0.5 * pc[Cc + jc]);
+    const R rdy_j = R(pc[2 * Cc + jc]), rdy_n = R(pc[2 * Cc + min(jc + 1, jm)]);
+    const R rdym_j = R(pc[3 * Cc + jc]), rdym_n = R(pc[3 * Cc + min(jc + 1, jm + 1)]);
     const bool ucol = j >= 2 && j <= jm - 1;           // u / p interior column
     const bool vcol = j >= 3 && j <= jm - 1;           // v interior column
     const bool u_upw_col = upwind_all || j == 2 || j == jm - 1;
     const bool out_col = j >= jlo && j < jhi;
 
-    Win<7> ut, Su;        // u_tau rows t-6..t ; Su = u_tau(r,j) + u_tau(r,j-1)
-    Win<8> vt;            // v_tau rows t-7..t
-    Win<6> Svn;           // v_tau(r,j+1) + v_tau(r-1,j+1), rows t-6..t-1
-    Win<6> pw;            // p rows t-7..t-2
-    Win<5> ie, ew, nuw, k1w, k2w, dpj, bw, du, dv;   // per-cell data, rows t-6..t-2
-    Win<5> w1, w2, x1, x2;                            // u1, u2, v1, v2 own windows
-    Win<2> w3, x3;                                    // u3, v3
-    ut.fill(0.0); Su.fill(0.0); vt.fill(0.0); Svn.fill(0.0); pw.fill(0.0);
-    ie.fill(1.0); ew.fill(1.0); k1w.fill(0.0); k2w.fill(0.0); dpj.fill(0.0); bw.fill(0.0);
-    if (MUPOLY) nuw.fill(0.0);
-    if (!STEADY) { du.fill(0.0); dv.fill(0.0); }
-    w1.fill(0.0); w2.fill(0.0); x1.fill(0.0); x2.fill(0.0); w3.fill(0.0); x3.fill(0.0);
+    Win<7, R> ut, Su;     // u_tau rows t-6..t ; Su = u_tau(r,j) + u_tau(r,j-1)
+    Win<8, R> vt;         // v_tau rows t-7..t
+    Win<6, R> Svn;        // v_tau(r,j+1) + v_tau(r-1,j+1), rows t-6..t-1
+    Win<6, double> pw;    // p rows t-7..t-2
+    Win<5, R> ie, ew, nuw, k1w, k2w, dpj, bw, du, dv;   // per-cell data, rows t-6..t-2
+    Win<5, R> w1, w2, x1, x2;                            // u1, u2, v1, v2 own windows
+    Win<2, R> w3, x3;                                    // u3, v3
+    ut.fill(R(0)); Su.fill(R(0)); vt.fill(R(0)); Svn.fill(R(0)); pw.fill(0.0);
+    ie.fill(R(1)); ew.fill(R(1)); k1w.fill(R(0)); k2w.fill(R(0)); dpj.fill(R(0)); bw.fill(R(0));
+    if (MUPOLY) nuw.fill(R(0));
+    if (!STEADY) { du.fill(R(0)); dv.fill(R(0)); }
+    w1.fill(R(0)); w2.fill(R(0)); x1.fill(R(0)); x2.fill(R(0)); w3.fill(R(0)); x3.fill(R(0));
     unsigned long long mu_ = 0, mv_ = 0, mp_ = 0;
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
     // next-row registers: the loads for row t+1 are issued before row t is computed
-    double q_ut, q_vt, q_p, q_T, q_e, q_L, q_um, q_vm;
+    R q_ut, q_vt, q_T, q_e, q_L, q_um, q_vm;
+    double q_p;
     auto fetch = [&](int t) {
-        q_ut = 0.0; q_vt = 0.0; q_p = 0.0; q_T = 1.0; q_e = 1.0; q_L = 0.0; q_um = 0.0; q_vm = 0.0;
+        q_ut = R(0); q_vt = R(0); q_p = 0.0; q_T = R(1); q_e = R(1); q_L = R(0); q_um = R(0); q_vm = R(0);
         if (jvalid && t >= 1) {
             if (t <= im + 1 && j <= jm) q_ut = a.ut[(size_t)t * su + j];
             if (t <= im) q_vt = a.vt[(size_t)t * sv + j];
@@ -210,8 +236,9 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
 
     for (int t = ilo - 6; t <= ihi + 6; ++t) {
         // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
-        const double ut_new = q_ut, vt_new = q_vt, p_rp = q_p, T_rp = q_T, e = q_e, Lp = q_L;
-        const double um_rp = q_um, vm_rp = q_vm;
+        const R ut_new = q_ut, vt_new = q_vt, T_rp = q_T, e = q_e, Lp = q_L;
+        const double p_rp = q_p;
+        const R um_rp = q_um, vm_rp = q_vm;
         fetch(t + 1);
         const int rp = t - 2;
         ut.push(ut_new);
@@ -233,22 +260,22 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
         Svn.push(sVT[s4(t - 1)][PAD + x + 1] + sVT[s4(t - 2)][PAD + x + 1]);
         pw.push(p_rp);
         {
-            double iev = 1.0, k1 = 0.0, k2 = 0.0;
+            R iev = R(1), k1 = R(0), k2 = R(0);
             if (rp >= ilo - 4 && rp <= ihi + 4) {
-                iev = e == 1.0 ? 1.0 : 1.0 / e;
-                if (Lp != 0.0) {
-                    k1 = e * Lp * a.inv_reda;
-                    k2 = Lp * a.cfsd * sqrt(e);
+                iev = e == R(1) ? R(1) : R(1) / e;
+                if (Lp != R(0)) {
+                    k1 = e * Lp * R(a.inv_reda);
+                    k2 = Lp * R(a.cfsd) * sqrt(e);
                 }
             }
             ie.push(iev); ew.push(e); k1w.push(k1); k2w.push(k2);
             if (MUPOLY) {
-                const double tt = T_rp;
-                nuw.push(e * a.inv_re * ((((a.mu[4] * tt + a.mu[3]) * tt + a.mu[2]) * tt + a.mu[1]) * tt + a.mu[0]));
+                const R tt = T_rp;
+                nuw.push(e * inv_re * ((((R(a.mu[4]) * tt + R(a.mu[3])) * tt + R(a.mu[2])) * tt + R(a.mu[1])) * tt + R(a.mu[0])));
             }
-            dpj.push(p_rp - sP[s2(rp)][PAD + x - 1]);
-            const double Tm = sT[s2(rp)][PAD + x - 1];
-            bw.push(bouss ? a.ri * ((T_rp + Tm) * 0.5 - a.t_ref) : a.invfr2 * (1.0 - 2.0 / (T_rp + Tm)));
+            dpj.push(R(p_rp - sP[s2(rp)][PAD + x - 1]));
+            const R Tm = sT[s2(rp)][PAD + x - 1];
+            bw.push(bouss ? R(a.ri) * ((T_rp + Tm) * R(0.5) - R(a.t_ref)) : R(a.invfr2) * (R(1) - R(2) / (T_rp + Tm)));
             if (!STEADY) { du.push(um_rp - ut[2]); dv.push(vm_rp - vt[2]); }
         }
 
@@ -257,60 +284,60 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
         {                                                                                         \
             const int r = t - 2 * K;                                                              \
             const int bk = 2 * K - 2; /* back index of row r in the rows t-6..t-2 windows */      \
-            double val = 0.0, rs = 0.0;                                                           \
+            R val = R(0), rs = R(0);                                                              \
             bool interior = false;                                                                \
             if (r >= ilo - 2 * (3 - K) && r <= ihi + 2 * (3 - K)) {                               \
                 if (r >= 3 && r <= im - 1) {                                                      \
                     if (ucol) {                                                                   \
                         interior = true;                                                          \
-                        const double uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0]; \
-                        const double *c = &RING_C[PAD + x];                                       \
-                        const double uS = c[-1], uN = c[1];                                       \
-                        const double iev = ie[bk], ev = ew[bk];                                   \
-                        const double nu = MUPOLY ? nuw[bk] : ev * a.inv_re;                       \
-                        const double aun_r = pr[r], rdx_r = pr[2 * R + r];                        \
-                        const double cns = 0.5 * aun_r * iev, cew = haue_j * iev;                 \
-                        const double sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;      \
-                        const double fn = Svn[2 * K - 1] * cns;                                   \
-                        const double fs = (vt[2 * K] + vt[2 * K + 1]) * cns;                      \
-                        const double fe = sE * cew, fw = sW * cew;                                \
-                        const double diff = (nu * aue_j) * (pr[3 * R + r] * (uW - uP) + pr[3 * R + r + 1] * (uE - uP)) \
-                                          + (nu * aun_r) * (rdy_j * (uS - uP) + rdy_n * (uN - uP)); \
-                        double pw_, pe_, ps_, pn_;                                                \
+                        const R uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0];     \
+                        const R *c = &RING_C[PAD + x];                                            \
+                        const R uS = c[-1], uN = c[1];                                            \
+                        const R iev = ie[bk], ev = ew[bk];                                        \
+                        const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                              \
+                        const R aun_r = R(pr[r]), rdx_r = R(pr[2 * RR + r]);                      \
+                        const R cns = R(0.5) * aun_r * iev, cew = haue_j * iev;                   \
+                        const R sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;           \
+                        const R fn = Svn[2 * K - 1] * cns;                                        \
+                        const R fs = (vt[2 * K] + vt[2 * K + 1]) * cns;                           \
+                        const R fe = sE * cew, fw = sW * cew;                                     \
+                        const R diff = (nu * aue_j) * (R(pr[3 * RR + r]) * (uW - uP) + R(pr[3 * RR + r + 1]) * (uE - uP)) \
+                                     + (nu * aun_r) * (rdy_j * (uS - uP) + rdy_n * (uN - uP));    \
+                        R pw_, pe_, ps_, pn_;                                                     \
                         if (u_upw_col) {                                                          \
-                            pw_ = fw > 0.0 ? uW : uP; pe_ = fe > 0.0 ? uP : uE;                   \
-                            ps_ = fs > 0.0 ? uS : uP; pn_ = fn > 0.0 ? uP : uN;                   \
+                            pw_ = fw > R(0) ? uW : uP; pe_ = fe > R(0) ? uP : uE;                 \
+                            ps_ = fs > R(0) ? uS : uP; pn_ = fn > R(0) ? uP : uN;                 \
                         } else {                                                                  \
-                            const double uSS = c[-2], uNN = c[2];                                 \
-                            const double cW = uWW - 2.0 * uW + uP, cPi = uW - 2.0 * uP + uE, cE = uP - 2.0 * uE + uEE; \
-                            const double cS = uSS - 2.0 * uS + uP, cPj = uS - 2.0 * uP + uN, cN = uP - 2.0 * uN + uNN; \
+                            const R uSS = c[-2], uNN = c[2];                                      \
+                            const R cW = uWW - R(2) * uW + uP, cPi = uW - R(2) * uP + uE, cE = uP - R(2) * uE + uEE; \
+                            const R cS = uSS - R(2) * uS + uP, cPj = uS - R(2) * uP + uN, cN = uP - R(2) * uN + uNN; \
                             pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE); \
                             ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
                         }                                                                         \
-                        const double conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;            \
-                        double Rr = rdx_r * (rdy_j * (diff + conv) - ev * (pw[bk] - pw[bk + 1])) - a.g * ev; \
-                        const double k2 = k2w[bk];                                                \
-                        if (k2 != 0.0) {                                                          \
-                            const double vP = vt[2 * K];                                          \
+                        const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                 \
+                        R Rr = rdx_r * (rdy_j * (diff + conv) - ev * R(pw[bk] - pw[bk + 1])) - gg * ev; \
+                        const R k2 = k2w[bk];                                                     \
+                        if (k2 != R(0)) {                                                         \
+                            const R vP = vt[2 * K];                                               \
                             Rr -= uP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                  \
                         }                                                                         \
-                        rs = ((STEADY ? 0.0 : du[bk]) + Rr * dt) * dtau;                          \
-                        const double tau = ut[2 * K];                                             \
+                        rs = ((STEADY ? R(0) : du[bk]) + Rr * dt) * dtau;                         \
+                        const R tau = ut[2 * K];                                                  \
                         if (K == 1) val = tau + rs;                                               \
-                        else if (K == 2) val = 0.75 * tau + 0.25 * (uP + rs);                     \
-                        else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (uP + rs);                      \
+                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (uP + rs);               \
+                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (uP + rs);                 \
                     }                                                                             \
                 } else if (r <= 2) {                                                              \
-                    val = chan ? a.u_in : 0.0;                                                    \
+                    val = chan ? R(a.u_in) : R(0);                                                \
                 } else {                                                                          \
-                    val = (chan && ucol) ? WOUT[0] : 0.0;                                         \
+                    val = (chan && ucol) ? WOUT[0] : R(0);                                        \
                 }                                                                                 \
                 RING_OUT                                                                          \
             }                                                                                     \
             WOUT.push(val);                                                                       \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.un[(size_t)r * su + j] = val;                                                   \
-                const unsigned long long bb = dbits(rs); mu_ = bb > mu_ ? bb : mu_;               \
+                const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_;       \
             }                                                                                     \
         }
 
@@ -319,60 +346,60 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
         {                                                                                         \
             const int r = t - 2 * K;                                                              \
             const int bk = 2 * K - 2;                                                             \
-            double val = 0.0, rs = 0.0;                                                           \
+            R val = R(0), rs = R(0);                                                              \
             bool interior = false;                                                                \
             if (r >= ilo - 2 * (3 - K) && r <= ihi - 1 + 2 * (3 - K)) {                           \
                 if (j <= 2) {                                                                     \
-                    val = chan ? 0.0 : vi;                                                        \
+                    val = chan ? R(0) : vi;                                                       \
                 } else if (r >= 2 && r <= im - 1) {                                               \
                     if (vcol) {                                                                   \
                         interior = true;                                                          \
-                        const double vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0]; \
-                        const double *c = &RING_C[PAD + x];                                       \
-                        const double vS = c[-1], vN = c[1];                                       \
-                        const double iev = ie[bk], ev = ew[bk];                                   \
-                        const double nu = MUPOLY ? nuw[bk] : ev * a.inv_re;                       \
-                        const double avn_r = pr[R + r], rdx_r = pr[2 * R + r];                    \
-                        const double cns = 0.5 * avn_r * iev, cew = have_j * iev;                 \
-                        const double sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;      \
-                        const double fn = sN * cns, fs = sS * cns;                                \
-                        const double fe = Su[2 * K - 1] * cew, fw = Su[2 * K] * cew;              \
-                        const double diff = (nu * ave_j) * (rdx_r * (vW - vP) + pr[2 * R + r + 1] * (vE - vP)) \
-                                          + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP)); \
-                        double pw_, pe_, ps_, pn_;                                                \
+                        const R vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0];     \
+                        const R *c = &RING_C[PAD + x];                                            \
+                        const R vS = c[-1], vN = c[1];                                            \
+                        const R iev = ie[bk], ev = ew[bk];                                        \
+                        const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                              \
+                        const R avn_r = R(pr[RR + r]), rdx_r = R(pr[2 * RR + r]);                 \
+                        const R cns = R(0.5) * avn_r * iev, cew = have_j * iev;                   \
+                        const R sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;           \
+                        const R fn = sN * cns, fs = sS * cns;                                     \
+                        const R fe = Su[2 * K - 1] * cew, fw = Su[2 * K] * cew;                   \
+                        const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[2 * RR + r + 1]) * (vE - vP)) \
+                                     + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP));  \
+                        R pw_, pe_, ps_, pn_;                                                     \
                         if (upwind_all || r == 2 || r == im - 1) {                                \
-                            pw_ = fw > 0.0 ? vW : vP; pe_ = fe > 0.0 ? vP : vE;                   \
-                            ps_ = fs > 0.0 ? vS : vP; pn_ = fn > 0.0 ? vP : vN;                   \
+                            pw_ = fw > R(0) ? vW : vP; pe_ = fe > R(0) ? vP : vE;                 \
+                            ps_ = fs > R(0) ? vS : vP; pn_ = fn > R(0) ? vP : vN;                 \
                         } else {                                                                  \
-                            const double vSS = c[-2], vNN = c[2];                                 \
-                            const double cW = vWW - 2.0 * vW + vP, cPi = vW - 2.0 * vP + vE, cE = vP - 2.0 * vE + vEE; \
-                            const double cS = vSS - 2.0 * vS + vP, cPj = vS - 2.0 * vP + vN, cN = vP - 2.0 * vN + vNN; \
+                            const R vSS = c[-2], vNN = c[2];                                      \
+                            const R cW = vWW - R(2) * vW + vP, cPi = vW - R(2) * vP + vE, cE = vP - R(2) * vE + vEE; \
+                            const R cS = vSS - R(2) * vS + vP, cPj = vS - R(2) * vP + vN, cN = vP - R(2) * vN + vNN; \
                             pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE); \
                             ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
                         }                                                                         \
-                        const double conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;            \
-                        double Rr = rdy_j * (rdx_r * (diff + conv) - ev * dpj[bk]) + bw[bk];      \
-                        const double k2 = k2w[bk];                                                \
-                        if (k2 != 0.0) {                                                          \
-                            const double uP = ut[2 * K];                                          \
+                        const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                 \
+                        R Rr = rdy_j * (rdx_r * (diff + conv) - ev * dpj[bk]) + bw[bk];           \
+                        const R k2 = k2w[bk];                                                     \
+                        if (k2 != R(0)) {                                                         \
+                            const R uP = ut[2 * K];                                               \
                             Rr -= vP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                  \
                         }                                                                         \
-                        rs = ((STEADY ? 0.0 : dv[bk]) + Rr * dt) * dtau;                          \
-                        const double tau = vt[2 * K];                                             \
+                        rs = ((STEADY ? R(0) : dv[bk]) + Rr * dt) * dtau;                         \
+                        const R tau = vt[2 * K];                                                  \
                         if (K == 1) val = tau + rs;                                               \
-                        else if (K == 2) val = 0.75 * tau + 0.25 * (vP + rs);                     \
-                        else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (vP + rs);                       \
+                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (vP + rs);               \
+                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (vP + rs);                 \
                         if (r == 2 && !chan) WOUT.set(0, val);   /* row 1 = row 2 (boundary.f90:34-36) */ \
                     }                                                                             \
                 } else if (r == im) {                                                             \
-                    val = vcol ? WOUT[0] : 0.0;                                                   \
+                    val = vcol ? WOUT[0] : R(0);                                                  \
                 }                                                                                 \
                 RING_OUT                                                                          \
             }                                                                                     \
             WOUT.push(val);                                                                       \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.vn[(size_t)r * sv + j] = val;                                                   \
-                const unsigned long long bb = dbits(rs); mv_ = bb > mv_ ? bb : mv_;               \
+                const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_;       \
             }                                                                                     \
         }
 
@@ -385,16 +412,16 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
 #undef U_STAGE
 #undef V_STAGE
 
-        // ---------------- P at row r = t - 7 (equations.f90:620-658)
+        // ---------------- P at row r = t - 7 (equations.f90:620-658), FP64 accumulation
         {
             const int r = t - 7;
             if (r >= 2 && r <= im - 1 && ucol && out_col && r >= ilo && r < ihi) {
-                const double u3p = w3[1], u3e = w3[0];
-                const double v3p = x3[1];
-                const double v3n = lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
-                const double avn_r = pr[R + r];
-                const double rpv = -((u3e * aue_j - u3p * aue_j) + (v3n * avn_r - v3p * avn_r));
-                const double dd = dtau * rpv * a.beta;
+                const double u3p = (double)w3[1], u3e = (double)w3[0];
+                const double v3p = (double)x3[1];
+                const double v3n = (double)lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
+                const double avn_r = pr[RR + r];
+                const double rpv = -((u3e * aue_jd - u3p * aue_jd) + (v3n * avn_r - v3p * avn_r));
+                const double dd = a.dtau * rpv * a.beta;
                 const double pp = pw[5];
                 double pi = pp + dd;
                 pi = 0.75 * pp + 0.25 * (pi + dd);
@@ -408,6 +435,36 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args a)
     fold_max(a.res + 2, mp_);
 }
 
+// OCC: waves per SIMD the register budget is sized for (1: no spills for FP64; 2: 256
+// registers). The autotuner picks the faster one per grid.
+template <typename R, bool STEADY, bool MUPOLY, int OCC>
+__global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args<R> a)
+{
+    __shared__ Smem1<R> sm;
+    const int G = a.nbi * a.nbj, per = (G + 7) / 8;
+    const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
+    if (L >= G) return;
+    uvp_tile<R, STEADY, MUPOLY>(a, L / a.nbj, L % a.nbj, sm);
+}
+
+// Persistent batch: `cases` holds two argument sets per case (even and odd tau parity);
+// ctl[c] bit 0 = active, bit 1 = parity. Every block walks the tile list with a stride of
+// the grid size; inactive cases are skipped.
+template <typename R, bool STEADY, bool MUPOLY, int OCC>
+__global__ __launch_bounds__(NT1, OCC) void k_uvp_batch(const K1Args<R> *cases, const BatchTile *tiles,
+                                                        int ntiles, const int *ctl)
+{
+    __shared__ Smem1<R> sm;
+    for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        const BatchTile tl = tiles[w];
+        const int cs = ctl[tl.c];
+        if (!(cs & 1)) continue;
+        const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
+        uvp_tile<R, STEADY, MUPOLY>(a, tl.bi, tl.bj, sm);
+        __syncthreads();   // the next tile re-initialises the LDS rings
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // K2: T (energy, 1/Pe) and C (species, 1/(Re Sc)), three RK stages each.
 // ---------------------------------------------------------------------------------------
@@ -416,90 +473,97 @@ constexpr int HL2 = 3, HR2 = 3;
 constexpr int W2 = NT2 - HL2 - HR2;
 constexpr int LROW2 = NT2 + 2 * PAD;
 
+template <typename R>
 struct K2Args {
-    const double *un, *vn, *T, *Tt, *C, *Ct, *eps, *lp, *y;
-    double *Tn, *Cn;
+    const R *un, *vn, *T, *Tt, *C, *Ct, *eps, *lp;
+    const double *y;
+    R *Tn, *Cn;
     const double *pk;
     int imax, jmax, nbi, nbj, hb, chan;
     double dt, dtau, kt, kc, tinf, t_bottom, t_top, v_i, u_in, slot_lo, slot_hi, slot_value;
 };
 
+template <typename R>
+struct Smem2 {
+    R sTT[3][LROW2], sCT[3][LROW2], sV[2][LROW2];
+    R sZ1[2][LROW2], sZ2[2][LROW2], sC1[2][LROW2], sC2[2][LROW2];
+};
+
 // T of the centre row at column j+dj (boundary.f90:85-88 / channel wall temperatures).
-__device__ __forceinline__ double lds_t(const double *row, int x, int j, int dj, int jm, bool chan, const K2Args &a)
+template <typename R>
+__device__ __forceinline__ R lds_t(const R *row, int x, int j, int dj, int jm, bool chan, R tinf, R tb, R ttop)
 {
     const int jj = j + dj;
-    if (jj == 1) return chan ? a.t_bottom : a.tinf;
+    if (jj == 1) return chan ? tb : tinf;
     if (jj == jm) {
-        if (chan) return a.t_top;
-        const double a1 = row[PAD + x + dj - 1], a2 = row[PAD + x + dj - 2];
-        return a1 + 1.0 * (a1 - a2);
+        if (chan) return ttop;
+        const R a1 = row[PAD + x + dj - 1], a2 = row[PAD + x + dj - 2];
+        return a1 + R(1) * (a1 - a2);
     }
     return row[PAD + x + dj];
 }
 
 // C of the centre row (boundary.f90:128-131 / channel: zero-gradient walls).
-__device__ __forceinline__ double lds_c(const double *row, int x, int j, int dj, int jm, bool chan)
+template <typename R>
+__device__ __forceinline__ R lds_c(const R *row, int x, int j, int dj, int jm, bool chan)
 {
     const int jj = j + dj;
-    if (jj == 1) return chan ? row[PAD + x + dj + 1] : 0.0;
+    if (jj == 1) return chan ? row[PAD + x + dj + 1] : R(0);
     if (jj == jm) {
         if (chan) return row[PAD + x + dj - 1];
-        const double a1 = row[PAD + x + dj - 1], a2 = row[PAD + x + dj - 2];
-        return a1 + 1.0 * (a1 - a2);
+        const R a1 = row[PAD + x + dj - 1], a2 = row[PAD + x + dj - 2];
+        return a1 + R(1) * (a1 - a2);
     }
     return row[PAD + x + dj];
 }
 
-template <bool STEADY, int OCC>
-__global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
+template <typename R, bool STEADY>
+__device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Smem2<R> &sm)
 {
-    const int G = a.nbi * a.nbj, per = (G + 7) / 8;
-    const int b = blockIdx.x, L = (b % 8) * per + b / 8;
-    if (L >= G) return;
-    const int bi = L / a.nbj, bj = L % a.nbj;
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);
     const int jlo = 2 + bj * W2, jhi = min(jlo + W2, jm);
     const int x = threadIdx.x;
     const int j = jlo - HL2 + x;
     const bool chan = a.chan != 0;
-    const double dt = a.dt, dtau = a.dtau, kt = a.kt, kc = a.kc;
-    const int R = im + 2, Cc = jm + 2;
-    const double *pr = a.pk, *pc = a.pk + 4 * R;
+    const R dt = R(a.dt), dtau = R(a.dtau), kt = R(a.kt), kc = R(a.kc);
+    const R tinf = R(a.tinf), tb = R(a.t_bottom), ttop = R(a.t_top);
+    const int RR = im + 2, Cc = jm + 2;
+    const double *pr = a.pk, *pc = a.pk + 4 * RR;
+    auto sTT = sm.sTT; auto sCT = sm.sCT; auto sV = sm.sV;
+    auto sZ1 = sm.sZ1; auto sZ2 = sm.sZ2; auto sC1 = sm.sC1; auto sC2 = sm.sC2;
 
-    __shared__ double sTT[3][LROW2], sCT[3][LROW2], sV[2][LROW2];
-    __shared__ double sZ1[2][LROW2], sZ2[2][LROW2], sC1[2][LROW2], sC2[2][LROW2];
     for (int k = x; k < LROW2; k += NT2) {
-        for (int q = 0; q < 3; ++q) { sTT[q][k] = 0.0; sCT[q][k] = 0.0; }
-        for (int q = 0; q < 2; ++q) { sV[q][k] = 0.0; sZ1[q][k] = 0.0; sZ2[q][k] = 0.0; sC1[q][k] = 0.0; sC2[q][k] = 0.0; }
+        for (int q = 0; q < 3; ++q) { sTT[q][k] = R(0); sCT[q][k] = R(0); }
+        for (int q = 0; q < 2; ++q) { sV[q][k] = R(0); sZ1[q][k] = R(0); sZ2[q][k] = R(0); sC1[q][k] = R(0); sC2[q][k] = R(0); }
     }
     const bool jvalid = j >= 1 && j <= jm;
     const int jc = min(max(j, 2), jm - 1);
-    const double aue_j = pc[min(max(j, 1), jm)];
-    const double rdy_j = pc[2 * Cc + jc], rdy_n = pc[2 * Cc + jc + 1];
-    const double rdym_n = pc[3 * Cc + jc + 1];
+    const R aue_j = R(pc[min(max(j, 1), jm)]);
+    const R rdy_j = R(pc[2 * Cc + jc]), rdy_n = R(pc[2 * Cc + jc + 1]);
+    const R rdym_n = R(pc[3 * Cc + jc + 1]);
     const bool scol = j >= 2 && j <= jm - 1;
     const bool out_col = j >= jlo && j < jhi;
     const double yj = a.y[min(max(j, 1), jm)];
-    const double cslot = chan ? ((yj >= a.slot_lo && yj <= a.slot_hi) ? a.slot_value : 0.0) : 0.0;
+    const R cslot = chan ? ((yj >= a.slot_lo && yj <= a.slot_hi) ? R(a.slot_value) : R(0)) : R(0);
 
-    Win<4> tt, ct, uw, vw, vnw, id, oml, dT, dC;
-    Win<3> z1, z2, c1, c2, z3, c3;
-    tt.fill(0.0); ct.fill(0.0); uw.fill(0.0); vw.fill(0.0); vnw.fill(0.0); id.fill(1.0); oml.fill(1.0);
-    if (!STEADY) { dT.fill(0.0); dC.fill(0.0); }
-    z1.fill(0.0); z2.fill(0.0); c1.fill(0.0); c2.fill(0.0); z3.fill(0.0); c3.fill(0.0);
+    Win<4, R> tt, ct, uw, vw, vnw, id, oml, dT, dC;
+    Win<3, R> z1, z2, c1, c2, z3, c3;
+    tt.fill(R(0)); ct.fill(R(0)); uw.fill(R(0)); vw.fill(R(0)); vnw.fill(R(0)); id.fill(R(1)); oml.fill(R(1));
+    if (!STEADY) { dT.fill(R(0)); dC.fill(R(0)); }
+    z1.fill(R(0)); z2.fill(R(0)); c1.fill(R(0)); c2.fill(R(0)); z3.fill(R(0)); c3.fill(R(0));
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
-    double q_t, q_c, q_u, q_v, q_T, q_C, q_e, q_L;
+    R q_t, q_c, q_u, q_v, q_T, q_C, q_e, q_L;
     // next row: u3/v3 from K1's interior cells, boundary rows/columns from the BC formulas
     auto fetch = [&](int t) {
-        q_t = 0.0; q_c = 0.0; q_u = 0.0; q_v = 0.0; q_T = 0.0; q_C = 0.0; q_e = 1.0; q_L = 0.0;
+        q_t = R(0); q_c = R(0); q_u = R(0); q_v = R(0); q_T = R(0); q_C = R(0); q_e = R(1); q_L = R(0);
         if (jvalid && t >= 1 && t <= im + 1) {
             if (t <= im) {
                 const size_t k = (size_t)t * su + j;
                 q_t = a.Tt[k]; q_c = a.Ct[k]; q_e = a.eps[k]; q_L = a.lp[k];
                 if (!STEADY) { q_T = a.T[k]; q_C = a.C[k]; }
-                if (j == 2) q_v = chan ? 0.0 : a.v_i;
+                if (j == 2) q_v = chan ? R(0) : R(a.v_i);
                 else if (t >= 2 && t <= im - 1 && j <= jm - 1) q_v = a.vn[(size_t)t * sv + j];
             }
             if (t > 2 && t < im) q_u = a.un[(size_t)t * su + j];
@@ -510,11 +574,11 @@ __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
 
     for (int t = ilo - 3; t <= ihi + 2; ++t) {
         // ---------------- phase A: row t
-        const double tnew = q_t, cnew = q_c, vnew = q_v, Tp = q_T, Cp = q_C, e = q_e, Lp = q_L;
-        double unew = q_u;
+        const R tnew = q_t, cnew = q_c, vnew = q_v, Tp = q_T, Cp = q_C, e = q_e, Lp = q_L;
+        R unew = q_u;
         fetch(t + 1);
-        if (t <= 2) unew = chan ? a.u_in : 0.0;             // u3 boundary rows (boundary.f90:29-32,39-42)
-        else if (t >= im) unew = chan ? uw[0] : 0.0;
+        if (t <= 2) unew = chan ? R(a.u_in) : R(0);         // u3 boundary rows (boundary.f90:29-32,39-42)
+        else if (t >= im) unew = chan ? uw[0] : R(0);
         tt.push(tnew); ct.push(cnew); uw.push(unew); vw.push(vnew);
         sTT[s3(t)][PAD + x] = tnew;
         sCT[s3(t)][PAD + x] = cnew;
@@ -523,11 +587,11 @@ __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
 
         // ---------------- phase B: per-cell data of row t
         {
-            double vn_ = sV[s2(t)][PAD + x + 1];
-            if (j + 1 == jm) vn_ = chan ? 0.0 : 2.0 * sV[s2(t)][PAD + x - 1] - sV[s2(t)][PAD + x];
+            R vn_ = sV[s2(t)][PAD + x + 1];
+            if (j + 1 == jm) vn_ = chan ? R(0) : R(2) * sV[s2(t)][PAD + x - 1] - sV[s2(t)][PAD + x];
             vnw.push(vn_);
-            id.push(Lp != 0.0 ? 1.0 / (Lp * (e - 1.0) + 1.0) : 1.0);
-            oml.push(1.0 - Lp);
+            id.push(Lp != R(0) ? R(1) / (Lp * (e - R(1)) + R(1)) : R(1));
+            oml.push(R(1) - Lp);
             if (!STEADY) { dT.push(Tp - tnew); dC.push(Cp - cnew); }
         }
 
@@ -536,36 +600,36 @@ __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
         {                                                                                         \
             const int r = t - K;                                                                  \
             const int bq = K;            /* back index of row r in windows whose newest is t */   \
-            double val = 0.0;                                                                     \
+            R val = R(0);                                                                         \
             bool interior = false;                                                                \
             if (r >= ilo - (3 - K) && r <= ihi - 1 + (3 - K)) {                                   \
                 if (r >= 2 && r <= im - 1) {                                                      \
                     if (scol) {                                                                   \
                         interior = true;                                                          \
-                        const double zE = WP[0], zP = WP[1], zW = WP[2];                          \
-                        const double *c = CROW;                                                   \
-                        const double zS = ISC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, a); \
-                        const double zN = ISC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, a);   \
-                        const double avn_r = pr[R + r];                                           \
-                        const double diff = (KAPPA * aue_j) * (pr[2 * R + r + 1] * (zE - zP) + pr[2 * R + r] * (zW - zP)) \
-                                          + (KAPPA * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP)); \
-                        const double uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];     \
-                        const double conv = 0.5 * ((zE + zP) * uE - (zW + zP) * uP) * aue_j       \
-                                          + 0.5 * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;      \
-                        const double vols = pr[3 * R + r + 1] * rdym_n;                           \
-                        const double Rr = ISC ? vols * (diff - conv * id[bq])                     \
-                                              : vols * (diff - oml[bq] * conv) * id[bq];          \
-                        const double rs = ((STEADY ? 0.0 : DW[bq]) + Rr * dt) * dtau;             \
-                        const double tau = TAUW[bq];                                              \
+                        const R zE = WP[0], zP = WP[1], zW = WP[2];                               \
+                        const R *c = CROW;                                                        \
+                        const R zS = ISC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, tinf, tb, ttop); \
+                        const R zN = ISC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, tinf, tb, ttop);   \
+                        const R avn_r = R(pr[RR + r]);                                            \
+                        const R diff = (KAPPA * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
+                                     + (KAPPA * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP)); \
+                        const R uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];          \
+                        const R conv = R(0.5) * ((zE + zP) * uE - (zW + zP) * uP) * aue_j         \
+                                     + R(0.5) * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;        \
+                        const R vols = R(pr[3 * RR + r + 1]) * rdym_n;                            \
+                        const R Rr = ISC ? vols * (diff - conv * id[bq])                          \
+                                         : vols * (diff - oml[bq] * conv) * id[bq];               \
+                        const R rs = ((STEADY ? R(0) : DW[bq]) + Rr * dt) * dtau;                 \
+                        const R tau = TAUW[bq];                                                   \
                         if (K == 1) val = tau + rs;                                               \
-                        else if (K == 2) val = 0.75 * tau + 0.25 * (zP + rs);                     \
-                        else val = 1.0 / 3.0 * tau + 2.0 / 3.0 * (zP + rs);                       \
+                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (zP + rs);               \
+                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (zP + rs);                 \
                         if (r == 2) WOUT.set(0, (ISC && chan) ? cslot : val);  /* row 1 */        \
                     }                                                                             \
                 } else if (r == im) {                                                             \
-                    val = scol ? WOUT[0] : 0.0;                                                   \
+                    val = scol ? WOUT[0] : R(0);                                                  \
                 } else if (r == 1) {                                                              \
-                    val = (ISC && chan) ? cslot : 0.0;                                            \
+                    val = (ISC && chan) ? cslot : R(0);                                           \
                 }                                                                                 \
                 RINGW                                                                             \
             }                                                                                     \
@@ -574,7 +638,7 @@ __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
                 OUTP[(size_t)r * su + j] = val;                                                   \
         }
 
-        double *const noout = nullptr;
+        R *const noout = nullptr;
         SCAL_STAGE(1, tt, sTT[s3(t - 1)], false, z1, { sZ1[s2(t - 1)][PAD + x] = val; }, noout, kt, dT, tt)
         SCAL_STAGE(1, ct, sCT[s3(t - 1)], true, c1, { sC1[s2(t - 1)][PAD + x] = val; }, noout, kc, dC, ct)
         SCAL_STAGE(2, z1, sZ1[s2(t - 2)], false, z2, { sZ2[s2(t - 2)][PAD + x] = val; }, noout, kt, dT, tt)
@@ -584,14 +648,106 @@ __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args a)
 #undef SCAL_STAGE
     }
 }
+
+template <typename R, bool STEADY, int OCC>
+__global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args<R> a)
+{
+    __shared__ Smem2<R> sm;
+    const int G = a.nbi * a.nbj, per = (G + 7) / 8;
+    const int b = blockIdx.x, L = (b % 8) * per + b / 8;
+    if (L >= G) return;
+    scal_tile<R, STEADY>(a, L / a.nbj, L % a.nbj, sm);
+}
+
+template <typename R, bool STEADY, int OCC>
+__global__ __launch_bounds__(NT2, OCC) void k_scal_batch(const K2Args<R> *cases, const BatchTile *tiles,
+                                                         int ntiles, const int *ctl)
+{
+    __shared__ Smem2<R> sm;
+    for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        const BatchTile tl = tiles[w];
+        const int cs = ctl[tl.c];
+        if (!(cs & 1)) continue;
+        const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
+        scal_tile<R, STEADY>(a, tl.bi, tl.bj, sm);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// boundary cells of the five outputs
+// ---------------------------------------------------------------------------------------
+__host__ __device__ inline int materialize_count(int im, int jm)
+{
+    return 2 * (im + 1) + 4 * (jm - 2) + 4 * im + 2 * (jm - 3) + 3 * (2 * im + 2 * (jm - 2));
+}
+
+// Boundary cell number t of a fused step's outputs, from their interior values. The
+// reference v corner (imax, jmax+1) reads the pre-BC column imax, which in the reference
+// holds the previous step's output: that is the step's input iterate vprev.
+template <typename F>
+__device__ __forceinline__ void materialize_cell(const Geo &g, const cfd_params &P, F *u, F *v, double *p, F *T,
+                                                 F *C, const F *vprev, int t)
+{
+    const int im = g.imax, jm = g.jmax;
+    const bool chan = P.bc == CFD_BC_CHANNEL;
+    const int nu_r = im + 1, nu_c = jm - 2;
+    const int n_u = 2 * nu_r + 4 * nu_c;
+    const int n_v = 4 * im + 2 * (jm - 3);
+    const int n_s = 2 * im + 2 * (jm - 2);
+    if (t < n_u) {
+        int i, j;
+        if (t < 2 * nu_r) { i = 1 + t % nu_r; j = t < nu_r ? 1 : jm; }
+        else { const int q = t - 2 * nu_r, c = q / nu_c; j = 2 + q % nu_c; i = c == 0 ? 1 : (c == 1 ? 2 : (c == 2 ? im : im + 1)); }
+        u[iu(g, i, j)] = (F)(chan ? bc_u_chan(g, P, u, i, j) : bc_u_ref(g, u, i, j));
+        return;
+    }
+    int q = t - n_u;
+    if (q < n_v) {
+        int i, j;
+        if (q < 4 * im) { const int r = q / im; i = 1 + q % im; j = r == 0 ? 1 : (r == 1 ? 2 : (r == 2 ? jm : jm + 1)); }
+        else { const int c = q - 4 * im, nc = jm - 3; i = c < nc ? 1 : im; j = 3 + c % nc; }
+        double val;
+        if (!chan && i == im && j == jm + 1) {
+            const double a = (double)vprev[iv(g, im, jm - 2)], b = (double)vprev[iv(g, im, jm - 1)];
+            val = 2.0 * b - (2.0 * a - b);
+        } else {
+            val = chan ? bc_v_chan(g, v, i, j) : bc_v_ref(g, P, v, i, j);
+        }
+        v[iv(g, i, j)] = (F)val;
+        return;
+    }
+    q -= n_v;
+    if (q < 3 * n_s) {
+        const int kind = q / n_s, r = q % n_s;
+        int i, j;
+        if (r < 2 * im) { i = 1 + r % im; j = r < im ? 1 : jm; }
+        else { const int c = r - 2 * im, nc = jm - 2; i = c < nc ? 1 : im; j = 2 + c % nc; }
+        if (kind == 0) {
+            p[iu(g, i, j)] = chan ? bc_s_chan(g, P, 0, p, i, j) : bc_s_ref(g, P, 0, p, i, j);
+        } else {
+            F *a = kind == 1 ? T : C;
+            a[iu(g, i, j)] = (F)(chan ? bc_s_chan(g, P, kind, a, i, j) : bc_s_ref(g, P, kind, a, i, j));
+        }
+    }
+}
+
+template <typename F>
+__global__ void k_materialize(Geo g, cfd_params P, F *u, F *v, double *p, F *T, F *C, const F *vprev)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    materialize_cell(g, P, u, v, p, T, C, vprev, t);
+}
+
 }  // namespace
 
-extern "C" {
-
-int cfdk_fused_available(void) { return 1; }
+// ---------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------
+namespace {
 
 // Row-block height: `want` > 0 (from the autotuner) or one block per CU, at least min_rows.
-static int pick_rows(int rows, int nbj, int min_rows, int want, int *hb)
+int pick_rows(int rows, int nbj, int min_rows, int want, int *hb)
 {
     if (want > 0) {
         *hb = want < rows ? want : rows;
@@ -604,18 +760,14 @@ static int pick_rows(int rows, int nbj, int min_rows, int want, int *hb)
     return cdiv(rows, *hb);
 }
 
-int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P,
-                    const cfd_fused_args *a, int kernel, hipStream_t s)
+template <typename R>
+void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params *P, const cfd_fused_args *a)
 {
-    (void)kernel;
-    const int im = g->imax, jm = g->jmax;
-    if (P->b_art != 0.0 || P->compat != CFD_COMPAT_FORTRAN) return CFD_ERR_UNSUPPORTED;
-    // K1
-    K1Args k1;
-    k1.um = a->um; k1.vm = a->vm; k1.ut = a->um_tau; k1.vt = a->vm_tau; k1.p = a->p; k1.T = a->T;
-    k1.eps = g->eps; k1.lp = g->lp;
-    k1.un = a->um_n; k1.vn = a->vm_n; k1.pn = a->pn; k1.res = (unsigned long long *)a->res_max;
-    k1.pk = f->pack;
+    k1.um = (const R *)a->um; k1.vm = (const R *)a->vm; k1.ut = (const R *)a->um_tau; k1.vt = (const R *)a->vm_tau;
+    k1.p = a->p; k1.T = (const R *)a->T;
+    k1.eps = (const R *)a->epsr; k1.lp = (const R *)a->lpr;
+    k1.un = (R *)a->um_n; k1.vn = (R *)a->vm_n; k1.pn = a->pn; k1.res = (unsigned long long *)a->res_max;
+    k1.pk = pack;
     k1.imax = im; k1.jmax = jm;
     k1.chan = P->bc == CFD_BC_CHANNEL; k1.upwind_all = P->scheme == CFD_SCHEME_UPWIND;
     k1.bouss = P->buoyancy == CFD_BUOY_BOUSSINESQ;
@@ -625,59 +777,104 @@ int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_para
     for (int q = 0; q < 5; ++q) k1.mu[q] = P->mu_coef[q];
     k1.nbj = cdiv(jm - 2, W1);
     k1.nbi = pick_rows(im - 2, k1.nbj, 24, a->hb1, &k1.hb);
-    const int G1 = k1.nbi * k1.nbj, grid1 = ((G1 + 7) / 8) * 8;
-    const bool mp = P->mu_model == CFD_MU_POLY4;
-    const bool timed = a->ev[0] != nullptr;
-    if (timed) hipEventRecord(a->ev[0], s);
-    const int o1 = a->occ1 == 2 ? 2 : 1;
-    const dim3 g1(grid1), b1(NT1);
-    if (P->steady) {
-        if (mp) {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<true, true, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<true, true, 1>), g1, b1, 0, s, k1);
-        } else {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<true, false, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<true, false, 1>), g1, b1, 0, s, k1);
-        }
-    } else {
-        if (mp) {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<false, true, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<false, true, 1>), g1, b1, 0, s, k1);
-        } else {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<false, false, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<false, false, 1>), g1, b1, 0, s, k1);
-        }
-    }
-    int r = launched();
-    if (r != CFD_OK) return r;
-    if (timed) hipEventRecord(a->ev[1], s);
-    // K2
-    K2Args k2;
-    k2.un = a->um_n; k2.vn = a->vm_n; k2.T = a->T; k2.Tt = a->T_tau; k2.C = a->C; k2.Ct = a->C_tau;
-    k2.eps = g->eps; k2.lp = g->lp; k2.y = g->y;
-    k2.Tn = a->T_n; k2.Cn = a->C_n;
-    k2.pk = f->pack;
+}
+
+template <typename R>
+void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack, const cfd_params *P,
+             const cfd_fused_args *a)
+{
+    k2.un = (const R *)a->um_n; k2.vn = (const R *)a->vm_n; k2.T = (const R *)a->T; k2.Tt = (const R *)a->T_tau;
+    k2.C = (const R *)a->C; k2.Ct = (const R *)a->C_tau;
+    k2.eps = (const R *)a->epsr; k2.lp = (const R *)a->lpr; k2.y = y;
+    k2.Tn = (R *)a->T_n; k2.Cn = (R *)a->C_n;
+    k2.pk = pack;
     k2.imax = im; k2.jmax = jm; k2.chan = P->bc == CFD_BC_CHANNEL;
     k2.dt = P->dt; k2.dtau = P->dtau; k2.kt = 1.0 / P->pe; k2.kc = 1.0 / P->re / P->sc;
     k2.tinf = P->tinf; k2.t_bottom = P->t_bottom; k2.t_top = P->t_top; k2.v_i = P->v_i; k2.u_in = P->u_in;
     k2.slot_lo = P->slot_lo; k2.slot_hi = P->slot_hi; k2.slot_value = P->slot_value;
     k2.nbj = cdiv(jm - 2, W2);
     k2.nbi = pick_rows(im - 2, k2.nbj, 16, a->hb2, &k2.hb);
+}
+
+template <typename R>
+void launch_k1(const K1Args<R> &k1, const cfd_params *P, int occ, hipStream_t s)
+{
+    const int G1 = k1.nbi * k1.nbj, grid1 = ((G1 + 7) / 8) * 8;
+    const bool mp = P->mu_model == CFD_MU_POLY4;
+    const dim3 g1(grid1), b1(NT1);
+    const int o1 = occ == 2 ? 2 : 1;
+    if (P->steady) {
+        if (mp) {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, true, true, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<R, true, true, 1>), g1, b1, 0, s, k1);
+        } else {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, true, false, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<R, true, false, 1>), g1, b1, 0, s, k1);
+        }
+    } else {
+        if (mp) {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, false, true, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<R, false, true, 1>), g1, b1, 0, s, k1);
+        } else {
+            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, false, false, 2>), g1, b1, 0, s, k1);
+            else hipLaunchKernelGGL((k_uvp<R, false, false, 1>), g1, b1, 0, s, k1);
+        }
+    }
+}
+
+template <typename R>
+void launch_k2(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
+{
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
     const dim3 g2(grid2), b2(NT2);
-    if (a->occ2 == 4) {
-        if (P->steady) hipLaunchKernelGGL((k_scal<true, 4>), g2, b2, 0, s, k2);
-        else hipLaunchKernelGGL((k_scal<false, 4>), g2, b2, 0, s, k2);
+    if (occ == 4) {
+        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<R, false, 4>), g2, b2, 0, s, k2);
     } else {
-        if (P->steady) hipLaunchKernelGGL((k_scal<true, 1>), g2, b2, 0, s, k2);
-        else hipLaunchKernelGGL((k_scal<false, 1>), g2, b2, 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 1>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<R, false, 1>), g2, b2, 0, s, k2);
     }
+}
+
+template <typename R>
+int fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P, const cfd_fused_args *a,
+               hipStream_t s)
+{
+    const bool timed = a->ev[0] != nullptr;
+    K1Args<R> k1;
+    fill_k1<R>(k1, g->imax, g->jmax, f->pack, P, a);
+    if (timed) (void)hipEventRecord(a->ev[0], s);
+    launch_k1<R>(k1, P, a->occ1, s);
+    int r = launched();
+    if (r != CFD_OK) return r;
+    if (timed) (void)hipEventRecord(a->ev[1], s);
+    K2Args<R> k2;
+    fill_k2<R>(k2, g->imax, g->jmax, g->y, f->pack, P, a);
+    launch_k2<R>(k2, P, a->occ2, s);
     r = launched();
     if (r != CFD_OK) return r;
-    if (timed) hipEventRecord(a->ev[2], s);
-    r = cfdk_materialize_bc(g, P, a->um_n, a->vm_n, a->pn, a->T_n, a->C_n, a->vm_tau, s);
-    if (timed) hipEventRecord(a->ev[3], s);
+    if (timed) (void)hipEventRecord(a->ev[2], s);
+    const int n = materialize_count(g->imax, g->jmax);
+    hipLaunchKernelGGL((k_materialize<R>), dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), *P, (R *)a->um_n,
+                       (R *)a->vm_n, a->pn, (R *)a->T_n, (R *)a->C_n, (const R *)a->vm_tau);
+    r = launched();
+    if (timed) (void)hipEventRecord(a->ev[3], s);
     return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfdk_fused_available(void) { return 1; }
+
+int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P,
+                    const cfd_fused_args *a, int kernel, hipStream_t s)
+{
+    (void)kernel;
+    if (P->b_art != 0.0 || P->compat != CFD_COMPAT_FORTRAN) return CFD_ERR_UNSUPPORTED;
+    if (a->prec == CFD_PREC_FP32) return fused_step<float>(g, f, P, a, s);
+    return fused_step<double>(g, f, P, a, s);
 }
 
 }  // extern "C"
@@ -713,50 +910,22 @@ __global__ void k_factors(Geo g, double *rdx, double *rdy, double *rdxm, double 
     }
 }
 
-// Boundary cells of the five outputs of a fused step, from their interior values. The
-// reference v corner (imax, jmax+1) reads the pre-BC column imax, which in the reference
-// holds the previous step's output: that is the step's input iterate vm_prev.
-__global__ void k_materialize(Geo g, cfd_params P, double *u, double *v, double *p, double *T,
-                              double *C, const double *vprev)
+__global__ void k_d2f(float *dst, const double *src, long n)
 {
-    const int im = g.imax, jm = g.jmax;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool chan = P.bc == CFD_BC_CHANNEL;
-    const int nu_r = im + 1, nu_c = jm - 2;
-    const int n_u = 2 * nu_r + 4 * nu_c;
-    const int n_v = 4 * im + 2 * (jm - 3);
-    const int n_s = 2 * im + 2 * (jm - 2);
-    if (t < n_u) {
-        int i, j;
-        if (t < 2 * nu_r) { i = 1 + t % nu_r; j = t < nu_r ? 1 : jm; }
-        else { const int q = t - 2 * nu_r, c = q / nu_c; j = 2 + q % nu_c; i = c == 0 ? 1 : (c == 1 ? 2 : (c == 2 ? im : im + 1)); }
-        u[iu(g, i, j)] = chan ? bc_u_chan(g, P, u, i, j) : bc_u_ref(g, u, i, j);
-        return;
-    }
-    int q = t - n_u;
-    if (q < n_v) {
-        int i, j;
-        if (q < 4 * im) { const int r = q / im; i = 1 + q % im; j = r == 0 ? 1 : (r == 1 ? 2 : (r == 2 ? jm : jm + 1)); }
-        else { const int c = q - 4 * im, nc = jm - 3; i = c < nc ? 1 : im; j = 3 + c % nc; }
-        double val;
-        if (!chan && i == im && j == jm + 1) {
-            const double a = vprev[iv(g, im, jm - 2)], b = vprev[iv(g, im, jm - 1)];
-            val = 2.0 * b - (2.0 * a - b);
-        } else {
-            val = chan ? bc_v_chan(g, v, i, j) : bc_v_ref(g, P, v, i, j);
-        }
-        v[iv(g, i, j)] = val;
-        return;
-    }
-    q -= n_v;
-    if (q < 3 * n_s) {
-        const int kind = q / n_s, r = q % n_s;
-        double *a = kind == 0 ? p : (kind == 1 ? T : C);
-        int i, j;
-        if (r < 2 * im) { i = 1 + r % im; j = r < im ? 1 : jm; }
-        else { const int c = r - 2 * im, nc = jm - 2; i = c < nc ? 1 : im; j = 2 + c % nc; }
-        a[iu(g, i, j)] = chan ? bc_s_chan(g, P, kind, a, i, j) : bc_s_ref(g, P, kind, a, i, j);
-    }
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        dst[t] = (float)src[t];
+}
+
+__global__ void k_f2d(double *dst, const float *src, long n)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        dst[t] = (double)src[t];
+}
+
+inline int conv_blocks(long n)
+{
+    const long b = (n + 255) / 256;
+    return (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
 }
 }  // namespace
 
@@ -772,9 +941,270 @@ int cfdk_factors(const cfd_devgeo *g, cfd_devfactors *f, hipStream_t s)
 int cfdk_materialize_bc(const cfd_devgeo *g, const cfd_params *P, double *um_n, double *vm_n,
                         double *pn, double *T_n, double *C_n, const double *vm_prev, hipStream_t s)
 {
-    const int im = g->imax, jm = g->jmax;
-    const int n = 2 * (im + 1) + 4 * (jm - 2) + 4 * im + 2 * (jm - 3) + 3 * (2 * im + 2 * (jm - 2));
-    hipLaunchKernelGGL(k_materialize, dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), *P, um_n, vm_n, pn, T_n, C_n, vm_prev);
+    const int n = materialize_count(g->imax, g->jmax);
+    hipLaunchKernelGGL((k_materialize<double>), dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), *P, um_n, vm_n, pn,
+                       T_n, C_n, vm_prev);
+    return launched();
+}
+
+int cfdk_d2f(float *dst, const double *src, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_d2f, dim3(conv_blocks(n)), dim3(256), 0, s, dst, src, n);
+    return launched();
+}
+
+int cfdk_f2d(double *dst, const float *src, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_f2d, dim3(conv_blocks(n)), dim3(256), 0, s, dst, src, n);
+    return launched();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// batched persistent path
+// ---------------------------------------------------------------------------------------
+namespace {
+
+template <typename F>
+struct MatArgs {
+    Geo g;
+    cfd_params P;
+    F *u, *v, *T, *C;
+    double *p;
+    const F *vprev;
+};
+
+// Boundary cells of every active case; blockIdx.y is the case.
+template <typename F>
+__global__ void k_materialize_batch(const MatArgs<F> *mats, const int *ctl)
+{
+    const int c = blockIdx.y;
+    const int cs = ctl[c];
+    if (!(cs & 1)) return;
+    const MatArgs<F> m = mats[2 * c + ((cs >> 1) & 1)];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < materialize_count(m.g.imax, m.g.jmax)) materialize_cell(m.g, m.P, m.u, m.v, m.p, m.T, m.C, m.vprev, t);
+}
+
+__global__ void k_batch_zero(unsigned long long *res, const int *ctl, int n)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n && (ctl[c] & 1)) { res[3 * c] = 0; res[3 * c + 1] = 0; res[3 * c + 2] = 0; }
+}
+
+// Exit test of main.f90:139-150 per case: err = max(u, v, p); stop without the tau update
+// when itc != 1 and err < tol; otherwise swap the tau roles (parity bit) and go on.
+__global__ void k_batch_check(int *ctl, const unsigned long long *res, int *status, long long *itc, double *best,
+                              double *last, int n, double tol, double diverge, long long itc_max)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int cs = ctl[c];
+    if (!(cs & 1)) return;
+    const long long k = itc[c] + 1;
+    itc[c] = k;
+    double r[3];
+    for (int q = 0; q < 3; ++q) {
+        r[q] = __longlong_as_double((long long)res[3 * c + q]);
+        last[3 * c + q] = r[q];
+    }
+    const double err = fmax(fmax(r[0], r[1]), r[2]);
+    if (!(r[0] <= 1.0e308 && r[1] <= 1.0e308 && r[2] <= 1.0e308)) {       // NaN or Inf
+        status[c] = 2;
+        cs &= ~1;
+    } else if (k != 1 && err < tol) {
+        status[c] = 1;
+        cs &= ~1;
+    } else {
+        const double b = fmin(best[c], err);
+        best[c] = b;
+        cs ^= 2;
+        if (diverge > 0.0 && err > diverge * b) { status[c] = 3; cs &= ~1; }
+        else if (k >= itc_max) { status[c] = 4; cs &= ~1; }
+    }
+    ctl[c] = cs;
+}
+
+__global__ void k_batch_count(const int *ctl, int n, int *out)
+{
+    __shared__ int part[256];
+    int s = 0;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) s += ctl[c] & 1;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+template <typename R, bool STEADY, bool MUPOLY>
+void batch_launch_k1(const cfd_batch_launch *L, hipStream_t s)
+{
+    const dim3 g1(L->grid1), b1(NT1);
+    const K1Args<R> *a = (const K1Args<R> *)L->k1;
+    const BatchTile *t = (const BatchTile *)L->tiles1;
+    if (L->occ1 == 2) hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 2>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
+    else hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 1>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
+}
+
+template <typename R, bool STEADY>
+void batch_launch_k2(const cfd_batch_launch *L, hipStream_t s)
+{
+    const dim3 g2(L->grid2), b2(NT2);
+    const K2Args<R> *a = (const K2Args<R> *)L->k2;
+    const BatchTile *t = (const BatchTile *)L->tiles2;
+    if (L->occ2 == 4) hipLaunchKernelGGL((k_scal_batch<R, STEADY, 4>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
+    else hipLaunchKernelGGL((k_scal_batch<R, STEADY, 1>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
+}
+
+template <typename R>
+int batch_step(const cfd_batch_launch *L, hipStream_t s)
+{
+    const int nb = cdiv(L->ncases, 256);
+    hipLaunchKernelGGL(k_batch_zero, dim3(nb), dim3(256), 0, s, (unsigned long long *)L->res, L->ctl, L->ncases);
+    if (L->steady) {
+        if (L->mupoly) batch_launch_k1<R, true, true>(L, s);
+        else batch_launch_k1<R, true, false>(L, s);
+        batch_launch_k2<R, true>(L, s);
+    } else {
+        if (L->mupoly) batch_launch_k1<R, false, true>(L, s);
+        else batch_launch_k1<R, false, false>(L, s);
+        batch_launch_k2<R, false>(L, s);
+    }
+    hipLaunchKernelGGL((k_materialize_batch<R>), dim3(cdiv(L->max_mat, 256), L->ncases), dim3(256), 0, s,
+                       (const MatArgs<R> *)L->mat, L->ctl);
+    hipLaunchKernelGGL(k_batch_check, dim3(nb), dim3(256), 0, s, L->ctl, (const unsigned long long *)L->res,
+                       L->status, (long long *)L->itc, L->best, L->last, L->ncases, L->tol, L->diverge,
+                       (long long)L->itc_max);
+    return launched();
+}
+
+// Boundary cells of the tau-level iterates of every case. The argument set of the other
+// parity writes its n buffers, which are this parity's tau buffers. Channel BC set only:
+// the reference set's corner (imax, jmax+1) reads the pre-BC column of the previous step.
+template <typename F>
+__global__ void k_apply_bc_batch(const MatArgs<F> *mats, const int *ctl)
+{
+    const int c = blockIdx.y;
+    const int cs = ctl[c];
+    const MatArgs<F> m = mats[2 * c + (1 - ((cs >> 1) & 1))];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < materialize_count(m.g.imax, m.g.jmax)) materialize_cell(m.g, m.P, m.u, m.v, m.p, m.T, m.C, m.v, t);
+}
+
+template <typename R>
+void build_args(int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1, const int32_t *hb2, void *k1v,
+                void *k2v, void *matv)
+{
+    K1Args<R> *k1 = (K1Args<R> *)k1v;
+    K2Args<R> *k2 = (K2Args<R> *)k2v;
+    MatArgs<R> *mat = (MatArgs<R> *)matv;
+    for (int c = 0; c < ncases; ++c) {
+        const cfd_batch_case &b = cases[c];
+        const int im = b.P.imax, jm = b.P.jmax;
+        for (int q = 0; q < 2; ++q) {
+            cfd_fused_args a = b.a[q];
+            a.hb1 = hb1 ? hb1[c] : 0;
+            a.hb2 = hb2 ? hb2[c] : 0;
+            fill_k1<R>(k1[2 * c + q], im, jm, b.pack, &b.P, &a);
+            fill_k2<R>(k2[2 * c + q], im, jm, b.y, b.pack, &b.P, &a);
+            MatArgs<R> &m = mat[2 * c + q];
+            m.g = Geo{};
+            m.g.y = b.y; m.g.imax = im; m.g.jmax = jm;
+            m.P = b.P;
+            m.u = (R *)a.um_n; m.v = (R *)a.vm_n; m.p = a.pn; m.T = (R *)a.T_n; m.C = (R *)a.C_n;
+            m.vprev = (const R *)a.vm_tau;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfdk_batch_arg_sizes(int32_t prec, size_t *k1, size_t *k2, size_t *mat)
+{
+    if (prec == CFD_PREC_FP32) {
+        *k1 = sizeof(K1Args<float>); *k2 = sizeof(K2Args<float>); *mat = sizeof(MatArgs<float>);
+    } else {
+        *k1 = sizeof(K1Args<double>); *k2 = sizeof(K2Args<double>); *mat = sizeof(MatArgs<double>);
+    }
+    return CFD_OK;
+}
+
+int cfdk_batch_build_args(int32_t prec, int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1,
+                          const int32_t *hb2, void *k1, void *k2, void *mat)
+{
+    if (prec == CFD_PREC_FP32) build_args<float>(ncases, cases, hb1, hb2, k1, k2, mat);
+    else build_args<double>(ncases, cases, hb1, hb2, k1, k2, mat);
+    return CFD_OK;
+}
+
+int cfdk_batch_case_tiles(int kernel, int32_t imax, int32_t jmax, int32_t hb, int32_t c, int32_t *out)
+{
+    const int W = kernel == 1 ? W1 : W2;
+    const int nbj = cdiv(jmax - 2, W);
+    int h = 0;
+    const int nbi = pick_rows(imax - 2, nbj, kernel == 1 ? 24 : 16, hb, &h);
+    if (out) {
+        int k = 0;
+        for (int bi = 0; bi < nbi; ++bi)
+            for (int bj = 0; bj < nbj; ++bj, ++k) {
+                out[4 * k] = c; out[4 * k + 1] = bi; out[4 * k + 2] = bj; out[4 * k + 3] = 0;
+            }
+    }
+    return nbi * nbj;
+}
+
+int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2,
+                         int *k1_blocks_per_cu, int *k2_blocks_per_cu)
+{
+    int b1 = 0, b2 = 0;
+    hipError_t e1, e2;
+#define OCC1(R, S, M)                                                                                      \
+    (occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NT1, 0)       \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NT1, 0))
+#define OCC2(R, S)                                                                                         \
+    (occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NT2, 0)         \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NT2, 0))
+    if (prec == CFD_PREC_FP32) {
+        if (steady) { e1 = mupoly ? OCC1(float, true, true) : OCC1(float, true, false); e2 = OCC2(float, true); }
+        else { e1 = mupoly ? OCC1(float, false, true) : OCC1(float, false, false); e2 = OCC2(float, false); }
+    } else {
+        if (steady) { e1 = mupoly ? OCC1(double, true, true) : OCC1(double, true, false); e2 = OCC2(double, true); }
+        else { e1 = mupoly ? OCC1(double, false, true) : OCC1(double, false, false); e2 = OCC2(double, false); }
+    }
+#undef OCC1
+#undef OCC2
+    if (e1 != hipSuccess || e2 != hipSuccess) return CFD_ERR_HIP;
+    *k1_blocks_per_cu = b1 > 0 ? b1 : 1;
+    *k2_blocks_per_cu = b2 > 0 ? b2 : 1;
+    return CFD_OK;
+}
+
+int cfdk_batch_step(const cfd_batch_launch *L, hipStream_t s)
+{
+    if (L->prec == CFD_PREC_FP32) return batch_step<float>(L, s);
+    return batch_step<double>(L, s);
+}
+
+int cfdk_batch_count(const cfd_batch_launch *L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(256), 0, s, L->ctl, L->ncases, L->nactive);
+    return launched();
+}
+
+int cfdk_batch_apply_bc(const cfd_batch_launch *L, hipStream_t s)
+{
+    if (L->prec == CFD_PREC_FP32)
+        hipLaunchKernelGGL((k_apply_bc_batch<float>), dim3(cdiv(L->max_mat, 256), L->ncases), dim3(256), 0, s,
+                           (const MatArgs<float> *)L->mat, L->ctl);
+    else
+        hipLaunchKernelGGL((k_apply_bc_batch<double>), dim3(cdiv(L->max_mat, 256), L->ncases), dim3(256), 0, s,
+                           (const MatArgs<double> *)L->mat, L->ctl);
     return launched();
 }
 

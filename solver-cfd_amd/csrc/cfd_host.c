@@ -52,6 +52,16 @@ static int fail(int code, const char *fmt, ...)
         }                                                                              \
     } while (0)
 
+/* the last-error channel for the other host sources of the library */
+int cfd__fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 const char *cfd_last_error(void) { return g_err; }
 const char *cfd_version(void) { return "cfd_amd 0.1 (gfx950)"; }
 
@@ -134,8 +144,7 @@ int cfd_geometry_derive(int32_t imax, int32_t jmax, const double *x, const doubl
 /* ------------------------------------------------------------------------------------ */
 /* context                                                                               */
 enum { S_UM, S_VM, S_UMT, S_VMT, S_UMN, S_VMN, S_P, S_PN, S_T, S_TT, S_TN, S_C, S_CT, S_CN, S_NSTATE };
-enum { X_RU, X_RV, X_RZ, X_RC, X_UI, X_VI, X_ZI, X_CI, X_PI, X_RP, X_RESU, X_RESV, X_RESP,
-       X_U1, X_U2, X_V1, X_V2, X_Z1, X_Z2, X_C1, X_C2, X_NSCR };
+enum { X_RU, X_RV, X_RZ, X_RC, X_UI, X_VI, X_ZI, X_CI, X_PI, X_RP, X_RESU, X_RESV, X_RESP, X_NSCR };
 
 struct cfd_ctx {
     cfd_params P;
@@ -156,10 +165,19 @@ struct cfd_ctx {
     double last_ms_per_step;
     int64_t n_fused, n_exact;
     cfd_tiles tiles;       /* fused kernel shapes (zeros: built-in choice) */
+    /* FP32 storage (precision CFD_PREC_FP32): float copies of every state buffer except
+     * p and pn, and of eps and L. live32 = the float copies hold the current iterates
+     * (the fused steps run on them); the FP64 buffers are brought up to date before any
+     * other use (exact steps, downloads, level copies). */
+    float *s32[S_NSTATE];
+    float *eps32, *lp32;
+    int live32, geo32_stale;
     int tiles_set;         /* chosen by the caller or the autotuner */
 };
 
 static void tiles_from_cache(cfd_ctx *c);
+static int materialize(cfd_ctx *c);
+static int to32(cfd_ctx *c);
 
 static size_t kind_count(const cfd_ctx *c, char kind)
 {
@@ -205,9 +223,18 @@ static int ctx_alloc(cfd_ctx *c)
     CHK(dalloc(&c->fac.pack, (size_t)4 * (im + 2) + (size_t)4 * (jm + 2)));
     static const char kinds[S_NSTATE] = {'u', 'v', 'u', 'v', 'u', 'v', 'p', 'p', 'p', 'p', 'p', 'p', 'p', 'p'};
     for (int k = 0; k < S_NSTATE; ++k) CHK(dalloc(&c->st[k], kind_count(c, kinds[k])));
-    static const char skinds[X_NSCR] = {'u', 'v', 'p', 'p', 'u', 'v', 'p', 'p', 'p', 'p', 'u', 'v', 'p',
-                                        'u', 'u', 'v', 'v', 'p', 'p', 'p', 'p'};
+    static const char skinds[X_NSCR] = {'u', 'v', 'p', 'p', 'u', 'v', 'p', 'p', 'p', 'p', 'u', 'v', 'p'};
     for (int k = 0; k < X_NSCR; ++k) CHK(dalloc(&c->scr[k], kind_count(c, skinds[k])));
+    if (c->prec == CFD_PREC_FP32) {
+        for (int k = 0; k < S_NSTATE; ++k) {
+            if (k == S_P || k == S_PN) continue;
+            HIPCHK(hipMalloc((void **)&c->s32[k], kind_count(c, kinds[k]) * sizeof(float)));
+            HIPCHK(hipMemset(c->s32[k], 0, kind_count(c, kinds[k]) * sizeof(float)));
+        }
+        HIPCHK(hipMalloc((void **)&c->eps32, c->np * sizeof(float)));
+        HIPCHK(hipMalloc((void **)&c->lp32, c->np * sizeof(float)));
+        c->geo32_stale = 1;
+    }
     CHK(ensure_hist(c, 64));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -225,6 +252,9 @@ static void ctx_free(cfd_ctx *c)
     for (size_t k = 0; k < sizeof geo / sizeof geo[0]; ++k) if (geo[k]) hipFree(geo[k]);
     for (int k = 0; k < S_NSTATE; ++k) if (c->st[k]) hipFree(c->st[k]);
     for (int k = 0; k < X_NSCR; ++k) if (c->scr[k]) hipFree(c->scr[k]);
+    for (int k = 0; k < S_NSTATE; ++k) if (c->s32[k]) hipFree(c->s32[k]);
+    if (c->eps32) hipFree(c->eps32);
+    if (c->lp32) hipFree(c->lp32);
     if (c->d_hist) hipFree(c->d_hist);
     if (c->h_hist) hipHostFree(c->h_hist);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -369,9 +399,11 @@ int cfd_ctx_upload(cfd_ctx *c, int32_t field, const double *host, int32_t layout
     if (!c) return fail(CFD_ERR_STATE, "no context");
     if (!host) return fail(CFD_ERR_ARG, "host pointer is NULL");
     if (field == CFD_F_PN) return fail(CFD_ERR_ARG, "CFD_F_PN is download-only");
+    CHK(materialize(c));
     char kind;
     double *d = field_ptr(c, field, &kind);
     if (!d) return fail(CFD_ERR_ARG, "unknown field %d", field);
+    if (field == CFD_F_EPS || field == CFD_F_LP) c->geo32_stale = 1;
     int n, m, ld;
     kind_dims(c, kind, &n, &m, &ld);
     if (layout == CFD_LAYOUT_C) {
@@ -393,7 +425,6 @@ int cfd_ctx_upload(cfd_ctx *c, int32_t field, const double *host, int32_t layout
     return CFD_OK;
 }
 
-static int materialize(cfd_ctx *c);
 
 int cfd_ctx_download(cfd_ctx *c, int32_t field, double *host, int32_t layout)
 {
@@ -437,6 +468,7 @@ int cfd_ctx_device_ptr(cfd_ctx *c, int32_t field, void **out)
 int cfd_ctx_sync_levels(cfd_ctx *c)
 {
     if (!c) return fail(CFD_ERR_STATE, "no context");
+    CHK(materialize(c));
     CHK(cfdk_copy(c->st[S_UMT], c->st[S_UM], (long)c->nu, c->s));
     CHK(cfdk_copy(c->st[S_VMT], c->st[S_VM], (long)c->nv, c->s));
     CHK(cfdk_copy(c->st[S_TT], c->st[S_T], (long)c->np, c->s));
@@ -538,8 +570,14 @@ static int x_solve_s(const xenv *e, int species, const double *um_n, const doubl
 
 static void swap_ptr(double **a, double **b) { double *t = *a; *a = *b; *b = t; }
 
+static void swap_f(float **a, float **b) { float *t = *a; *a = *b; *b = t; }
+
 static void tau_swap(cfd_ctx *c)
 {
+    swap_f(&c->s32[S_UMT], &c->s32[S_UMN]);
+    swap_f(&c->s32[S_VMT], &c->s32[S_VMN]);
+    swap_f(&c->s32[S_TT], &c->s32[S_TN]);
+    swap_f(&c->s32[S_CT], &c->s32[S_CN]);
     swap_ptr(&c->st[S_UMT], &c->st[S_UMN]);
     swap_ptr(&c->st[S_VMT], &c->st[S_VMN]);
     swap_ptr(&c->st[S_P], &c->st[S_PN]);
@@ -591,19 +629,27 @@ static int step_exact(cfd_ctx *c, uint64_t *slot)
 static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
 {
     const int st = c->P.steady;
-    double **S = c->st, **X = c->scr;
+    double **S = c->st;
     cfd_fused_args a;
     memset(&a, 0, sizeof a);
-    a.um = st ? S[S_UMT] : S[S_UM];
-    a.vm = st ? S[S_VMT] : S[S_VM];
-    a.T = st ? S[S_TT] : S[S_T];
-    a.C = st ? S[S_CT] : S[S_C];
-    a.um_tau = S[S_UMT]; a.vm_tau = S[S_VMT];
+    void *F[S_NSTATE];
+    if (c->prec == CFD_PREC_FP32) {
+        CHK(to32(c));
+        for (int k = 0; k < S_NSTATE; ++k) F[k] = c->s32[k];
+        a.epsr = c->eps32; a.lpr = c->lp32;
+    } else {
+        for (int k = 0; k < S_NSTATE; ++k) F[k] = S[k];
+        a.epsr = c->g.eps; a.lpr = c->g.lp;
+    }
+    a.prec = c->prec;
+    a.um = st ? F[S_UMT] : F[S_UM];
+    a.vm = st ? F[S_VMT] : F[S_VM];
+    a.T = st ? F[S_TT] : F[S_T];
+    a.C = st ? F[S_CT] : F[S_C];
+    a.um_tau = F[S_UMT]; a.vm_tau = F[S_VMT];
     a.p = c->uv_reads_pn ? S[S_PN] : S[S_P];
-    a.T_tau = S[S_TT]; a.C_tau = S[S_CT];
-    a.um_n = S[S_UMN]; a.vm_n = S[S_VMN]; a.pn = S[S_PN]; a.T_n = S[S_TN]; a.C_n = S[S_CN];
-    a.u1 = X[X_U1]; a.u2 = X[X_U2]; a.v1 = X[X_V1]; a.v2 = X[X_V2];
-    a.z1 = X[X_Z1]; a.z2 = X[X_Z2]; a.c1 = X[X_C1]; a.c2 = X[X_C2];
+    a.T_tau = F[S_TT]; a.C_tau = F[S_CT];
+    a.um_n = F[S_UMN]; a.vm_n = F[S_VMN]; a.pn = S[S_PN]; a.T_n = F[S_TN]; a.C_n = F[S_CN];
     a.res_max = slot;
     a.hb1 = c->tiles.rows_uvp; a.hb2 = c->tiles.rows_tc;
     a.occ1 = c->tiles.waves_uvp; a.occ2 = c->tiles.waves_tc;
@@ -618,19 +664,40 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
 
 static int step_fused(cfd_ctx *c, uint64_t *slot) { return step_fused_ev(c, slot, 0); }
 
+/* Float copies of the iterates the fused FP32 step reads (the physical level, the tau
+ * level), and of the porosity fields when they changed. */
+static int to32(cfd_ctx *c)
+{
+    static const int idx[] = {S_UM, S_VM, S_UMT, S_VMT, S_T, S_TT, S_C, S_CT};
+    if (!c->live32) {
+        for (size_t q = 0; q < sizeof idx / sizeof idx[0]; ++q) {
+            const int k = idx[q];
+            const size_t n = (k == S_UM || k == S_UMT) ? c->nu : ((k == S_VM || k == S_VMT) ? c->nv : c->np);
+            CHK(cfdk_d2f(c->s32[k], c->st[k], (long)n, c->s));
+        }
+        c->live32 = 1;
+    }
+    if (c->geo32_stale) {
+        CHK(cfdk_d2f(c->eps32, c->g.eps, (long)c->np, c->s));
+        CHK(cfdk_d2f(c->lp32, c->g.lp, (long)c->np, c->s));
+        c->geo32_stale = 0;
+    }
+    return CFD_OK;
+}
+
+/* Brings the FP64 buffers up to date (they are the only copy in FP64 contexts): after
+ * fused FP32 steps the float buffers hold the state. */
 static int materialize(cfd_ctx *c)
 {
-    if (c->bc_clean != 2) return CFD_OK;
-    /* The last fused step's outputs lack their boundary cells: in the *_tau buffers after
-     * the swap, in the *_n buffers after an exit. The step's input v iterate (the other
-     * buffer) supplies the pre-BC cells the reference corner (imax, jmax+1) reads. */
-    double **S = c->st;
-    if (c->pending_in_tau)
-        CHK(cfdk_materialize_bc(&c->g, &c->P, S[S_UMT], S[S_VMT], S[S_P], S[S_TT], S[S_CT], S[S_VMN], c->s));
-    else
-        CHK(cfdk_materialize_bc(&c->g, &c->P, S[S_UMN], S[S_VMN], S[S_PN], S[S_TN], S[S_CN], S[S_VMT], c->s));
+    if (!c->live32) return CFD_OK;
+    for (int k = 0; k < S_NSTATE; ++k) {
+        if (k == S_P || k == S_PN) continue;
+        const size_t n = (k == S_UM || k == S_UMT || k == S_UMN) ? c->nu
+                       : ((k == S_VM || k == S_VMT || k == S_VMN) ? c->nv : c->np);
+        CHK(cfdk_f2d(c->st[k], c->s32[k], (long)n, c->s));
+    }
     HIPCHK(hipStreamSynchronize(c->s));
-    c->bc_clean = 1;
+    c->live32 = 0;
     return CFD_OK;
 }
 
@@ -639,7 +706,7 @@ static int materialize(cfd_ctx *c)
  * that already satisfy the boundary conditions; anything else takes the exact path. */
 static int fused_ok(const cfd_ctx *c)
 {
-    return c->kernel == 1 && c->have_fused && c->bc_clean && !c->uv_reads_pn && c->prec == CFD_PREC_FP64
+    return c->kernel == 1 && c->have_fused && c->bc_clean && !c->uv_reads_pn
         && c->P.b_art == 0.0 && c->P.compat == CFD_COMPAT_FORTRAN;
 }
 
@@ -809,6 +876,8 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
         CHK(one_step(c, c->d_hist));
         tau_swap(c);
     }
+    /* precision conversions happen outside the graph: it replays fused steps only */
+    if (fused_ok(c) && c->prec == CFD_PREC_FP32) CHK(to32(c));
     CHK(ensure_hist(c, 1));
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
@@ -944,8 +1013,7 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)
 {
     CHK(check_ready(c));
     if (steps < 1) steps = 1;
-    if (!(c->kernel == 1 && c->have_fused && c->prec == CFD_PREC_FP64 && c->P.b_art == 0.0
-          && c->P.compat == CFD_COMPAT_FORTRAN))
+    if (!(c->kernel == 1 && c->have_fused && c->P.b_art == 0.0 && c->P.compat == CFD_COMPAT_FORTRAN))
         return fail(CFD_ERR_UNSUPPORTED, "autotune: this context does not run the fused kernels");
     CHK(materialize(c));
     /* snapshot: every state buffer's contents, the buffer roles and the step flags */
@@ -955,7 +1023,9 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)
     double *snap = NULL;
     HIPCHK(hipMalloc((void **)&snap, total * sizeof(double)));
     double *roles[S_NSTATE];
+    float *roles32[S_NSTATE];
     memcpy(roles, c->st, sizeof roles);
+    memcpy(roles32, c->s32, sizeof roles32);
     const int f_bc = c->bc_clean, f_uv = c->uv_reads_pn, f_first = c->first_step, f_pend = c->pending_in_tau;
     const int64_t n_f = c->n_fused, n_e = c->n_exact;
     const double last_ms = c->last_ms_per_step;
@@ -987,8 +1057,11 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)
         r = time_tiles(c, &cand, steps, &t);
         if (r == CFD_OK && t < best) { best = t; best_t = cand; }
     }
-    /* restore the state exactly as it was */
+    /* restore the state exactly as it was (the FP64 buffers hold it; float copies are
+     * rebuilt by the next fused step) */
     memcpy(c->st, roles, sizeof roles);
+    memcpy(c->s32, roles32, sizeof roles32);
+    c->live32 = 0;
     for (int k = 0; k < S_NSTATE; ++k) {
         const int rr = cfdk_copy(c->st[k], snap + off[k], (long)kind_count(c, kinds[k]), c->s);
         if (r == CFD_OK) r = rr;

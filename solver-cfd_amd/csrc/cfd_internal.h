@@ -55,14 +55,16 @@ int cfdk_to_fortran(double *dst_f, const double *src_c, int n, int m, int ld, hi
 
 /* ---- fused pseudo-step ------------------------------------------------------------- */
 typedef struct cfd_fused_args {
-    /* inputs (C layout, FP64) */
-    const double *um, *vm, *um_tau, *vm_tau, *p, *T, *T_tau, *C, *C_tau;
-    /* outputs */
-    double *um_n, *vm_n, *pn, *T_n, *C_n;
-    /* scratch for stage values (u/v layouts), per-level path only */
-    double *u1, *u2, *v1, *v2, *z1, *z2, *c1, *c2;
+    /* fields in the context precision (double, or float when prec == CFD_PREC_FP32);
+     * pressure is double in both */
+    const void *um, *vm, *um_tau, *vm_tau, *T, *T_tau, *C, *C_tau;
+    const double *p;
+    void *um_n, *vm_n, *T_n, *C_n;
+    double *pn;
+    const void *epsr, *lpr;        /* porosity fields in the field precision */
     /* residual accumulators: res_max[0..2] (uint64 bit patterns of max|r3| for u, v, p) */
     uint64_t *res_max;
+    int32_t prec;
     /* row-block heights of K1 / K2 (0: built-in heuristic) and the register-budget variants
      * (occ1 in {1, 2}, occ2 in {1, 4} waves per SIMD); chosen by the autotuner */
     int32_t hb1, hb2, occ1, occ2;
@@ -92,6 +94,54 @@ int cfdk_fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_para
  * imax the reference's corner (imax, jmax+1) depends on. */
 int cfdk_materialize_bc(const cfd_devgeo *g, const cfd_params *P, double *um_n, double *vm_n,
                         double *pn, double *T_n, double *C_n, const double *vm_prev, hipStream_t s);
+/* ---- batched persistent path --------------------------------------------------------- */
+/* One case of a batch: its parameters, 1-D device geometry and the fused-step arguments for
+ * both tau parities (parity 1 has the tau and n buffers of parity 0 swapped). */
+typedef struct cfd_batch_case {
+    cfd_params P;
+    const double *y;        /* device y[0..jmax] */
+    const double *pack;     /* device packed 1-D factors (cfd_devfactors.pack layout) */
+    cfd_fused_args a[2];
+} cfd_batch_case;
+
+/* Host-side sizes of one entry of the device argument tables (per precision). */
+int cfdk_batch_arg_sizes(int32_t prec, size_t *k1, size_t *k2, size_t *mat);
+/* Fill the three host tables (2 entries per case, parity-major within a case) and the tile
+ * geometry: rows per tile hb1/hb2 per case (0: built-in rule). */
+int cfdk_batch_build_args(int32_t prec, int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1,
+                          const int32_t *hb2, void *k1, void *k2, void *mat);
+/* Tiles of one case: count and fill (c, bi, bj) triples into out (4 int32 per tile). */
+int cfdk_batch_case_tiles(int kernel, int32_t imax, int32_t jmax, int32_t hb, int32_t c, int32_t *out);
+
+typedef struct cfd_batch_launch {
+    int32_t prec, steady, mupoly, occ1, occ2, ncases, max_mat;
+    const void *k1, *k2, *mat;       /* device argument tables */
+    const void *tiles1, *tiles2;     /* device tile lists (4 int32 per tile) */
+    int32_t ntiles1, ntiles2, grid1, grid2;
+    int32_t *ctl;                    /* per case: bit 0 active, bit 1 tau parity */
+    uint64_t *res;                   /* 3 per case */
+    /* convergence control (device) */
+    double tol, diverge;
+    int64_t itc_max;
+    int32_t *status;                 /* 0 running, 1 converged, 2 non-finite, 3 diverged, 4 itc_max */
+    int64_t *itc;                    /* pseudo-steps taken */
+    double *best, *last;             /* best max residual; last residuals (3 per case) */
+    int32_t *nactive;                /* written by cfdk_batch_count */
+} cfd_batch_launch;
+
+/* Blocks per CU the batch kernels can keep resident (for the persistent grid size). */
+int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2,
+                         int *k1_blocks_per_cu, int *k2_blocks_per_cu);
+/* One batched pseudo-step: residual slots, K1, K2, boundary cells, convergence control. */
+int cfdk_batch_step(const cfd_batch_launch *L, hipStream_t s);
+/* Active-case count into *L->nactive. */
+int cfdk_batch_count(const cfd_batch_launch *L, hipStream_t s);
+/* Boundary cells of every case's tau-level iterates (makes an uploaded state BC-consistent). */
+int cfdk_batch_apply_bc(const cfd_batch_launch *L, hipStream_t s);
+
+/* Precision conversion of n contiguous elements. */
+int cfdk_d2f(float *dst, const double *src, long n, hipStream_t s);
+int cfdk_f2d(double *dst, const float *src, long n, hipStream_t s);
 
 #ifdef __cplusplus
 }
