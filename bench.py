@@ -67,6 +67,96 @@ def cpu_baseline(cfg, seconds):
                        % (n, cfg["params"].imax, cfg["params"].jmax, el))
 
 
+def _build_batch(cs, prec):
+    b = L.Batch([c["params"] for c in cs], precision=prec)
+    for i, c in enumerate(cs):
+        b.set_mesh(i, c["x"], c["y"])
+        b.upload(i, "eps", c["eps"])
+        b.upload(i, "lp", c["lp"])
+        b.set_state(i, c["state"])
+    b.apply_bc()
+    return b
+
+
+def bench_batch(args):
+    """Time to residual `tol` for every case of the seeded mixed batch (grids {64..512} x
+    {64..256}, Re, Da, porosity, Ri drawn per case; steady channel flow from zero fields).
+
+    plain FP64 path: one context per case, cfd_run_to_tol, autotuned tiles. Its arithmetic is bit-identical to
+    the FP64 batch (tested), so its per-case step counts are the FP64 batch's; its time is
+    sum over cases of steps x the context's measured time per step for that grid class.
+    batched: FP32 batch (persistent launches, graph replay) until the FP32 storage floor,
+    then the FP64 batch from the converted state down to `tol`, both timed with events."""
+    from cfd_amd.params import PREC_FP32 as P32, PREC_FP64 as P64
+    cs = C.case_mixed_batch(n=args.batch_cases, beta=args.batch_beta)
+    cells = np.array([c["params"].imax * c["params"].jmax for c in cs], dtype=np.int64)
+    log("batch: %d cases, %d cells" % (len(cs), int(cells.sum())))
+    # FP64 batch: the plain path's step counts
+    b64 = _build_batch(cs, P64)
+    tiles64, _ = b64.autotune(steps=20)
+    code, it64, res64, st64 = b64.run(args.batch_tol, args.batch_itc, steps_per_graph=500, diverge_factor=0.0)
+    ms64, steps64, cu64 = b64.last_timing()
+    conv = st64 == 1
+    for i in np.nonzero((st64 == 2) | (st64 == 3))[0]:
+        c = cs[i]
+        log("  case %s failed (status %d at step %d): Re %.1f Da %.2e Ri %.2f dt %.3e"
+            % (c["name"], st64[i], it64[i], c["re"], c["da"], c["ri"], c["params"].dt))
+    log("FP64 batch: %d/%d converged to %.0e, %.1f s, %d graph steps, tiles %s"
+        % (int(conv.sum()), len(cs), args.batch_tol, ms64 * 1e-3, steps64, tiles64))
+    b64.close()
+    # plain path: measured time per step of one context per grid class
+    t_class = {}
+    for c in cs:
+        k = (c["params"].imax, c["params"].jmax)
+        if k in t_class:
+            continue
+        ctx = L.Context(c["params"], c["x"], c["y"], c["eps"], c["lp"])
+        ctx.set_state(c["state"])
+        ctx.autotune(steps=10)          # the plain path at its best tile shape for this grid
+        ctx.run_to_tol(0.0, 50, check_every=50, raise_on_fail=False)
+        ctx.run_to_tol(0.0, 400, check_every=100, raise_on_fail=False)
+        t_class[k] = ctx.last_ms_per_step()
+        ctx.close()
+    # every case: converged ones stop at their step, the others run to itc_max on both paths
+    t_plain = sum(int(it64[i]) * t_class[(c["params"].imax, c["params"].jmax)] for i, c in enumerate(cs)) * 1e-3
+    log("plain FP64 path: %.1f s; ms/step per class %s"
+        % (t_plain, {"%dx%d" % k: round(v, 4) for k, v in sorted(t_class.items())}))
+    # batched FP32 -> FP64
+    b32 = _build_batch(cs, P32)
+    tiles32, _ = b32.autotune(steps=20)
+    tol32 = max(args.batch_tol, 5e-7)
+    code, it32, res32, st32 = b32.run(tol32, args.batch_itc, steps_per_graph=500)
+    ms32, steps32, cu32 = b32.last_timing()
+    bfin = _build_batch(cs, P64)
+    bfin.set_tiles(*tiles64)
+    bfin.copy_state_from(b32)
+    code, itf, resf, stf = bfin.run(args.batch_tol, args.batch_itc, steps_per_graph=500)
+    msf, stepsf, cuf = bfin.last_timing()
+    convb = stf == 1
+    both = conv & convb
+    t_batch = (ms32 + msf) * 1e-3
+    fields_err = []
+    log("batched FP32 %.1f s (%d steps, tiles %s) + FP64 finish %.1f s (%d steps): %d/%d converged"
+        % (ms32 * 1e-3, steps32, tiles32, msf * 1e-3, stepsf, int(convb.sum()), len(cs)))
+    b32.close(); bfin.close()
+    out = {
+        "metric": "time-to-residual speedup, batched FP32+FP64 vs plain FP64 path", "value": t_plain / t_batch,
+        "unit": "x", "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded mixed batch: %d cases)" % len(cs),
+        "config": {"workload": "mixed batch of steady channel cases, grids {64..512}x{64..256}",
+                   "cases": len(cs), "cells": int(cells.sum()), "tol": args.batch_tol, "itc_max": args.batch_itc,
+                   "beta": args.batch_beta},
+        "plain_fp64_s": t_plain, "batched_s": t_batch, "batched_fp32_s": ms32 * 1e-3, "batched_fp64_finish_s": msf * 1e-3,
+        "fp64_batch_s": ms64 * 1e-3,
+        "converged_fp64": int(conv.sum()), "converged_batched": int(convb.sum()), "converged_both": int(both.sum()),
+        "steps_fp64_median": float(np.median(it64[conv])) if conv.any() else None,
+        "steps_fp32_median": float(np.median(it32)), "steps_finish_median": float(np.median(itf)),
+        "fp64_batch_cell_updates_per_s": cu64 / (ms64 * 1e-3), "fp32_batch_cell_updates_per_s": cu32 / (ms32 * 1e-3),
+        "plain_ms_per_step": {"%dx%d" % k: v for k, v in sorted(t_class.items())},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +178,16 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
+    ap.add_argument("--workload", default="grid", choices=["grid", "batch"],
+                    help="grid: one large grid (the cell-update rate); batch: time to residual of the "
+                         "256-case mixed batch against the plain FP64 path")
+    ap.add_argument("--batch-tol", type=float, default=1e-8)
+    ap.add_argument("--batch-itc", type=int, default=50000)
+    ap.add_argument("--batch-cases", type=int, default=256)
+    ap.add_argument("--batch-beta", type=float, default=200.0)
     args = ap.parse_args()
+    if args.workload == "batch":
+        return bench_batch(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the pseudo-time "

@@ -271,6 +271,10 @@ int  cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *ms_
  * states. */
 int  cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
                    int64_t *iters, double *res, int32_t *status);
+/* Copies every case's current iterates and porosity from src into dst (same cases, same
+ * grid sizes), converting precision on the device: e.g. an FP32 batch that has reached the
+ * residual its storage can resolve hands over to an FP64 batch for the last decades. */
+int  cfd_batch_copy_state(cfd_batch *dst, cfd_batch *src);
 /* Wall time of the last run (HIP events on the batch stream), graph steps replayed, and
  * cell-updates done (sum over cases of pseudo-steps x imax x jmax). */
 int  cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates);

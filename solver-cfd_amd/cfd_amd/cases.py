@@ -219,3 +219,56 @@ def case_full_physics(imax=2048, jmax=2048):
     eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
     st = perturbed_state(imax, jmax, seed=3, amp=1e-3, T0=0.5)
     return dict(name="full_physics_tanh_%dx%d" % (imax, jmax), params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
+
+
+# ---- mixed batch of steady cases -----------------------------------------------------------
+def steady_dt(x, y, re, pe, da, cf, eps_min, umax=1.5, dtau=0.05, safety=0.4):
+    """Pseudo-time scale dt of a steady (steady=1) case: the update is dtau * dt * R(phi), so
+    dtau * dt * |lambda_max| must stay inside SSP-RK3's stability interval (~2.5).
+    |lambda_max| bounds diffusion 4 max(nu, kappa) (1/dx^2 + 1/dy^2), convection
+    2 |u| (1/dx + 1/dy) (both divided by the smallest porosity: inside the porous block the
+    transport terms are scaled by 1/eps) and the Darcy-Forchheimer drag eps/(Re Da) +
+    Cf |u| / sqrt(Da);
+    convective eigenvalues lie near the imaginary axis, where the interval is ~1.7, hence
+    the default safety 0.4 (dtau dt |lambda_max| <= 1)."""
+    dx = float(np.min(np.diff(x[1:]))); dy = float(np.min(np.diff(y[1:])))
+    d = max(1.0 / re, 1.0 / pe)
+    lam = 4.0 * d * (1.0 / dx ** 2 + 1.0 / dy ** 2) + 2.0 * umax * (1.0 / dx + 1.0 / dy)
+    lam /= eps_min          # the scalar equations divide by L (eps - 1) + 1 = eps in the porous block
+    lam += 1.0 / (re * da) + cf * umax / math.sqrt(da)
+    return safety * 2.5 / (dtau * lam)
+
+
+def case_mixed_batch(n=256, seed=4, sizes_i=(64, 128, 256, 512), sizes_j=(64, 128, 256), beta=1.0,
+                     safety=0.4):
+    """Independent steady cases for the batched solver, all seeded: grid imax from sizes_i,
+    jmax from sizes_j (uniform choice), Re log-uniform [10, 500], Da log-uniform [1e-4, 1e-1],
+    porosity per cell uniform [0.3, 0.95] inside one random porous rectangle covering 10-40 %
+    of the unit square, Boussinesq buoyancy with Ri uniform [0, 5] (hot bottom, cold top),
+    species slot Z = 1 of width 0.1 on the inflow wall, Sc = 0.7, channel BC set, zero
+    initial fields. Returns a list of dicts (name, params, x, y, eps, lp, state)."""
+    r = uniform(seed, 12 * n).reshape(n, 12)
+    out = []
+    for c in range(n):
+        u = r[c]
+        im = int(sizes_i[min(int(u[0] * len(sizes_i)), len(sizes_i) - 1)])
+        jm = int(sizes_j[min(int(u[1] * len(sizes_j)), len(sizes_j) - 1)])
+        re = 10.0 * (50.0 ** u[2])
+        da = 1e-4 * (1e3 ** u[3])
+        ri = 5.0 * u[4]
+        area = 0.1 + 0.3 * u[5]
+        aspect = 0.5 + 1.5 * u[6]                      # width / height of the rectangle
+        w = min(math.sqrt(area * aspect), 0.9); h = min(area / w, 0.9)
+        x0 = 0.05 + (0.9 - w) * u[7]; y0 = 0.05 + (0.9 - h) * u[8]
+        x = uniform_nodes(im); y = uniform_nodes(jm)
+        eps, lp = porous_block(x, y, x0, x0 + w, y0, y0 + h, 0.3, 0.95, seed=seed * 100003 + c)
+        pe = 0.7 * re
+        dt = steady_dt(x, y, re, pe, da, 0.55, 0.3, safety=safety)
+        prm = make_params(im, jm, steady=1, bc=BC_CHANNEL, re=re, pe=pe, sc=0.7, beta=beta, darcy=da, cf=0.55,
+                          g=0.0, invfr2=0.0, buoyancy=BUOY_BOUSSINESQ, ri=ri, t_ref=0.5, u_in=1.0,
+                          t_bottom=1.0, t_top=0.0, slot_lo=0.45, slot_hi=0.55, slot_value=1.0,
+                          dt=dt, dtau=0.05)
+        st = perturbed_state(im, jm, seed=seed * 7919 + c, amp=0.0, u0=0.0, v0=0.0, p0=0.0, T0=0.5, C0=0.0)
+        out.append(dict(name="mixed%03d_%dx%d" % (c, im, jm), params=prm, x=x, y=y, eps=eps, lp=lp, state=st,
+                        re=re, da=da, ri=ri))
+    return out

@@ -27,7 +27,7 @@ EXPORTS_CFD = [
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
-    "cfd_batch_last_timing", "cfd_last_error", "cfd_version",
+    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
     "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
@@ -102,6 +102,7 @@ def load():
         "cfd_batch_autotune": (ctypes.c_int, [_vp, i32, ctypes.POINTER(Tiles), _dp]),
         "cfd_batch_run": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_batch_last_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "cfd_batch_copy_state": (ctypes.c_int, [_vp, _vp]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -353,6 +354,11 @@ class Batch:
         if raise_on_fail:
             check(code)
         return code, np.array(it[:], dtype=np.int64), res, np.array(st[:], dtype=np.int32)
+
+    def copy_state_from(self, other):
+        """Take every case's current iterates from another batch of the same cases
+        (precision converted on the device)."""
+        check(load().cfd_batch_copy_state(self.h, other.h))
 
     def last_timing(self):
         ms = ctypes.c_double(0); steps = ctypes.c_int64(0); cells = ctypes.c_int64(0)

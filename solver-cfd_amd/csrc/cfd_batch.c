@@ -603,6 +603,7 @@ int cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *out_
             b->args_dirty = 1;
             /* tol 0: nothing converges; one untimed graph replay, then a timed one */
             r = cfd_batch_run(b, 0.0, 2 * steps, steps, 0.0, NULL, NULL, NULL);
+            if (r == CFD_ERR_NONFINITE) r = CFD_OK;   /* a failing case does not change the timing */
             if (r == CFD_OK) {
                 const double ms = b->last_ms / (double)b->last_steps;
                 if (ms < best) { best = ms; best_t = t; }
@@ -621,5 +622,41 @@ int cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *out_
     b->args_dirty = 1;
     if (out) *out = best_t;
     if (out_ms) *out_ms = best;
+    return CFD_OK;
+}
+
+/* Copies every case's current iterates (and eps, L) from src into dst, converting between
+ * precisions on the device; the cases must have the same grid sizes. dst continues from
+ * them like from an upload (its next run starts a fresh exit-test count). */
+int cfd_batch_copy_state(cfd_batch *dst, cfd_batch *src)
+{
+    if (!dst || !src) return cfd__fail(CFD_ERR_STATE, "no batch");
+    if (dst->n != src->n || dst->steady != src->steady) return cfd__fail(CFD_ERR_ARG, "batches differ in cases or mode");
+    for (int32_t c = 0; c < dst->n; ++c)
+        if (dst->cs[c].P.imax != src->cs[c].P.imax || dst->cs[c].P.jmax != src->cs[c].P.jmax)
+            return cfd__fail(CFD_ERR_ARG, "case %d: grid sizes differ", c);
+    static const int32_t fields[] = {CFD_F_UM_TAU, CFD_F_VM_TAU, CFD_F_P, CFD_F_T_TAU, CFD_F_C_TAU, CFD_F_UM, CFD_F_VM,
+                                     CFD_F_T, CFD_F_C, CFD_F_EPS, CFD_F_LP};
+    const size_t nf = dst->steady ? 5 : 9;
+    BHIP(hipStreamSynchronize(src->s));
+    for (int32_t c = 0; c < dst->n; ++c) {
+        for (size_t q = 0; q < nf + 2; ++q) {
+            const int32_t f = q < nf ? fields[q] : fields[9 + (q - nf)];
+            int ks = 0, kd = 0;
+            BCHK(field_buffer(src, c, f, &ks));
+            BCHK(field_buffer(dst, c, f, &kd));
+            const bcase *a = &src->cs[c];
+            bcase *b = &dst->cs[c];
+            const size_t n = field_count(a, ks), es = field_esz(src, ks), ed = field_esz(dst, kd);
+            if (es == ed) {
+                BHIP(hipMemcpyAsync(b->buf[kd], a->buf[ks], n * es, hipMemcpyDeviceToDevice, dst->s));
+            } else if (es == 4) {
+                BCHK(cfdk_f2d((double *)b->buf[kd], (const float *)a->buf[ks], (long)n, dst->s));
+            } else {
+                BCHK(cfdk_d2f((float *)b->buf[kd], (const double *)a->buf[ks], (long)n, dst->s));
+            }
+        }
+    }
+    BHIP(hipStreamSynchronize(dst->s));
     return CFD_OK;
 }
