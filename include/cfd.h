@@ -233,6 +233,42 @@ int  cfd_ctx_step_counts(const cfd_ctx *ctx, int64_t *fused, int64_t *exact);
  * The host waits after every step, so use it for measurement, not production runs. */
 int  cfd_pseudo_steps_timed(cfd_ctx *ctx, int32_t nsteps, double res[3], double ms[4]);
 
+/* ---- mesh, porous flags, nodal fields and state files (either side of the iteration) -- */
+/* The reference mesh of nonsymetric_mesh.f90:7-44: x over [-Lhori/2, Lhori/2] and y from
+ * -y_down, both with spacing dx_c = Lhori/(imax-1), then tanh-stretched about the centre
+ * (x, parameters Px, Q) and about the row int(y_down/dx_c)+1 (y, parameters Py, Q), in the
+ * reference's update order. x[1..imax], y[1..jmax] (index 0 unused). */
+int cfd_mesh_reference(int32_t imax, int32_t jmax, double lhori, double y_down, double px, double py, double q,
+                       double *x, double *y);
+/* Porous-block flags of nonsymetric_mesh.f90:165-218 for the block x <= x_edge, y >= y_edge:
+ * interior, boundary and boundary-side cells are porous (eps = porosity, L = 1), the rest
+ * fluid (eps = 1, L = 0). eps, lp (and flags, may be NULL: 0 fluid, 1 boundary, 2 interior,
+ * 3 boundary side) are p-layout host arrays. */
+int cfd_porous_flags_reference(int32_t imax, int32_t jmax, const double *x, const double *y, double x_edge,
+                               double y_edge, double porosity, double *eps, double *lp, int32_t *flags);
+/* comp_mean.f90:1-25: nodal u, v (p layout) from the staggered um, vm (C layouts), host. */
+int cfd_nodal_velocity(int32_t imax, int32_t jmax, const double *um, const double *vm, double *u, double *v);
+
+/* The final-state table of output.f90:12-18 (one record per node, i outer, j inner:
+ * X Y u v p T C; u, v nodal, all p-layout host arrays). The reader accepts list-directed
+ * files whose records wrap over lines (as the reference's compilers write them); any
+ * output pointer may be NULL. */
+int cfd_write_state_table(const char *path, int32_t imax, int32_t jmax, const double *x, const double *y,
+                          const double *u, const double *v, const double *p, const double *T, const double *C);
+int cfd_read_state_table(const char *path, int32_t imax, int32_t jmax, double *x, double *y, double *u, double *v,
+                         double *p, double *T, double *C);
+
+/* Restart files restartU.dat, restartV.dat, restartPTC.dat in directory `dir`, written as
+ * output.f90:20-43 does (i outer). initial.f90:52-87 reads them back with j outer:
+ * CFD_RESTART_ORDER_REFERENCE reproduces that read, CFD_RESTART_ORDER_WRITTEN reads in the
+ * order of the writes (exact round trip). */
+#define CFD_RESTART_ORDER_REFERENCE 0
+#define CFD_RESTART_ORDER_WRITTEN   1
+int cfd_write_restart(const char *dir, int32_t imax, int32_t jmax, const double *um, const double *vm,
+                      const double *p, const double *T, const double *C);
+int cfd_read_restart(const char *dir, int32_t imax, int32_t jmax, int32_t order, double *um, double *vm, double *p,
+                     double *T, double *C);
+
 /* ---- batched solves ------------------------------------------------------------------
  * Many independent cases (each its own grid size, parameters, mesh and fields) advance
  * together: one persistent launch per kernel walks the tiles of every case, the per-case

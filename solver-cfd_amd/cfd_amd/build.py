@@ -34,7 +34,7 @@ HIP_SOURCES = [
     ("cfd_ops.hip", ["-ffp-contract=off"]),
     ("cfd_fused.hip", ["-ffp-contract=fast"]),
 ]
-C_SOURCES = ["cfd_host.c", "cfd_batch.c"]
+C_SOURCES = ["cfd_host.c", "cfd_batch.c", "cfd_mesh_io.c"]
 
 
 def _run(cmd):

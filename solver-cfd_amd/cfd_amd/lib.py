@@ -27,7 +27,9 @@ EXPORTS_CFD = [
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
-    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_last_error", "cfd_version",
+    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_mesh_reference", "cfd_porous_flags_reference",
+    "cfd_nodal_velocity", "cfd_write_state_table", "cfd_read_state_table", "cfd_write_restart", "cfd_read_restart",
+    "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
     "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
@@ -103,6 +105,14 @@ def load():
         "cfd_batch_run": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_batch_last_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_batch_copy_state": (ctypes.c_int, [_vp, _vp]),
+        "cfd_mesh_reference": (ctypes.c_int, [i32, i32, dbl, dbl, dbl, dbl, dbl, _dp, _dp]),
+        "cfd_porous_flags_reference": (ctypes.c_int, [i32, i32, _dp, _dp, dbl, dbl, dbl, _dp, _dp,
+                                                      ctypes.POINTER(i32)]),
+        "cfd_nodal_velocity": (ctypes.c_int, [i32, i32, _dp, _dp, _dp, _dp]),
+        "cfd_write_state_table": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 7),
+        "cfd_read_state_table": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 7),
+        "cfd_write_restart": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 5),
+        "cfd_read_restart": (ctypes.c_int, [ctypes.c_char_p, i32, i32, i32] + [_dp] * 5),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -375,3 +385,57 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- mesh, flags, nodal fields and state files --------------------------------------------
+def mesh_reference(imax, jmax, lhori=12.0, y_down=5.0, px=1.0, py=1.6, q=1.0):
+    """x[0..imax], y[0..jmax] of the reference mesh routine (index 0 unused)."""
+    x = np.zeros(imax + 1); y = np.zeros(jmax + 1)
+    check(load().cfd_mesh_reference(int(imax), int(jmax), float(lhori), float(y_down), float(px), float(py),
+                                     float(q), _ptr(x), _ptr(y)))
+    return x, y
+
+
+def porous_flags_reference(x, y, x_edge=-4.0, y_edge=0.5, porosity=0.5):
+    imax, jmax = len(x) - 1, len(y) - 1
+    eps = np.zeros((imax + 1, jmax + 1)); lp = np.zeros_like(eps)
+    fl = np.zeros((imax + 1, jmax + 1), dtype=np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+    check(load().cfd_porous_flags_reference(imax, jmax, _ptr(x), _ptr(y), float(x_edge), float(y_edge),
+                                            float(porosity), _ptr(eps), _ptr(lp),
+                                            fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return eps, lp, fl
+
+
+def nodal_velocity(um, vm):
+    imax, jmax = um.shape[0] - 2, um.shape[1] - 1
+    u = np.zeros((imax + 1, jmax + 1)); v = np.zeros_like(u)
+    um = np.ascontiguousarray(um, dtype=np.float64); vm = np.ascontiguousarray(vm, dtype=np.float64)
+    check(load().cfd_nodal_velocity(imax, jmax, _ptr(um), _ptr(vm), _ptr(u), _ptr(v)))
+    return u, v
+
+
+def write_state_table(path, x, y, u, v, p, T, C):
+    imax, jmax = len(x) - 1, len(y) - 1
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, p, T, C)]
+    check(load().cfd_write_state_table(path.encode(), imax, jmax, *[_ptr(a) for a in arrs]))
+
+
+def read_state_table(path, imax, jmax):
+    x = np.zeros(imax + 1); y = np.zeros(jmax + 1)
+    f = [np.zeros((imax + 1, jmax + 1)) for _ in range(5)]
+    check(load().cfd_read_state_table(path.encode(), imax, jmax, _ptr(x), _ptr(y), *[_ptr(a) for a in f]))
+    return dict(x=x, y=y, u=f[0], v=f[1], p=f[2], T=f[3], C=f[4])
+
+
+def write_restart(d, um, vm, p, T, C):
+    imax, jmax = um.shape[0] - 2, um.shape[1] - 1
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (um, vm, p, T, C)]
+    check(load().cfd_write_restart(d.encode(), imax, jmax, *[_ptr(a) for a in arrs]))
+
+
+def read_restart(d, imax, jmax, order=0):
+    um = np.zeros((imax + 2, jmax + 1)); vm = np.zeros((imax + 1, jmax + 2))
+    p, T, C = (np.zeros((imax + 1, jmax + 1)) for _ in range(3))
+    check(load().cfd_read_restart(d.encode(), imax, jmax, int(order), _ptr(um), _ptr(vm), _ptr(p), _ptr(T), _ptr(C)))
+    return dict(um=um, vm=vm, p=p, T=T, C=C)

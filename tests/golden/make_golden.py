@@ -252,7 +252,22 @@ def case_mesh51():
     print("wrote mesh51")
 
 
-CASES = dict(ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
+def case_refout51x84():
+    """The reference's own shipped final state, OpenMp_Multicore/data/output_variables.dat
+    (51 x 84 nodes, records wrapped over lines by the writing compiler): node coordinates and
+    the first 60 records, plus the raw text of its first 12 lines for the wrapped-record
+    parser."""
+    path = os.path.join(os.environ.get("CFD_REFERENCE_DIR", "/root/reference"),
+                        "OpenMp_Multicore", "data", "output_variables.dat")
+    text = open(path).read()
+    d = np.array([float(t) for t in text.split()]).reshape(-1, 7)
+    assert d.shape[0] == 51 * 84
+    np.savez_compressed(os.path.join(OUT, "refout51x84.npz"), x=d[::84, 0], y=d[:84, 1], head=d[:60],
+                        text=np.array("".join(text.splitlines(True)[:12])))
+    print("wrote refout51x84")
+
+
+CASES = dict(refout51x84=case_refout51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
              phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 
 if __name__ == "__main__":

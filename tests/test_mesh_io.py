@@ -1,0 +1,118 @@
+"""CPU: the producers and consumers either side of the pseudo-time iteration, in the C
+library — reference mesh, porous-block flags, nodal velocities, state table and restart
+files — against the flang-built reference mesh routine (mesh51), the reference's own shipped
+final-state file (refout51x84) and closed forms."""
+import os
+
+import numpy as np
+import pytest
+
+from cfd_amd import lib as L
+from helpers import load_golden, GOLDEN
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    L.load()
+
+
+def test_mesh_reference_matches_fortran_mesh_exactly():
+    prm, x, y, eps, lp, st, out, meta = load_golden("mesh51")
+    X, Y = L.mesh_reference(51, 84, meta["Lhori"], 5.0, meta["Px"], meta["Py"], meta["Q"])
+    assert np.array_equal(X[1:], out["x"][1:])
+    assert np.array_equal(Y[1:], out["y"][1:])
+
+
+def test_porous_flags_match_fortran_mesh_exactly():
+    """Block x <= -4, y >= 0.5 with porosidade 0.5 (comum.f90:29), flags in the reference's
+    in-place loop order."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("mesh51")
+    e, l, fl = L.porous_flags_reference(out["x"], out["y"], -4.0, 0.5, 0.5)
+    assert np.array_equal(e[1:, 1:], out["eps"][1:, 1:])
+    assert np.array_equal(l[1:, 1:], out["lp"][1:, 1:])
+    # closed-form checks of the flag classes: interior cells sit inside the block, every
+    # boundary cell touches a fluid cell, every side cell touches a boundary cell
+    X, Y = np.meshgrid(out["x"], out["y"], indexing="ij")
+    assert np.all((fl == 2) <= ((X <= -4.0) & (Y >= 0.5)))
+    assert (fl == 1).sum() > 0 and (fl == 3).sum() > 0
+
+
+def test_mesh_reference_reproduces_reference_output_file():
+    """The node coordinates of the reference's shipped final state (data/output_variables.dat,
+    written with 16-17 significant digits) are the mesh of comum.f90's default parameters
+    (Lhori 12, y_down 5, Px 1.0, Py 1.6, Q 1.0)."""
+    g = np.load(os.path.join(GOLDEN, "refout51x84.npz"), allow_pickle=False)
+    X, Y = L.mesh_reference(51, 84, 12.0, 5.0, 1.0, 1.6, 1.0)
+    assert np.max(np.abs(X[1:] - g["x"])) <= 4e-15 * 6
+    assert np.max(np.abs(Y[1:] - g["y"])) <= 4e-15 * 15
+
+
+def test_state_table_reads_wrapped_reference_records(tmp_path):
+    """The reader takes the reference file's own text (records wrapped over lines) and our
+    one-record-per-line output alike."""
+    g = np.load(os.path.join(GOLDEN, "refout51x84.npz"), allow_pickle=False)
+    text = str(g["text"])
+    nums = np.array([float(t) for t in text.split()])
+    n_rec = nums.size // 7
+    assert n_rec >= 3
+    # a 1 x n_rec table is the first n_rec records of column i = 1
+    f = tmp_path / "head.dat"
+    f.write_text(" ".join(text.split()[:7 * n_rec]) + "\n")
+    t = L.read_state_table(str(f), 1, n_rec)
+    head = g["head"]
+    assert np.array_equal(t["y"][1:], head[:n_rec, 1])
+    for k, col in (("u", 2), ("v", 3), ("p", 4), ("T", 5), ("C", 6)):
+        assert np.array_equal(t[k][1, 1:], head[:n_rec, col]), k
+
+
+def test_state_table_round_trip(tmp_path):
+    rng = np.random.default_rng(5)
+    im, jm = 7, 5
+    x = np.concatenate([[0.0], np.sort(rng.random(im))]); y = np.concatenate([[0.0], np.sort(rng.random(jm))])
+    f = {k: rng.standard_normal((im + 1, jm + 1)) for k in ("u", "v", "p", "T", "C")}
+    path = str(tmp_path / "out.dat")
+    L.write_state_table(path, x, y, f["u"], f["v"], f["p"], f["T"], f["C"])
+    t = L.read_state_table(path, im, jm)
+    assert np.array_equal(t["x"][1:], x[1:]) and np.array_equal(t["y"][1:], y[1:])
+    for k in f:
+        assert np.array_equal(t[k][1:, 1:], f[k][1:, 1:]), k
+    # the write order is i outer, j inner: record 2 is (i=1, j=2)
+    nums = np.array([float(s) for s in open(path).read().split()]).reshape(-1, 7)
+    assert nums[1, 0] == x[1] and nums[1, 1] == y[2] and nums[jm, 0] == x[2]
+
+
+def test_restart_round_trip_and_reference_read_order(tmp_path):
+    im, jm = 6, 4
+    um = np.zeros((im + 2, jm + 1)); vm = np.zeros((im + 1, jm + 2))
+    p, T, C = (np.zeros((im + 1, jm + 1)) for _ in range(3))
+    um[1:, 1:] = np.arange((im + 1) * jm).reshape(im + 1, jm) + 0.25
+    vm[1:, 1:] = -np.arange(im * (jm + 1)).reshape(im, jm + 1) - 0.5
+    for a, s in ((p, 1.0), (T, 2.0), (C, 3.0)):
+        a[1:, 1:] = s * 1000 + np.arange(im * jm).reshape(im, jm)
+    d = str(tmp_path)
+    L.write_restart(d, um, vm, p, T, C)
+    w = L.read_restart(d, im, jm, order=1)
+    for k, a in (("um", um), ("vm", vm), ("p", p), ("T", T), ("C", C)):
+        assert np.array_equal(w[k], a), k
+    # initial.f90:52-87 reads j outer: the i-outer stream lands in Fortran (column-major) order
+    r = L.read_restart(d, im, jm, order=0)
+    exp_u = np.zeros_like(um)
+    exp_u[1:, 1:] = um[1:, 1:].reshape(-1).reshape((im + 1, jm), order="F")
+    assert np.array_equal(r["um"], exp_u)
+    exp_p = np.zeros_like(p)
+    exp_p[1:, 1:] = p[1:, 1:].reshape(-1).reshape((im, jm), order="F")
+    assert np.array_equal(r["p"], exp_p)
+
+
+def test_nodal_velocity_closed_form():
+    """comp_mean of a linear staggered field is the field at the node."""
+    im, jm = 9, 7
+    um = np.zeros((im + 2, jm + 1)); vm = np.zeros((im + 1, jm + 2))
+    i = np.arange(im + 2)[:, None]; j = np.arange(jm + 1)[None, :]
+    um[:] = 2.0 * i + 0.5 * j
+    i2 = np.arange(im + 1)[:, None]; j2 = np.arange(jm + 2)[None, :]
+    vm[:] = -1.0 * i2 + 3.0 * j2
+    u, v = L.nodal_velocity(um, vm)
+    I = np.arange(1, im + 1)[:, None]; J = np.arange(1, jm + 1)[None, :]
+    assert np.array_equal(u[1:, 1:], 2.0 * (I + 0.5) + 0.5 * J)
+    assert np.array_equal(v[1:, 1:], -1.0 * I + 3.0 * (J + 0.5))
