@@ -94,8 +94,9 @@ def bench_batch(args):
     # FP64 batch: the plain path's step counts
     b64 = _build_batch(cs, P64)
     tiles64, _ = b64.autotune(steps=20)
-    code, it64, res64, st64 = b64.run(args.batch_tol, args.batch_itc, steps_per_graph=500, diverge_factor=0.0)
-    ms64, steps64, cu64 = b64.last_timing()
+    (code, it64, res64, st64), ms64 = b64.run_sliced(args.batch_tol, args.batch_itc, log=log)
+    cu64 = int(np.sum(it64 * cells))
+    steps64 = int(it64.max())
     conv = st64 == 1
     for i in np.nonzero((st64 == 2) | (st64 == 3))[0]:
         c = cs[i]
@@ -125,13 +126,14 @@ def bench_batch(args):
     b32 = _build_batch(cs, P32)
     tiles32, _ = b32.autotune(steps=20)
     tol32 = max(args.batch_tol, 5e-7)
-    code, it32, res32, st32 = b32.run(tol32, args.batch_itc, steps_per_graph=500)
-    ms32, steps32, cu32 = b32.last_timing()
+    (code, it32, res32, st32), ms32 = b32.run_sliced(tol32, args.batch_itc, log=log)
+    cu32 = int(np.sum(it32 * cells))
+    steps32 = int(it32.max())
     bfin = _build_batch(cs, P64)
     bfin.set_tiles(*tiles64)
     bfin.copy_state_from(b32)
-    code, itf, resf, stf = bfin.run(args.batch_tol, args.batch_itc, steps_per_graph=500)
-    msf, stepsf, cuf = bfin.last_timing()
+    (code, itf, resf, stf), msf = bfin.run_sliced(args.batch_tol, args.batch_itc, log=log)
+    stepsf = int(itf.max())
     convb = stf == 1
     both = conv & convb
     t_batch = (ms32 + msf) * 1e-3

@@ -307,6 +307,11 @@ int  cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *ms_
  * states. */
 int  cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
                    int64_t *iters, double *res, int32_t *status);
+/* Continues the last run: cases it stopped at itc_max go on (their step counts keep
+ * growing, itc_max is the new total), the others stay stopped. Lets a caller run a long
+ * solve in slices. */
+int  cfd_batch_resume(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
+                      int64_t *iters, double *res, int32_t *status);
 /* Copies every case's current iterates and porosity from src into dst (same cases, same
  * grid sizes), converting precision on the device: e.g. an FP32 batch that has reached the
  * residual its storage can resolve hands over to an FP64 batch for the last decades. */

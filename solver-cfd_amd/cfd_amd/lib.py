@@ -27,7 +27,7 @@ EXPORTS_CFD = [
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
-    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_mesh_reference", "cfd_porous_flags_reference",
+    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_mesh_reference", "cfd_porous_flags_reference",
     "cfd_nodal_velocity", "cfd_write_state_table", "cfd_read_state_table", "cfd_write_restart", "cfd_read_restart",
     "cfd_last_error", "cfd_version",
 ]
@@ -105,6 +105,7 @@ def load():
         "cfd_batch_run": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_batch_last_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_batch_copy_state": (ctypes.c_int, [_vp, _vp]),
+        "cfd_batch_resume": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_mesh_reference": (ctypes.c_int, [i32, i32, dbl, dbl, dbl, dbl, dbl, _dp, _dp]),
         "cfd_porous_flags_reference": (ctypes.c_int, [i32, i32, _dp, _dp, dbl, dbl, dbl, _dp, _dp,
                                                       ctypes.POINTER(i32)]),
@@ -354,13 +355,14 @@ class Batch:
         check(load().cfd_batch_autotune(self.h, int(steps), ctypes.byref(t), _ptr(ms)))
         return t.as_tuple(), float(ms[0])
 
-    def run(self, tol, itc_max, steps_per_graph=50, diverge_factor=0.0, raise_on_fail=False):
-        """Returns (code, iters[n], res[n, 3], status[n])."""
+    def run(self, tol, itc_max, steps_per_graph=50, diverge_factor=0.0, raise_on_fail=False, resume=False):
+        """Returns (code, iters[n], res[n, 3], status[n]). resume=True continues the last run
+        (cases stopped at itc_max go on up to the new total itc_max)."""
         it = (ctypes.c_int64 * self.n)()
         st = (ctypes.c_int32 * self.n)()
         res = np.zeros((self.n, 3))
-        code = load().cfd_batch_run(self.h, float(tol), int(itc_max), int(steps_per_graph), float(diverge_factor),
-                                    it, _ptr(res), st)
+        fn = load().cfd_batch_resume if resume else load().cfd_batch_run
+        code = fn(self.h, float(tol), int(itc_max), int(steps_per_graph), float(diverge_factor), it, _ptr(res), st)
         if raise_on_fail:
             check(code)
         return code, np.array(it[:], dtype=np.int64), res, np.array(st[:], dtype=np.int32)
@@ -369,6 +371,24 @@ class Batch:
         """Take every case's current iterates from another batch of the same cases
         (precision converted on the device)."""
         check(load().cfd_batch_copy_state(self.h, other.h))
+
+    def run_sliced(self, tol, itc_max, slice_steps=10000, steps_per_graph=500, diverge_factor=0.0, log=None):
+        """run() in slices of slice_steps (progress through `log`); returns the final
+        (code, iters, res, status) and the summed event time in ms."""
+        total_ms, done, first = 0.0, 0, True
+        while True:
+            done = min(itc_max, done + slice_steps)
+            out = self.run(tol, done, steps_per_graph, diverge_factor, resume=not first)
+            first = False
+            ms, steps, cells = self.last_timing()
+            total_ms += ms
+            code, it, res, st = out
+            if log:
+                log("  %d steps: %d running, %d converged, %d failed, %.1f s"
+                    % (done, int(((st == 0) | (st == 4)).sum()), int((st == 1).sum()), int(((st == 2) | (st == 3)).sum()),
+                       total_ms * 1e-3))
+            if done >= itc_max or not ((st == 4) | (st == 0)).any():
+                return out, total_ms
 
     def last_timing(self):
         ms = ctypes.c_double(0); steps = ctypes.c_int64(0); cells = ctypes.c_int64(0)

@@ -495,15 +495,32 @@ static int reset_control(cfd_batch *b)
     return r;
 }
 
-int cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
-                  int64_t *iters, double *res, int32_t *status)
+static int batch_run(cfd_batch *b, int resume, double tol, int64_t itc_max, int32_t steps_per_graph,
+                     double diverge_factor, int64_t *iters, double *res, int32_t *status)
 {
     if (!b) return cfd__fail(CFD_ERR_STATE, "no batch");
     if (itc_max < 1) return cfd__fail(CFD_ERR_ARG, "itc_max must be >= 1");
     if (steps_per_graph < 1) steps_per_graph = 1;
     cfd_batch_launch L;
     BCHK(make_launch(b, &L, tol, itc_max, diverge_factor));
-    BCHK(reset_control(b));
+    if (!resume) {
+        BCHK(reset_control(b));
+    } else {
+        /* cases stopped only by the previous itc_max go on; the others stay stopped */
+        const int32_t n = b->n;
+        int32_t *ctl = (int32_t *)malloc((size_t)n * sizeof(int32_t)), *st = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+        hipError_t e = (ctl && st) ? hipStreamSynchronize(b->s) : hipErrorOutOfMemory;
+        if (e == hipSuccess) e = hipMemcpy(ctl, b->ctl, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(st, b->status, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) {
+            for (int32_t c = 0; c < n; ++c)
+                if (st[c] == 4 || st[c] == 0) { ctl[c] |= 1; st[c] = 0; }
+            e = hipMemcpy(b->ctl, ctl, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(b->status, st, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice);
+        }
+        free(ctl); free(st);
+        if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch resume: %s", hipGetErrorString(e));
+    }
     /* capture steps_per_graph batched steps + the active count */
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
@@ -560,6 +577,18 @@ int cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_g
     if (failed) return cfd__fail(CFD_ERR_NONFINITE, "%d case(s) produced a non-finite or diverging residual", failed);
     if (tol > 0 && unconv) return cfd__fail(CFD_ERR_NOCONV, "%d case(s) did not reach %.3e in %lld steps", unconv, tol, (long long)itc_max);
     return CFD_OK;
+}
+
+int cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
+                  int64_t *iters, double *res, int32_t *status)
+{
+    return batch_run(b, 0, tol, itc_max, steps_per_graph, diverge_factor, iters, res, status);
+}
+
+int cfd_batch_resume(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
+                     int64_t *iters, double *res, int32_t *status)
+{
+    return batch_run(b, 1, tol, itc_max, steps_per_graph, diverge_factor, iters, res, status);
 }
 
 int cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates)

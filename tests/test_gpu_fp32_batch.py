@@ -226,3 +226,25 @@ def test_batch_autotune_restores_state():
             assert np.array_equal(b.download(i, k), ctx.download(k)), (i, k)
         ctx.close()
     b.close()
+
+
+def test_batch_resume_equals_one_run():
+    """A run sliced with resume (10 + 15 steps) gives the state and counts of one 25-step run."""
+    say("case: batch resume")
+    outs = []
+    for sliced in (False, True):
+        cases = _bc_consistent(PREC_FP64, 1)
+        b = _make_batch(cases, PREC_FP64)
+        if sliced:
+            b.run(0.0, 10, steps_per_graph=5)
+            code, it, res, st = b.run(0.0, 25, steps_per_graph=5, resume=True)
+        else:
+            code, it, res, st = b.run(0.0, 25, steps_per_graph=5)
+        outs.append((list(it), [{k: b.download(i, k) for k in FIELDS} for i in range(len(cases))]))
+        for c in cases:
+            c[6].close()
+        b.close()
+    assert outs[0][0] == outs[1][0] == [25] * len(BATCH_SHAPES)
+    for i in range(len(BATCH_SHAPES)):
+        for k in FIELDS:
+            assert np.array_equal(outs[0][1][i][k], outs[1][1][i][k]), (i, k)
