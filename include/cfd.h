@@ -320,6 +320,47 @@ int  cfd_batch_copy_state(cfd_batch *dst, cfd_batch *src);
  * cell-updates done (sum over cases of pseudo-steps x imax x jmax). */
 int  cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates);
 
+/* ---- learned warm start --------------------------------------------------------------
+ * A 4-level convolutional U-Net (channels 32/64/128/256, 3x3 kernels, 2x2 max pooling,
+ * nearest upsampling with skip concatenation) trained and evaluated with hand-written HIP
+ * kernels: BF16 activations, BF16 MFMA convolutions with FP32 accumulation, FP32 master
+ * weights and Adam. Inputs [B][H][W][5] (mask L, porosity, log Re, log Da, Ri per cell),
+ * outputs [B][H][W][4] (u, v, p, T), host float arrays; H and W multiples of 8. */
+typedef struct cfd_unet cfd_unet;
+int  cfd_unet_create(int32_t height, int32_t width, int32_t max_batch, uint64_t seed, cfd_unet **out);
+int  cfd_unet_destroy(cfd_unet *net);
+int  cfd_unet_num_params(const cfd_unet *net, int64_t *n);
+/* Flat FP32 parameters: per convolution, weights [Cout][3][3][Cin] then biases [Cout], in
+ * network order (encoder, bottleneck, decoder, head). set_params also resets Adam. */
+int  cfd_unet_get_params(cfd_unet *net, float *host);
+int  cfd_unet_set_params(cfd_unet *net, const float *host);
+int  cfd_unet_predict(cfd_unet *net, const float *x, int32_t batch, float *y);
+/* Mean squared error over all outputs and its gradient w.r.t. the parameters (grad may be
+ * NULL: it then stays on the device for the next update). */
+int  cfd_unet_loss_grad(cfd_unet *net, const float *x, const float *target, int32_t batch, double *loss, float *grad);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on the batch; *loss before the update. */
+int  cfd_unet_train_step(cfd_unet *net, const float *x, const float *target, int32_t batch, float lr, double *loss);
+
+/* The U-Net's building blocks on device pointers (BF16 as uint16 bit patterns, NHWC,
+ * NULL stream): 3x3 convolution (stride 1, zero padding 1; weights [Cout][9 Cin]) with
+ * optional bias and ReLU into BF16 y and/or FP32 y32; its weight gradient accumulated into
+ * FP32 dw [Cout][9 Cin]; the flipped weights that turn the forward kernel into the input
+ * gradient; 2x2 max pooling and its backward (gradient to the first maximal input, added
+ * into FP32 gx); nearest x2 upsampling into channels cofs.. of a Cy-channel tensor, and its
+ * backward (sum of the 2x2 children, added into FP32 gx). */
+int  cfd_nn_conv3x3(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t Cin, const uint16_t *w,
+                    const float *bias, int32_t Cout, int32_t relu, uint16_t *y, float *y32);
+int  cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin,
+                          int32_t Cout, float *dw);
+int  cfd_nn_flip_weights(const float *w, int32_t Cin, int32_t Cout, uint16_t *wf);
+int  cfd_nn_maxpool2(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t *y);
+int  cfd_nn_maxpool2_bwd(const uint16_t *x, const uint16_t *y, const uint16_t *gy, int32_t B, int32_t H, int32_t W,
+                         int32_t C, float *gx);
+int  cfd_nn_upsample2(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t *y, int32_t Cy,
+                      int32_t cofs);
+int  cfd_nn_upsample2_bwd(const uint16_t *gy, int32_t B, int32_t H, int32_t W, int32_t C, int32_t Cy, int32_t cofs,
+                          float *gx);
+
 const char *cfd_last_error(void);
 const char *cfd_version(void);
 

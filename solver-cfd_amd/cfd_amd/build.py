@@ -33,6 +33,7 @@ HIP_COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wno-unus
 HIP_SOURCES = [
     ("cfd_ops.hip", ["-ffp-contract=off"]),
     ("cfd_fused.hip", ["-ffp-contract=fast"]),
+    ("cfd_unet.hip", ["-ffp-contract=fast"]),
 ]
 C_SOURCES = ["cfd_host.c", "cfd_batch.c", "cfd_mesh_io.c"]
 

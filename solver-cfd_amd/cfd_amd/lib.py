@@ -28,6 +28,9 @@ EXPORTS_CFD = [
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
     "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_mesh_reference", "cfd_porous_flags_reference",
+    "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_num_params", "cfd_unet_get_params", "cfd_unet_set_params",
+    "cfd_unet_predict", "cfd_unet_loss_grad", "cfd_unet_train_step", "cfd_nn_conv3x3", "cfd_nn_conv3x3_wgrad",
+    "cfd_nn_flip_weights", "cfd_nn_maxpool2", "cfd_nn_maxpool2_bwd", "cfd_nn_upsample2", "cfd_nn_upsample2_bwd",
     "cfd_nodal_velocity", "cfd_write_state_table", "cfd_read_state_table", "cfd_write_restart", "cfd_read_restart",
     "cfd_last_error", "cfd_version",
 ]
@@ -55,6 +58,7 @@ class CfdError(RuntimeError):
 
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -114,6 +118,21 @@ def load():
         "cfd_read_state_table": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 7),
         "cfd_write_restart": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 5),
         "cfd_read_restart": (ctypes.c_int, [ctypes.c_char_p, i32, i32, i32] + [_dp] * 5),
+        "cfd_unet_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+        "cfd_unet_destroy": (ctypes.c_int, [_vp]),
+        "cfd_unet_num_params": (ctypes.c_int, [_vp, ctypes.POINTER(i64)]),
+        "cfd_unet_get_params": (ctypes.c_int, [_vp, _fp]),
+        "cfd_unet_set_params": (ctypes.c_int, [_vp, _fp]),
+        "cfd_unet_predict": (ctypes.c_int, [_vp, _fp, i32, _fp]),
+        "cfd_unet_loss_grad": (ctypes.c_int, [_vp, _fp, _fp, i32, _dp, _fp]),
+        "cfd_unet_train_step": (ctypes.c_int, [_vp, _fp, _fp, i32, ctypes.c_float, _dp]),
+        "cfd_nn_conv3x3": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp, _vp, i32, i32, _vp, _vp]),
+        "cfd_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, i32, i32, i32, i32, i32, _vp]),
+        "cfd_nn_flip_weights": (ctypes.c_int, [_vp, i32, i32, _vp]),
+        "cfd_nn_maxpool2": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp]),
+        "cfd_nn_maxpool2_bwd": (ctypes.c_int, [_vp, _vp, _vp, i32, i32, i32, i32, _vp]),
+        "cfd_nn_upsample2": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp, i32, i32]),
+        "cfd_nn_upsample2_bwd": (ctypes.c_int, [_vp, i32, i32, i32, i32, i32, i32, _vp]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -459,3 +478,59 @@ def read_restart(d, imax, jmax, order=0):
     p, T, C = (np.zeros((imax + 1, jmax + 1)) for _ in range(3))
     check(load().cfd_read_restart(d.encode(), imax, jmax, int(order), _ptr(um), _ptr(vm), _ptr(p), _ptr(T), _ptr(C)))
     return dict(um=um, vm=vm, p=p, T=T, C=C)
+
+
+class UNet:
+    """The warm-start U-Net (include/cfd.h, cfd_unet_*): host float32 arrays in and out."""
+
+    def __init__(self, height, width, max_batch, seed=5):
+        h = _vp()
+        check(load().cfd_unet_create(int(height), int(width), int(max_batch), int(seed), ctypes.byref(h)))
+        self.h = h
+        self.H, self.W, self.max_batch = height, width, max_batch
+
+    def num_params(self):
+        n = ctypes.c_int64(0)
+        check(load().cfd_unet_num_params(self.h, ctypes.byref(n)))
+        return n.value
+
+    def get_params(self):
+        a = np.zeros(self.num_params(), dtype=np.float32)
+        check(load().cfd_unet_get_params(self.h, a.ctypes.data_as(_fp)))
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        check(load().cfd_unet_set_params(self.h, a.ctypes.data_as(_fp)))
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[:3] + (4,), dtype=np.float32)
+        check(load().cfd_unet_predict(self.h, x.ctypes.data_as(_fp), x.shape[0], y.ctypes.data_as(_fp)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32); t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        loss = ctypes.c_double(0)
+        check(load().cfd_unet_loss_grad(self.h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), x.shape[0],
+                                        ctypes.byref(loss), g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32); t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = ctypes.c_double(0)
+        check(load().cfd_unet_train_step(self.h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), x.shape[0],
+                                         ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            load().cfd_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

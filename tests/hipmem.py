@@ -24,23 +24,26 @@ H2D, D2H = 1, 2
 
 
 class DeviceArray:
-    """A device copy of a float64 numpy array (same shape)."""
+    """A device copy of a numpy array (same shape and dtype: float64 unless the host array is
+    float32 or uint16)."""
 
     def __init__(self, host):
+        host = np.asarray(host)
+        self.dtype = host.dtype if host.dtype in (np.float32, np.uint16) else np.dtype(np.float64)
         self.shape = host.shape
-        self.nbytes = host.nbytes
+        self.nbytes = int(np.prod(host.shape)) * self.dtype.itemsize
         p = ctypes.c_void_p()
         assert hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 8)) == 0
         self.ptr = p.value
         self.put(host)
 
     def put(self, host):
-        h = np.ascontiguousarray(host, dtype=np.float64)
+        h = np.ascontiguousarray(host, dtype=self.dtype)
         assert h.nbytes == self.nbytes
         assert hip().hipMemcpy(self.ptr, h.ctypes.data, self.nbytes, H2D) == 0
 
     def get(self):
-        out = np.zeros(self.shape)
+        out = np.zeros(self.shape, dtype=self.dtype)
         assert hip().hipDeviceSynchronize() == 0
         assert hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H) == 0
         return out

@@ -1,0 +1,66 @@
+"""CPU: the U-Net oracle (oracle/unet_ref.py) pinned by central finite differences of its
+own loss (float64) and by closed forms of its pieces. No reference counterpart exists (the
+reference solver has no learned part), so these checks are what pins it."""
+import numpy as np
+
+from oracle import unet_ref as U
+
+
+def _params(seed, scale=0.3):
+    lay, n = U.param_layout()
+    rng = np.random.default_rng(seed)
+    th = np.zeros(n)
+    for cin, cout, relu, wo, bo in lay:
+        th[wo:wo + cout * 9 * cin] = rng.uniform(-1, 1, cout * 9 * cin) * scale / np.sqrt(9 * cin) * 3
+        th[bo:bo + cout] = rng.uniform(0.0, 0.2, cout)        # positive biases keep units active
+    return th
+
+
+def test_gradient_matches_central_differences():
+    rng = np.random.default_rng(1)
+    th = _params(2)
+    x = rng.standard_normal((1, 8, 8, 5))
+    t = rng.standard_normal((1, 8, 8, 4))
+    loss, g, _ = U.forward_backward(th, x, t, bf=False)
+    lay, n = U.param_layout()
+    # one weight and one bias of every convolution
+    idx = []
+    for q, (cin, cout, relu, wo, bo) in enumerate(lay):
+        idx.append(wo + int(rng.integers(cout * 9 * cin)))
+        idx.append(bo + int(rng.integers(cout)))
+    h = 1e-6
+    for k in idx:
+        tp = th.copy(); tp[k] += h
+        tm = th.copy(); tm[k] -= h
+        fd = (U.loss_only(tp, x, t) - U.loss_only(tm, x, t)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 + 1e-4 * abs(fd), (k, fd, g[k])
+
+
+def test_conv_identity_kernel_and_input_grad_adjoint():
+    """A centre-tap identity kernel reproduces its input; <conv(x), g> = <x, dX(g)> and
+    <conv(x), g> = <W, dW(x, g)> (the two gradients are the adjoints of the forward)."""
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 6, 5, 3))
+    W = np.zeros((3, 3, 3, 3))
+    for c in range(3):
+        W[c, 1, 1, c] = 1.0
+    assert np.allclose(U.conv(x, W, None), x)
+    W = rng.standard_normal((4, 3, 3, 3))
+    g = rng.standard_normal((2, 6, 5, 4))
+    y = U.conv(x, W, None)
+    assert np.isclose(np.sum(y * g), np.sum(x * U.conv_input_grad(g, W)))
+    assert np.isclose(np.sum(y * g), np.sum(W * U.conv_weight_grad(x, g)))
+
+
+def test_pool_upsample_and_bf16_closed_forms():
+    x = np.array([[[[1.0], [3.0]], [[3.0], [2.0]]]])        # one 2x2 window, two maxima
+    y = U.maxpool(x)
+    assert y[0, 0, 0, 0] == 3.0
+    gx = U.maxpool_bwd(x, y, np.full(y.shape, 5.0))
+    assert gx[0, 0, 1, 0] == 5.0 and gx[0, 1, 0, 0] == 0.0     # first maximum in row-major order
+    g = np.arange(16.0).reshape(1, 4, 4, 1)
+    assert np.array_equal(U.upsample_bwd(g)[0, :, :, 0], [[10.0, 18.0], [42.0, 50.0]])
+    assert np.array_equal(U.upsample(np.array([[[[7.0]]]]))[0, :, :, 0], [[7.0, 7.0], [7.0, 7.0]])
+    assert U.bf16(1.0 + 2 ** -9) == 1.0                        # tie to even
+    assert U.bf16(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
+    assert U.bf16(-2.5) == -2.5
