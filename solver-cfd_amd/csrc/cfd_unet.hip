@@ -230,9 +230,9 @@ __global__ void k_maxpool2_bwd(const bf16 *x, const bf16 *y, const bf16 *gy, int
         const int b = (int)(p / (H / 2));
         const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + c;
         const size_t off[4] = {0, (size_t)C, (size_t)W * C, (size_t)W * C + C};
-        const bf16 mv = y[i];
+        const float mv = bf2f(y[i]);           // compared as values: -0 equals +0
         for (int q = 0; q < 4; ++q)
-            if (x[base + off[q]] == mv) { gx[base + off[q]] += bf2f(gy[i]); break; }
+            if (bf2f(x[base + off[q]]) == mv) { gx[base + off[q]] += bf2f(gy[i]); break; }
     }
 }
 
