@@ -155,6 +155,67 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_warmstart(args):
+    """Learned warm start on held-out cases: solver-generated steady states (one grid size)
+    train the U-Net with the library's own kernels; the held-out cases are then solved from
+    the network's prediction (with fallback to zero fields) and from zero fields."""
+    from cfd_amd import warmstart as WS
+    n = args.ws_train + args.ws_test
+    H = args.ws_size
+    tol = args.batch_tol
+    cs = WS.steady_cases(n, seed=5, size=H, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
+    log("warm start: %d cases on %dx%d, training data to %.0e, held-out solves to %.0e"
+        % (n, H, H, args.ws_train_tol, tol))
+    sol0 = WS.solve(cs, args.ws_train_tol, args.ws_itc, log=log)
+    conv = sol0["status"] == 1
+    log("training-data solve: %d/%d converged, %.1f s" % (int(conv.sum()), n, sol0["seconds"]))
+    X = np.stack([WS.features(c) for c in cs])
+    Y = np.stack([WS.targets(sol0["batch"], i) for i in range(n)])
+    sol0["batch"].close()
+    train_idx = [i for i in range(args.ws_train) if conv[i]]
+    test_idx = [i for i in range(args.ws_train, n) if conv[i]]
+    mu = Y[train_idx].mean(axis=(0, 1, 2)); sd = Y[train_idx].std(axis=(0, 1, 2)) + 1e-6
+    Yn = ((Y - mu) / sd).astype(np.float32)
+    net = L.UNet(H, H, 16, seed=5)
+    t1 = time.perf_counter()
+    hist = WS.train(net, X[train_idx], Yn[train_idx], args.ws_epochs, 16, 1e-3, log=log)
+    t_train = time.perf_counter() - t1
+    pred = np.concatenate([net.predict(X[test_idx[k:k + 16]]) for k in range(0, len(test_idx), 16)])
+    pred = pred * sd + mu
+    pred_err = float(np.sqrt(np.mean((pred - Y[test_idx]) ** 2)) / np.sqrt(np.mean(Y[test_idx] ** 2)))
+    test_cases = [cs[i] for i in test_idx]
+    states = [WS.state_from_prediction(pred[k], H, H) for k in range(len(test_idx))]
+    # held-out cases from the prediction (fallback to zero fields) and from zero fields
+    solw = WS.solve(test_cases, tol, args.ws_itc, states=states, fallback_factor=10.0, log=log)
+    solz = WS.solve(test_cases, tol, args.ws_itc, log=log)
+    itw = solw["iters32"] + solw["iters64"]; itz = solz["iters32"] + solz["iters64"]
+    both = (solw["status"] == 1) & (solz["status"] == 1)
+    errs = {k: 0.0 for k in ("um_tau", "vm_tau", "p", "T_tau")}
+    for k in np.nonzero(both)[0][:32]:
+        for f in errs:
+            a, b = solw["batch"].download(int(k), f), solz["batch"].download(int(k), f)
+            errs[f] = max(errs[f], float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)))
+    solw["batch"].close(); solz["batch"].close()
+    out = {
+        "metric": "warm-start speedup (held-out batch time to residual, zero init / U-Net init)",
+        "value": solz["seconds"] / solw["seconds"], "unit": "x", "n_gpus": 1, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (U-Net) + f32/f64 (solver)",
+        "data": "synthetic (solver-generated steady states, seeded cases)",
+        "config": {"workload": "U-Net warm start, %dx%d grids" % (H, H), "train": len(train_idx),
+                   "held_out": len(test_idx), "tol": tol, "train_tol": args.ws_train_tol, "epochs": args.ws_epochs},
+        "train_loss_first_last": [hist[0], hist[-1]], "train_s": t_train,
+        "prediction_rel_rms_err": pred_err,
+        "iters_median_zero": float(np.median(itz)), "iters_median_warm": float(np.median(itw)),
+        "iters_total_zero": int(itz.sum()), "iters_total_warm": int(itw.sum()),
+        "fallbacks": int(solw["fallback"].sum()),
+        "converged_zero": int((solz["status"] == 1).sum()), "converged_warm": int((solw["status"] == 1).sum()),
+        "solve_s_zero": solz["seconds"], "solve_s_warm": solw["seconds"],
+        "steady_field_max_rel_diff_warm_vs_zero": errs,
+    }
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,16 +237,24 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
-    ap.add_argument("--workload", default="grid", choices=["grid", "batch"],
+    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart"],
                     help="grid: one large grid (the cell-update rate); batch: time to residual of the "
                          "256-case mixed batch against the plain FP64 path")
     ap.add_argument("--batch-tol", type=float, default=1e-6)
     ap.add_argument("--batch-itc", type=int, default=50000)
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--batch-beta", type=float, default=200.0)
+    ap.add_argument("--ws-train", type=int, default=2048)
+    ap.add_argument("--ws-test", type=int, default=256)
+    ap.add_argument("--ws-size", type=int, default=64)
+    ap.add_argument("--ws-epochs", type=int, default=30)
+    ap.add_argument("--ws-itc", type=int, default=60000)
+    ap.add_argument("--ws-train-tol", type=float, default=1e-5)
     args = ap.parse_args()
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "warmstart":
+        return bench_warmstart(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the pseudo-time "

@@ -68,7 +68,7 @@ def build_batch(cases, prec, states=None):
 
 def solve(cases, tol, itc_max, states=None, switch_tol=5e-7, fallback_factor=0.0, log=None):
     """Steady solve of every case: FP32 batch down to max(tol, switch_tol), FP64 finish to
-    tol. With fallback_factor > 0, cases whose residual grows past that factor x their best
+    tol when tol is below what FP32 storage resolves (switch_tol). With fallback_factor > 0, cases whose residual grows past that factor x their best
     (from a warm start) are restarted from their zero initial state. Returns a dict with
     per-case iterations (FP32 + FP64), status, event time (s), and the final FP64 batch."""
     b32 = build_batch(cases, PREC_FP32, states)
@@ -90,6 +90,9 @@ def solve(cases, tol, itc_max, states=None, switch_tol=5e-7, fallback_factor=0.0
                 it32[i] += it2[k]
                 st32[i] = s2[k]
             bfb.close()
+    if tol >= switch_tol:          # FP32 storage resolves tol: no FP64 finish
+        return dict(iters32=it32, iters64=np.zeros_like(it32), status=st32, res=res32, fallback=fallback,
+                    seconds=(ms32 + ms_fb) * 1e-3, seconds32=(ms32 + ms_fb) * 1e-3, seconds64=0.0, batch=b32)
     b64 = L.Batch([c["params"] for c in cases], precision=PREC_FP64)
     for i, c in enumerate(cases):
         b64.set_mesh(i, c["x"], c["y"])
