@@ -216,6 +216,43 @@ def bench_warmstart(args):
     print(json.dumps(out), flush=True)
 
 
+def unet_flops(H, W, B):
+    """Multiply-add FLOPs (x2) of one forward pass of the U-Net on B H x W inputs."""
+    from oracle import unet_ref as U       # layer table only (channel counts)
+    spec = U.SPEC
+    res = [1, 1, 2, 2, 4, 4, 8, 8, 4, 4, 2, 2, 1, 1, 1]   # downsampling factor of each conv
+    return sum(2.0 * B * (H // r) * (W // r) * 9 * cin * cout for (cin, cout, _), r in zip(spec, res))
+
+
+def bench_unet(args):
+    """U-Net training throughput with the library's kernels: Adam steps on a synthetic batch.
+    One step = forward + input gradients + weight gradients (~3x the forward FLOPs), priced
+    against MI355X's dense BF16 MFMA peak."""
+    H = W = args.ws_size
+    B = 16
+    net = L.UNet(H, W, B, seed=1)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((B, H, W, 5)).astype(np.float32)
+    t = rng.standard_normal((B, H, W, 4)).astype(np.float32)
+    for _ in range(5):
+        net.train_step(x, t, 1e-4)
+    n = max(args.steps // 4, 20)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        net.train_step(x, t, 1e-4)
+    dt = (time.perf_counter() - t0) / n
+    fl = 3.0 * unet_flops(H, W, B)
+    net.close()
+    out = {"metric": "U-Net training step throughput", "value": fl / dt / 1e12, "unit": "TFLOP/s", "n_gpus": 1,
+           "steps": n, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+           "config": {"workload": "U-Net 32/64/128/256, %dx%d, batch %d" % (H, W, B)},
+           "roofline": {"bound": "mfma", "achieved": fl / dt / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": fl / dt / 1e12 / 2500.0, "traffic": None},
+           "note": "whole-step time including host-to-device batch copies and the elementwise kernels"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,7 +274,7 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
-    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart"],
+    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart", "unet"],
                     help="grid: one large grid (the cell-update rate); batch: time to residual of the "
                          "256-case mixed batch against the plain FP64 path")
     ap.add_argument("--batch-tol", type=float, default=1e-6)
@@ -255,6 +292,8 @@ def main():
         return bench_batch(args)
     if args.workload == "warmstart":
         return bench_warmstart(args)
+    if args.workload == "unet":
+        return bench_unet(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the pseudo-time "
