@@ -207,7 +207,8 @@ typedef struct cfd_tiles {
     int32_t rows_uvp;    /* row-block height of the u/v/p kernel, 0 or 1..imax            */
     int32_t rows_tc;     /* row-block height of the T/C kernel, 0 or 1..imax              */
     int32_t waves_uvp;   /* register budget of the u/v/p kernel: 0|1 = one wave per SIMD,
-                            2 = two waves per SIMD (excess registers spilled to scratch) */
+                            2 = two waves per SIMD (excess registers spilled to scratch),
+                            3 = u and v chains in separate waves of a 512-lane workgroup */
     int32_t waves_tc;    /* register budget of the T/C kernel: 0|1 = as compiled (3 waves
                             per SIMD), 4 = four waves per SIMD (with spills)              */
 } cfd_tiles;
@@ -268,6 +269,12 @@ int cfd_write_restart(const char *dir, int32_t imax, int32_t jmax, const double 
                       const double *p, const double *T, const double *C);
 int cfd_read_restart(const char *dir, int32_t imax, int32_t jmax, int32_t order, double *um, double *vm, double *p,
                      double *T, double *C);
+
+/* The ParaView file of transient.f90:55-103 (data/paraview_output*.vtk): structured grid of
+ * the nodes, point data |u|, T, P and the vector (u, v, 0), nodal p-layout host arrays.
+ * title NULL writes the reference's title line. */
+int cfd_write_vtk(const char *path, const char *title, int32_t imax, int32_t jmax, const double *x, const double *y,
+                  const double *u, const double *v, const double *p, const double *T);
 
 /* ---- batched solves ------------------------------------------------------------------
  * Many independent cases (each its own grid size, parameters, mesh and fields) advance

@@ -32,6 +32,7 @@ EXPORTS_CFD = [
     "cfd_unet_predict", "cfd_unet_loss_grad", "cfd_unet_train_step", "cfd_nn_conv3x3", "cfd_nn_conv3x3_wgrad",
     "cfd_nn_flip_weights", "cfd_nn_maxpool2", "cfd_nn_maxpool2_bwd", "cfd_nn_upsample2", "cfd_nn_upsample2_bwd",
     "cfd_nodal_velocity", "cfd_write_state_table", "cfd_read_state_table", "cfd_write_restart", "cfd_read_restart",
+    "cfd_write_vtk",
     "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
@@ -118,6 +119,7 @@ def load():
         "cfd_read_state_table": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 7),
         "cfd_write_restart": (ctypes.c_int, [ctypes.c_char_p, i32, i32] + [_dp] * 5),
         "cfd_read_restart": (ctypes.c_int, [ctypes.c_char_p, i32, i32, i32] + [_dp] * 5),
+        "cfd_write_vtk": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, i32, i32] + [_dp] * 6),
         "cfd_unet_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
         "cfd_unet_destroy": (ctypes.c_int, [_vp]),
         "cfd_unet_num_params": (ctypes.c_int, [_vp, ctypes.POINTER(i64)]),
@@ -471,6 +473,15 @@ def write_restart(d, um, vm, p, T, C):
     imax, jmax = um.shape[0] - 2, um.shape[1] - 1
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (um, vm, p, T, C)]
     check(load().cfd_write_restart(d.encode(), imax, jmax, *[_ptr(a) for a in arrs]))
+
+
+def write_vtk(path, x, y, u, v, p, T, title=None):
+    """ParaView structured-grid file of nodal u, v, p, T (p-layout arrays), transient.f90's
+    format."""
+    imax, jmax = len(x) - 1, len(y) - 1
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, p, T)]
+    check(load().cfd_write_vtk(path.encode(), title.encode() if title is not None else None, imax, jmax,
+                               *[_ptr(a) for a in arrs]))
 
 
 def read_restart(d, imax, jmax, order=0):

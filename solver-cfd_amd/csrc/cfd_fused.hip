@@ -316,10 +316,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         }                                                                         \
                         const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                 \
                         R Rr = rdx_r * (rdy_j * (diff + conv) - ev * R(pw[bk] - pw[bk + 1])) - gg * ev; \
-                        const R k2 = k2w[bk];                                                     \
-                        if (k2 != R(0)) {                                                         \
+                        const R k1 = k1w[bk], k2 = k2w[bk];                                       \
+                        if (k1 != R(0) || k2 != R(0)) {                                           \
                             const R vP = vt[2 * K];                                               \
-                            Rr -= uP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                  \
+                            Rr -= uP * (k1 + k2 * sqrt(uP * uP + vP * vP));                   \
                         }                                                                         \
                         rs = ((STEADY ? R(0) : du[bk]) + Rr * dt) * dtau;                         \
                         const R tau = ut[2 * K];                                                  \
@@ -379,10 +379,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         }                                                                         \
                         const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                 \
                         R Rr = rdy_j * (rdx_r * (diff + conv) - ev * dpj[bk]) + bw[bk];           \
-                        const R k2 = k2w[bk];                                                     \
-                        if (k2 != R(0)) {                                                         \
+                        const R k1 = k1w[bk], k2 = k2w[bk];                                       \
+                        if (k1 != R(0) || k2 != R(0)) {                                           \
                             const R uP = ut[2 * K];                                               \
-                            Rr -= vP * (k1w[bk] + k2 * sqrt(uP * uP + vP * vP));                  \
+                            Rr -= vP * (k1 + k2 * sqrt(uP * uP + vP * vP));                   \
                         }                                                                         \
                         rs = ((STEADY ? R(0) : dv[bk]) + Rr * dt) * dtau;                         \
                         const R tau = vt[2 * K];                                                  \
@@ -462,6 +462,326 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp_batch(const K1Args<R> *cases, 
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
         uvp_tile<R, STEADY, MUPOLY>(a, tl.bi, tl.bj, sm);
         __syncthreads();   // the next tile re-initialises the LDS rings
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// K1, split chains: the u chain and the v chain of a tile in separate waves.
+//
+// The three u stages read v only at the tau level (and v likewise u), so the two chains are
+// independent until the pressure update. A 512-thread workgroup runs the u chain in waves
+// 0-3 and the v chain in waves 4-7 over the same 256 columns; each chain holds only its own
+// register windows (no spills, two waves per SIMD), loads its own inputs, and reads the
+// other field's tau rows from deeper LDS rings (u: 8 rows, v: 16 rows). The pressure update runs in the u
+// chain with v3 from the v chain's ring. Values and operation order are those of
+// uvp_tile, so both forms give identical results.
+// ---------------------------------------------------------------------------------------
+constexpr int NT1S = 2 * NT1;
+
+template <typename R>
+struct Smem1S {
+    R sUT[8][LROW1], sVT[16][LROW1], sT[2][LROW1];
+    R sU1[3][LROW1], sU2[3][LROW1], sV1[3][LROW1], sV2[3][LROW1], sV3[2][LROW1];
+    double sP[2][LROW1];
+};
+
+// The v ring is read down to row t-7 by the u stages while a v wave may already write row
+// t+1, so it holds 16 rows; the u ring is read down to row t-6 and holds 8.
+__device__ __forceinline__ int s8(int r) { return (r + 24) & 7; }
+__device__ __forceinline__ int s16(int r) { return (r + 32) & 15; }
+
+template <typename R, bool STEADY, bool MUPOLY>
+__device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int bj, Smem1S<R> &sm)
+{
+    const int im = a.imax, jm = a.jmax;
+    const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);
+    const int jlo = 2 + bj * W1, jhi = min(jlo + W1, jm);
+    const int x = threadIdx.x & (NT1 - 1);
+    const bool uchain = threadIdx.x < NT1;
+    const int j = jlo - HL1 + x;
+    const bool chan = a.chan != 0, upwind_all = a.upwind_all != 0, bouss = a.bouss != 0;
+    const R dt = R(a.dt), dtau = R(a.dtau), vi = R(a.v_i), inv_re = R(a.inv_re), gg = R(a.g);
+    const int RR = im + 2, Cc = jm + 2;
+    const double *pr = a.pk, *pc = a.pk + 4 * RR;
+    auto sUT = sm.sUT; auto sVT = sm.sVT; auto sT = sm.sT; auto sP = sm.sP;
+    auto sU1 = sm.sU1; auto sU2 = sm.sU2; auto sV1 = sm.sV1; auto sV2 = sm.sV2; auto sV3 = sm.sV3;
+
+    for (int k = threadIdx.x; k < LROW1; k += NT1S) {
+        for (int q = 0; q < 8; ++q) sUT[q][k] = R(0);
+        for (int q = 0; q < 16; ++q) sVT[q][k] = R(0);
+        for (int q = 0; q < 3; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
+        for (int q = 0; q < 2; ++q) { sP[q][k] = 0.0; sT[q][k] = R(1); sV3[q][k] = R(0); }
+    }
+
+    const bool jvalid = j >= 1 && j <= jm + 1;
+    const int jc = min(max(j, 2), jm);
+    const double aue_jd = pc[min(max(j, 1), jm)];
+    const bool ucol = j >= 2 && j <= jm - 1;
+    const bool vcol = j >= 3 && j <= jm - 1;
+    const bool u_upw_col = upwind_all || j == 2 || j == jm - 1;
+    const bool out_col = j >= jlo && j < jhi;
+
+    Win<7, R> own;                      // own tau field, rows t-6..t
+    Win<6, double> aux;                 // u chain: p rows t-7..t-2; v chain: dp/dj rows t-6..t-2
+    Win<5, R> ie, ew, nuw, k1w, k2w, dd;   // per-cell data rows t-6..t-2 (dd: du or dv)
+    Win<5, R> bw;                       // buoyancy (v chain)
+    Win<5, R> w1, w2;                   // own stage 1 and 2 windows
+    Win<2, R> w3;                       // own stage 3
+    own.fill(R(0)); aux.fill(0.0);
+    ie.fill(R(1)); ew.fill(R(1)); k1w.fill(R(0)); k2w.fill(R(0)); bw.fill(R(0));
+    if (MUPOLY) nuw.fill(R(0));
+    if (!STEADY) dd.fill(R(0));
+    w1.fill(R(0)); w2.fill(R(0)); w3.fill(R(0));
+    unsigned long long mu_ = 0, mv_ = 0, mp_ = 0;
+
+    const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
+    R q_t, q_T, q_e, q_L, q_m;
+    double q_p;
+    auto fetch = [&](int t) {
+        q_t = R(0); q_T = R(1); q_e = R(1); q_L = R(0); q_m = R(0); q_p = 0.0;
+        if (jvalid && t >= 1) {
+            if (uchain) { if (t <= im + 1 && j <= jm) q_t = a.ut[(size_t)t * su + j]; }
+            else if (t <= im) q_t = a.vt[(size_t)t * sv + j];
+        }
+        const int rp = t - 2;
+        if (jvalid && rp >= 1 && rp <= im) {
+            if (j <= jm) {
+                const size_t k = (size_t)rp * su + j;
+                q_e = a.eps[k]; q_L = a.lp[k];
+                if (uchain) { q_p = a.p[k]; if (MUPOLY) q_T = a.T[k]; }
+                else q_T = a.T[k];
+                if (!STEADY && uchain) q_m = a.um[k];
+            }
+            if (!STEADY && !uchain) q_m = a.vm[(size_t)rp * sv + j];
+        }
+    };
+    __syncthreads();
+    fetch(ilo - 6);
+
+    for (int t = ilo - 6; t <= ihi + 6; ++t) {
+        const R t_new = q_t, T_own = q_T, e = q_e, Lp = q_L, m_rp = q_m;
+        const double p_rp = q_p;
+        fetch(t + 1);
+        const int rp = t - 2;
+        own.push(t_new);
+        if (uchain) {
+            sUT[s8(t)][PAD + x] = t_new;
+            sP[s2(rp)][PAD + x] = p_rp;
+        } else {
+            sVT[s16(t)][PAD + x] = t_new;
+            sT[s2(rp)][PAD + x] = T_own;
+        }
+        __syncthreads();
+
+        if (!chan) {
+            if (uchain) { fix_u(sU1[s3(t - 3)], x, j, jm); fix_u(sU2[s3(t - 5)], x, j, jm); }
+            else { fix_v(sV1[s3(t - 3)], x, j, jm); fix_v(sV2[s3(t - 5)], x, j, jm); }
+        }
+        {
+            const R T_rp = uchain ? (MUPOLY ? T_own : R(1)) : T_own;
+            R iev = R(1), k1 = R(0), k2 = R(0);
+            if (rp >= ilo - 4 && rp <= ihi + 4) {
+                iev = e == R(1) ? R(1) : R(1) / e;
+                if (Lp != R(0)) {
+                    k1 = e * Lp * R(a.inv_reda);
+                    k2 = Lp * R(a.cfsd) * sqrt(e);
+                }
+            }
+            ie.push(iev); ew.push(e); k1w.push(k1); k2w.push(k2);
+            if (MUPOLY) {
+                const R tt = T_rp;
+                nuw.push(e * inv_re * ((((R(a.mu[4]) * tt + R(a.mu[3])) * tt + R(a.mu[2])) * tt + R(a.mu[1])) * tt + R(a.mu[0])));
+            }
+            if (uchain) {
+                aux.push(p_rp);
+            } else {
+                const double pj = sP[s2(rp)][PAD + x];
+                aux.push((double)R(pj - sP[s2(rp)][PAD + x - 1]));
+                const R Tm = sT[s2(rp)][PAD + x - 1];
+                bw.push(bouss ? R(a.ri) * ((T_rp + Tm) * R(0.5) - R(a.t_ref)) : R(a.invfr2) * (R(1) - R(2) / (T_rp + Tm)));
+            }
+            if (!STEADY) dd.push(m_rp - own[2]);
+        }
+
+        if (uchain) {
+            const R haue_j = R(0.5 * pc[min(max(j, 1), jm)]);
+            const R aue_j = R(aue_jd);
+            const R rdy_j = R(pc[2 * Cc + jc]), rdy_n = R(pc[2 * Cc + min(jc + 1, jm)]);
+#define US_STAGE(K, WP, RING_C, WOUT, RING_OUT)                                                  \
+            {                                                                                     \
+                const int r = t - 2 * K;                                                          \
+                const int bk = 2 * K - 2;                                                         \
+                R val = R(0), rs = R(0);                                                          \
+                bool interior = false;                                                            \
+                if (r >= ilo - 2 * (3 - K) && r <= ihi + 2 * (3 - K)) {                           \
+                    if (r >= 3 && r <= im - 1) {                                                  \
+                        if (ucol) {                                                               \
+                            interior = true;                                                      \
+                            const R uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0]; \
+                            const R *c = &RING_C[PAD + x];                                        \
+                            const R uS = c[-1], uN = c[1];                                        \
+                            const R iev = ie[bk], ev = ew[bk];                                    \
+                            const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                          \
+                            const R aun_r = R(pr[r]), rdx_r = R(pr[2 * RR + r]);                  \
+                            const R cns = R(0.5) * aun_r * iev, cew = haue_j * iev;               \
+                            const R sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;       \
+                            const R vr = sVT[s16(r)][PAD + x], vr1 = sVT[s16(r - 1)][PAD + x];    \
+                            const R fn = (sVT[s16(r)][PAD + x + 1] + sVT[s16(r - 1)][PAD + x + 1]) * cns; \
+                            const R fs = (vr + vr1) * cns;                                        \
+                            const R fe = sE * cew, fw = sW * cew;                                 \
+                            const R diff = (nu * aue_j) * (R(pr[3 * RR + r]) * (uW - uP) + R(pr[3 * RR + r + 1]) * (uE - uP)) \
+                                         + (nu * aun_r) * (rdy_j * (uS - uP) + rdy_n * (uN - uP)); \
+                            R pw_, pe_, ps_, pn_;                                                 \
+                            if (u_upw_col) {                                                      \
+                                pw_ = fw > R(0) ? uW : uP; pe_ = fe > R(0) ? uP : uE;             \
+                                ps_ = fs > R(0) ? uS : uP; pn_ = fn > R(0) ? uP : uN;             \
+                            } else {                                                              \
+                                const R uSS = c[-2], uNN = c[2];                                  \
+                                const R cW = uWW - R(2) * uW + uP, cPi = uW - R(2) * uP + uE, cE = uP - R(2) * uE + uEE; \
+                                const R cS = uSS - R(2) * uS + uP, cPj = uS - R(2) * uP + uN, cN = uP - R(2) * uN + uNN; \
+                                pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE); \
+                                ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
+                            }                                                                     \
+                            const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;             \
+                            R Rr = rdx_r * (rdy_j * (diff + conv) - ev * R(aux[bk] - aux[bk + 1])) - gg * ev; \
+                            const R k1 = k1w[bk], k2 = k2w[bk];                                   \
+                            if (k1 != R(0) || k2 != R(0))                                         \
+                                Rr -= uP * (k1 + k2 * sqrt(uP * uP + vr * vr));                   \
+                            rs = ((STEADY ? R(0) : dd[bk]) + Rr * dt) * dtau;                     \
+                            const R tau = own[2 * K];                                             \
+                            if (K == 1) val = tau + rs;                                           \
+                            else if (K == 2) val = R(0.75) * tau + R(0.25) * (uP + rs);           \
+                            else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (uP + rs);             \
+                        }                                                                         \
+                    } else if (r <= 2) {                                                          \
+                        val = chan ? R(a.u_in) : R(0);                                            \
+                    } else {                                                                      \
+                        val = (chan && ucol) ? WOUT[0] : R(0);                                    \
+                    }                                                                             \
+                    RING_OUT                                                                      \
+                }                                                                                 \
+                WOUT.push(val);                                                                   \
+                if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                       \
+                    a.un[(size_t)r * su + j] = val;                                               \
+                    const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_;   \
+                }                                                                                 \
+            }
+            US_STAGE(1, own, sUT[s8(t - 2)], w1, { sU1[s3(t - 2)][PAD + x] = val; })
+            US_STAGE(2, w1, sU1[s3(t - 4)], w2, { sU2[s3(t - 4)][PAD + x] = val; })
+            US_STAGE(3, w2, sU2[s3(t - 6)], w3, {})
+#undef US_STAGE
+            // P at row r = t - 7 (equations.f90:620-658), FP64 accumulation, v3 from the v chain
+            const int r = t - 7;
+            if (r >= 2 && r <= im - 1 && ucol && out_col && r >= ilo && r < ihi) {
+                const double u3p = (double)w3[1], u3e = (double)w3[0];
+                const double v3p = (double)sV3[s2(r)][PAD + x];
+                const double v3n = (double)lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
+                const double avn_r = pr[RR + r];
+                const double rpv = -((u3e * aue_jd - u3p * aue_jd) + (v3n * avn_r - v3p * avn_r));
+                const double ddp = a.dtau * rpv * a.beta;
+                const double pp = aux[5];
+                double pi = pp + ddp;
+                pi = 0.75 * pp + 0.25 * (pi + ddp);
+                a.pn[(size_t)r * su + j] = 1.0 / 3.0 * pp + 2.0 / 3.0 * (pi + ddp);
+                const unsigned long long bb = dbits(ddp); mp_ = bb > mp_ ? bb : mp_;
+            }
+        } else {
+            const R ave_j = R(pc[Cc + jc]), have_j = R(0.5 * pc[Cc + jc]);
+            const R rdy_j = R(pc[2 * Cc + jc]);
+            const R rdym_j = R(pc[3 * Cc + jc]), rdym_n = R(pc[3 * Cc + min(jc + 1, jm + 1)]);
+#define VS_STAGE(K, WP, RING_C, WOUT, RING_OUT)                                                  \
+            {                                                                                     \
+                const int r = t - 2 * K;                                                          \
+                const int bk = 2 * K - 2;                                                         \
+                R val = R(0), rs = R(0);                                                          \
+                bool interior = false;                                                            \
+                if (r >= ilo - 2 * (3 - K) && r <= ihi - 1 + 2 * (3 - K)) {                       \
+                    if (j <= 2) {                                                                 \
+                        val = chan ? R(0) : vi;                                                   \
+                    } else if (r >= 2 && r <= im - 1) {                                           \
+                        if (vcol) {                                                               \
+                            interior = true;                                                      \
+                            const R vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0]; \
+                            const R *c = &RING_C[PAD + x];                                        \
+                            const R vS = c[-1], vN = c[1];                                        \
+                            const R iev = ie[bk], ev = ew[bk];                                    \
+                            const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                          \
+                            const R avn_r = R(pr[RR + r]), rdx_r = R(pr[2 * RR + r]);             \
+                            const R cns = R(0.5) * avn_r * iev, cew = have_j * iev;               \
+                            const R sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;       \
+                            const R fn = sN * cns, fs = sS * cns;                                 \
+                            const R ur = sUT[s8(r)][PAD + x];                                     \
+                            const R fe = (sUT[s8(r + 1)][PAD + x] + sUT[s8(r + 1)][PAD + x - 1]) * cew; \
+                            const R fw = (ur + sUT[s8(r)][PAD + x - 1]) * cew;                    \
+                            const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[2 * RR + r + 1]) * (vE - vP)) \
+                                         + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP)); \
+                            R pw_, pe_, ps_, pn_;                                                 \
+                            if (upwind_all || r == 2 || r == im - 1) {                            \
+                                pw_ = fw > R(0) ? vW : vP; pe_ = fe > R(0) ? vP : vE;             \
+                                ps_ = fs > R(0) ? vS : vP; pn_ = fn > R(0) ? vP : vN;             \
+                            } else {                                                              \
+                                const R vSS = c[-2], vNN = c[2];                                  \
+                                const R cW = vWW - R(2) * vW + vP, cPi = vW - R(2) * vP + vE, cE = vP - R(2) * vE + vEE; \
+                                const R cS = vSS - R(2) * vS + vP, cPj = vS - R(2) * vP + vN, cN = vP - R(2) * vN + vNN; \
+                                pw_ = quick_face(fw, sW, cW, cPi); pe_ = quick_face(fe, sE, cPi, cE); \
+                                ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
+                            }                                                                     \
+                            const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;             \
+                            R Rr = rdy_j * (rdx_r * (diff + conv) - ev * R(aux[bk])) + bw[bk];        \
+                            const R k1 = k1w[bk], k2 = k2w[bk];                                   \
+                            if (k1 != R(0) || k2 != R(0))                                         \
+                                Rr -= vP * (k1 + k2 * sqrt(ur * ur + vP * vP));                   \
+                            rs = ((STEADY ? R(0) : dd[bk]) + Rr * dt) * dtau;                     \
+                            const R tau = own[2 * K];                                             \
+                            if (K == 1) val = tau + rs;                                           \
+                            else if (K == 2) val = R(0.75) * tau + R(0.25) * (vP + rs);           \
+                            else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (vP + rs);             \
+                            if (r == 2 && !chan) WOUT.set(0, val);                                \
+                        }                                                                         \
+                    } else if (r == im) {                                                         \
+                        val = vcol ? WOUT[0] : R(0);                                              \
+                    }                                                                             \
+                    RING_OUT                                                                      \
+                }                                                                                 \
+                WOUT.push(val);                                                                   \
+                if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                       \
+                    a.vn[(size_t)r * sv + j] = val;                                               \
+                    const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_;   \
+                }                                                                                 \
+            }
+            VS_STAGE(1, own, sVT[s16(t - 2)], w1, { sV1[s3(t - 2)][PAD + x] = val; })
+            VS_STAGE(2, w1, sV1[s3(t - 4)], w2, { sV2[s3(t - 4)][PAD + x] = val; })
+            VS_STAGE(3, w2, sV2[s3(t - 6)], w3, { sV3[s2(t - 6)][PAD + x] = val; })
+#undef VS_STAGE
+        }
+    }
+    fold_max(a.res + 0, mu_);
+    fold_max(a.res + 1, mv_);
+    fold_max(a.res + 2, mp_);
+}
+
+template <typename R, bool STEADY, bool MUPOLY>
+__global__ __launch_bounds__(NT1S, 1) void k_uvp_split(const K1Args<R> a)
+{
+    __shared__ Smem1S<R> sm;
+    const int G = a.nbi * a.nbj, per = (G + 7) / 8;
+    const int b = blockIdx.x, L = (b % 8) * per + b / 8;
+    if (L >= G) return;
+    uvp_tile_split<R, STEADY, MUPOLY>(a, L / a.nbj, L % a.nbj, sm);
+}
+
+template <typename R, bool STEADY, bool MUPOLY>
+__global__ __launch_bounds__(NT1S, 1) void k_uvp_split_batch(const K1Args<R> *cases, const BatchTile *tiles,
+                                                             int ntiles, const int *ctl)
+{
+    __shared__ Smem1S<R> sm;
+    for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        const BatchTile tl = tiles[w];
+        const int cs = ctl[tl.c];
+        if (!(cs & 1)) continue;
+        const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
+        uvp_tile_split<R, STEADY, MUPOLY>(a, tl.bi, tl.bj, sm);
+        __syncthreads();
     }
 }
 
@@ -796,29 +1116,27 @@ void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack,
     k2.nbi = pick_rows(im - 2, k2.nbj, 16, a->hb2, &k2.hb);
 }
 
+// occ: 1 or 2 select k_uvp's register budget, 3 the split-chain kernel.
+template <typename R, bool STEADY, bool MUPOLY>
+void launch_k1_v(const K1Args<R> &k1, int occ, int grid1, hipStream_t s)
+{
+    const dim3 g1(grid1);
+    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY>), g1, dim3(NT1S), 0, s, k1);
+    else if (occ == 2) hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 2>), g1, dim3(NT1), 0, s, k1);
+    else hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 1>), g1, dim3(NT1), 0, s, k1);
+}
+
 template <typename R>
 void launch_k1(const K1Args<R> &k1, const cfd_params *P, int occ, hipStream_t s)
 {
     const int G1 = k1.nbi * k1.nbj, grid1 = ((G1 + 7) / 8) * 8;
     const bool mp = P->mu_model == CFD_MU_POLY4;
-    const dim3 g1(grid1), b1(NT1);
-    const int o1 = occ == 2 ? 2 : 1;
     if (P->steady) {
-        if (mp) {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, true, true, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<R, true, true, 1>), g1, b1, 0, s, k1);
-        } else {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, true, false, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<R, true, false, 1>), g1, b1, 0, s, k1);
-        }
+        if (mp) launch_k1_v<R, true, true>(k1, occ, grid1, s);
+        else launch_k1_v<R, true, false>(k1, occ, grid1, s);
     } else {
-        if (mp) {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, false, true, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<R, false, true, 1>), g1, b1, 0, s, k1);
-        } else {
-            if (o1 == 2) hipLaunchKernelGGL((k_uvp<R, false, false, 2>), g1, b1, 0, s, k1);
-            else hipLaunchKernelGGL((k_uvp<R, false, false, 1>), g1, b1, 0, s, k1);
-        }
+        if (mp) launch_k1_v<R, false, true>(k1, occ, grid1, s);
+        else launch_k1_v<R, false, false>(k1, occ, grid1, s);
     }
 }
 
@@ -1046,7 +1364,10 @@ void batch_launch_k1(const cfd_batch_launch *L, hipStream_t s)
     const dim3 g1(L->grid1), b1(NT1);
     const K1Args<R> *a = (const K1Args<R> *)L->k1;
     const BatchTile *t = (const BatchTile *)L->tiles1;
-    if (L->occ1 == 2) hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 2>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
+    if (L->occ1 == 3)
+        hipLaunchKernelGGL((k_uvp_split_batch<R, STEADY, MUPOLY>), g1, dim3(NT1S), 0, s, a, t, L->ntiles1, L->ctl);
+    else if (L->occ1 == 2)
+        hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 2>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
     else hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 1>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
 }
 
@@ -1165,8 +1486,9 @@ int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t o
     int b1 = 0, b2 = 0;
     hipError_t e1, e2;
 #define OCC1(R, S, M)                                                                                      \
-    (occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NT1, 0)       \
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NT1, 0))
+    (occ1 == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_split_batch<R, S, M>, NT1S, 0)   \
+     : occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NT1, 0)     \
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NT1, 0))
 #define OCC2(R, S)                                                                                         \
     (occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NT2, 0)         \
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NT2, 0))

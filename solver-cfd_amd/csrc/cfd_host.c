@@ -952,8 +952,8 @@ static int tiles_valid(const cfd_ctx *c, const cfd_tiles *t)
 {
     if (t->rows_uvp < 0 || t->rows_tc < 0 || t->rows_uvp > c->P.imax || t->rows_tc > c->P.imax)
         return fail(CFD_ERR_ARG, "row-block heights must lie in [0, imax], got %d, %d", t->rows_uvp, t->rows_tc);
-    if (!(t->waves_uvp == 0 || t->waves_uvp == 1 || t->waves_uvp == 2))
-        return fail(CFD_ERR_ARG, "waves_uvp must be 0, 1 or 2, got %d", t->waves_uvp);
+    if (!(t->waves_uvp >= 0 && t->waves_uvp <= 3))
+        return fail(CFD_ERR_ARG, "waves_uvp must be 0, 1, 2 or 3, got %d", t->waves_uvp);
     if (!(t->waves_tc == 0 || t->waves_tc == 1 || t->waves_tc == 4))
         return fail(CFD_ERR_ARG, "waves_tc must be 0, 1 or 4, got %d", t->waves_tc);
     return CFD_OK;
@@ -998,11 +998,13 @@ static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_
 
 /* Candidate shapes. The u/v/p kernel holds ~300 registers per lane: one wave per SIMD
  * without spills, or two with the excess spilled to scratch (then more, shorter row blocks
- * fill the 2 x 256 wave slots). The T/C kernel runs 3 waves per SIMD as compiled, or 4 with
+ * fill the 2 x 256 wave slots); variant 3 runs the u and v chains in separate waves of a
+ * 512-lane workgroup, each with half the register windows. The T/C kernel runs 3 waves per SIMD as compiled, or 4 with
  * spills. */
 static const cfd_tiles CAND_UVP[] = {
     {0, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0}, {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
     {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
+    {0, 0, 3, 0}, {16, 0, 3, 0}, {24, 0, 3, 0}, {32, 0, 3, 0}, {48, 0, 3, 0}, {64, 0, 3, 0}, {96, 0, 3, 0},
 };
 static const cfd_tiles CAND_TC[] = {
     {0, 0, 0, 1}, {0, 8, 0, 1}, {0, 12, 0, 1}, {0, 16, 0, 1}, {0, 24, 0, 1}, {0, 32, 0, 1}, {0, 64, 0, 1},

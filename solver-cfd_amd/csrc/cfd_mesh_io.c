@@ -296,3 +296,45 @@ int cfd_read_restart(const char *dir, int32_t imax, int32_t jmax, int32_t order,
     free(a);
     return CFD_OK;
 }
+
+/* ParaView file of transient.f90:55-103: structured grid of the nodes (x(i), y(j), 0), point
+ * data |u| (nodal velocity magnitude), T, P, and the vector (u, v, 0); i outer, j inner,
+ * fixed-point numbers with 4 decimals in 14 columns. The header counts use the reference's
+ * widths (I3 for the dimensions, I6 for the point counts) and are written in full where they
+ * do not fit (the reference's format would print asterisks there). */
+int cfd_write_vtk(const char *path, const char *title, int32_t imax, int32_t jmax, const double *x, const double *y,
+                  const double *u, const double *v, const double *p, const double *T)
+{
+    if (!path || !x || !y || !u || !v || !p || !T || imax < 1 || jmax < 1)
+        return cfd__fail(CFD_ERR_ARG, "cfd_write_vtk: bad arguments");
+    FILE *f = fopen(path, "w");
+    if (!f) return cfd__fail(CFD_ERR_ARG, "cannot open %s: %s", path, strerror(errno));
+    const size_t ld = (size_t)jmax + 1;
+    const long n = (long)imax * jmax;
+    int bad = fprintf(f, "# vtk DataFile Version 3.0\n%s\nASCII\nDATASET STRUCTURED_GRID\n",
+                      title ? title : "Droplet Combustion") < 0;
+    bad = bad || fprintf(f, "DIMENSIONS %3d %3d 1\nPOINTS %6ld float\n", (int)jmax, (int)imax, n) < 0;
+    for (int i = 1; i <= imax && !bad; ++i)
+        for (int j = 1; j <= jmax && !bad; ++j) bad = fprintf(f, "%14.4f%14.4f%14.4f\n", x[i], y[j], 0.0) < 0;
+    bad = bad || fprintf(f, "POINT_DATA %6ld\nSCALARS VEL_MAGNITUDE float\nLOOKUP_TABLE default\n", n) < 0;
+    for (int i = 1; i <= imax && !bad; ++i)
+        for (int j = 1; j <= jmax && !bad; ++j) {
+            const size_t k = i * ld + j;
+            bad = fprintf(f, "%14.4f\n", sqrt(u[k] * u[k] + v[k] * v[k])) < 0;
+        }
+    const double *sc[2] = {T, p};
+    const char *nm[2] = {"T", "P"};
+    for (int s = 0; s < 2 && !bad; ++s) {
+        bad = fprintf(f, "SCALARS %s float\nLOOKUP_TABLE default\n", nm[s]) < 0;
+        for (int i = 1; i <= imax && !bad; ++i)
+            for (int j = 1; j <= jmax && !bad; ++j) bad = fprintf(f, "%14.4f\n", sc[s][i * ld + j]) < 0;
+    }
+    bad = bad || fprintf(f, "\n\nVECTORS Vectors float\n") < 0;
+    for (int i = 1; i <= imax && !bad; ++i)
+        for (int j = 1; j <= jmax && !bad; ++j) {
+            const size_t k = i * ld + j;
+            bad = fprintf(f, "%14.4f%14.4f%14.4f\n", u[k], v[k], 0.0) < 0;
+        }
+    if (fclose(f) != 0) bad = 1;
+    return bad ? cfd__fail(CFD_ERR_ARG, "write error on %s", path) : CFD_OK;
+}

@@ -267,7 +267,25 @@ def case_refout51x84():
     print("wrote refout51x84")
 
 
-CASES = dict(refout51x84=case_refout51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
+def case_refvtk51x84():
+    """The reference's shipped ParaView file OpenMp_Multicore/data/paraview_output100.vtk, written
+    by transient.f90 from the same final state as data/output_variables.dat: the nodal fields
+    of that state (inputs) and the file's SHA-256, size and every 500th line (expected)."""
+    import hashlib
+    base = os.path.join(os.environ.get("CFD_REFERENCE_DIR", "/root/reference"), "OpenMp_Multicore", "data")
+    d = np.array([float(t) for t in open(os.path.join(base, "output_variables.dat")).read().split()]).reshape(-1, 7)
+    raw = open(os.path.join(base, "paraview_output100.vtk"), "rb").read()
+    im, jm = 51, 84
+    lines = raw.decode().split("\n")
+    np.savez_compressed(os.path.join(OUT, "refvtk51x84.npz"), x=d[::jm, 0], y=d[:jm, 1],
+                        u=d[:, 2].reshape(im, jm), v=d[:, 3].reshape(im, jm), p=d[:, 4].reshape(im, jm),
+                        T=d[:, 5].reshape(im, jm), sha256=np.array(hashlib.sha256(raw).hexdigest()),
+                        size=np.array(len(raw)), line_no=np.arange(0, len(lines), 500),
+                        lines=np.array(lines[::500]))
+    print("wrote refvtk51x84")
+
+
+CASES = dict(refout51x84=case_refout51x84, refvtk51x84=case_refvtk51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
              phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 
 if __name__ == "__main__":

@@ -141,16 +141,18 @@ def _make_batch(cases, prec):
     return b
 
 
-@pytest.mark.parametrize("prec,steady", [(PREC_FP64, 1), (PREC_FP64, 0), (PREC_FP32, 1)])
-def test_batch_matches_single_contexts_bitwise(prec, steady):
-    say("case: batch vs contexts", "fp32" if prec == PREC_FP32 else "fp64", "steady", steady)
+@pytest.mark.parametrize("prec,steady,waves", [(PREC_FP64, 1, 1), (PREC_FP64, 0, 1), (PREC_FP32, 1, 1),
+                                               (PREC_FP64, 1, 3), (PREC_FP32, 0, 3)])
+def test_batch_matches_single_contexts_bitwise(prec, steady, waves):
+    say("case: batch vs contexts", "fp32" if prec == PREC_FP32 else "fp64", "steady", steady, "waves", waves)
     cases = _bc_consistent(prec, steady)
     b = _make_batch(cases, prec)
-    b.set_tiles(24, 12, 1, 1)
+    b.set_tiles(24, 12, waves, 1)
     code, iters, res, status = b.run(0.0, 25, steps_per_graph=10)
     say("batch", code, iters, status)
     assert code == 0 and list(iters) == [25] * len(cases) and list(status) == [4] * len(cases)
     for i, (prm, x, y, eps, lp, st1, ctx) in enumerate(cases):
+        ctx.set_tiles(24, 12, waves, 1)
         r = ctx.pseudo_steps(25)
         for k in FIELDS:
             a, c = b.download(i, k), ctx.download(k)
@@ -218,9 +220,10 @@ def test_batch_autotune_restores_state():
     for i in range(len(cases)):
         for k in FIELDS:
             assert np.array_equal(b.download(i, k), before[i][k]), (i, k)
-    # and the tuned shape still reproduces the contexts
+    # and the tuned shape still reproduces the contexts run with the same kernel variants
     code, iters, res, status = b.run(0.0, 8, steps_per_graph=8)
     for i, (prm, x, y, eps, lp, st1, ctx) in enumerate(cases):
+        ctx.set_tiles(*tiles)
         ctx.pseudo_steps(8)
         for k in FIELDS:
             assert np.array_equal(b.download(i, k), ctx.download(k)), (i, k)

@@ -13,7 +13,8 @@ import pytest
 import oracle as O
 from cfd_amd import lib as L
 from cfd_amd import cases as C
-from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4)
+from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4, PREC_FP32,
+                             PREC_FP64)
 from helpers import load_golden, max_rel, field_rel
 
 pytestmark = pytest.mark.gpu
@@ -48,11 +49,13 @@ def test_fused_trajectory_matches_reference(name):
     ctx.close()
 
 
-def _vs_oracle(prm, x, y, eps, lp, st, n):
+def _vs_oracle(prm, x, y, eps, lp, st, n, tiles=None):
     o = O.Oracle(prm, x, y, eps, lp)
     s, cs = o.make_state(st)
     h_ref = o.pseudo_steps(s, cs, n)
     ctx = L.Context(prm, x, y, eps, lp)
+    if tiles is not None:
+        ctx.set_tiles(*tiles)
     ctx.set_state(st)
     res, hist = ctx.pseudo_steps(n, history=True)
     nf, ne = ctx.step_counts()
@@ -86,11 +89,16 @@ def test_fused_multiblock_reference_bcs(shape):
     assert nf == 59 and h <= TOL and max(errs.values()) <= TOL
 
 
-@pytest.mark.parametrize("variant", ["channel", "upwind", "mu_bouss_slot", "steady"])
+@pytest.mark.parametrize("variant", ["channel", "upwind", "mu_bouss_slot", "steady", "darcy_only",
+                                     "darcy_only_split"])
 def test_fused_multiblock_channel_variants(variant):
+    """Channel BC set with the model switches; darcy_only (Cf = 0: porous cells keep the
+    linear Darcy drag without the Forchheimer term) on both u/v/p kernel forms."""
     say("case: multiblock channel", variant)
     kw = dict()
-    if variant == "upwind":
+    if variant.startswith("darcy_only"):
+        kw = dict(cf=0.0, darcy=2e-3)
+    elif variant == "upwind":
         kw = dict(scheme=SCHEME_UPWIND)
     elif variant == "mu_bouss_slot":
         kw = dict(buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, slot_lo=0.4, slot_hi=0.6, slot_value=1.0,
@@ -98,7 +106,7 @@ def test_fused_multiblock_channel_variants(variant):
     elif variant == "steady":
         kw = dict(steady=1, buoyancy=BUOY_BOUSSINESQ, ri=1.0, t_ref=0.5)
     prm, x, y, eps, lp, st = _multiblock_case(140, 520, bc=BC_CHANNEL, invfr2=0.0, **kw)
-    h, errs, nf = _vs_oracle(prm, x, y, eps, lp, st, 60)
+    h, errs, nf = _vs_oracle(prm, x, y, eps, lp, st, 60, tiles=(24, 12, 3, 1) if variant.endswith("split") else None)
     say(variant, h, errs, nf)
     assert nf == 59 and h <= TOL and max(errs.values()) <= TOL
 
@@ -147,7 +155,8 @@ def test_graph_replays_match_eager():
         c.close()
 
 
-@pytest.mark.parametrize("tiles", [(16, 8, 1, 1), (37, 11, 2, 4), (64, 64, 2, 1), (24, 5, 1, 4)])
+@pytest.mark.parametrize("tiles", [(16, 8, 1, 1), (37, 11, 2, 4), (64, 64, 2, 1), (24, 5, 1, 4), (29, 8, 3, 1),
+                                   (0, 0, 3, 4)])
 def test_fused_explicit_tiles_match_oracle(tiles):
     """Row-block heights other than the built-in choice (including ones that do not divide
     the row count) and both register-budget variants of each kernel give the same
@@ -189,6 +198,39 @@ def test_fused_two_wave_variants_channel_mu():
     ctx.close()
 
 
+@pytest.mark.parametrize("variant", ["reference_bc", "channel_mu_bouss", "unsteady_upwind"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_split_chain_kernel_matches_one_wave_kernel(variant, prec):
+    """The split-chain u/v/p kernel (u and v chains in separate waves) evaluates the same
+    expressions as the one-wave kernel; with FMA contraction the compiler may pair products
+    and sums differently in the two kernels, so trajectories agree to rounding (a few ulps
+    after 25 steps), far inside the oracle bar that both meet."""
+    say("case: split-chain kernel", variant, "fp32" if prec == PREC_FP32 else "fp64")
+    if variant == "reference_bc":
+        prm, x, y, eps, lp, st = _multiblock_case(170, 530, bc=0, invfr2=0.7, g=1.5)
+    elif variant == "channel_mu_bouss":
+        prm, x, y, eps, lp, st = _multiblock_case(
+            140, 520, bc=BC_CHANNEL, invfr2=0.0, buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, slot_lo=0.4,
+            slot_hi=0.6, slot_value=1.0, mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3), t_bottom=1.0,
+            t_top=0.0)
+    else:
+        prm, x, y, eps, lp, st = _multiblock_case(90, 300, bc=BC_CHANNEL, invfr2=0.0, scheme=SCHEME_UPWIND, steady=0)
+    out = []
+    for waves in (1, 3):
+        ctx = L.Context(prm, x, y, eps, lp, precision=prec)
+        ctx.set_tiles(24, 12, waves, 1)
+        ctx.set_state(st)
+        res, hist = ctx.pseudo_steps(25, history=True)
+        assert ctx.step_counts() == (24, 1)
+        out.append((hist, {k: ctx.download(k) for k in FIELDS}))
+        ctx.close()
+    (h1, f1), (h3, f3) = out
+    errs = {k: max_rel(f3[k], f1[k]) for k in FIELDS}
+    say("split vs one-wave", errs, max_rel(h3, h1))
+    tol_f, tol_h = (1e-13, 1e-11) if prec == PREC_FP64 else (1e-5, 1e-3)
+    assert max(errs.values()) <= tol_f and max_rel(h3, h1) <= tol_h
+
+
 def test_autotune_restores_state_and_keeps_parity():
     """The autotuner leaves the state, the step counters and the first-step semantics as they
     were, picks a shape from its candidate set, and later contexts of the grid reuse it."""
@@ -204,7 +246,7 @@ def test_autotune_restores_state_and_keeps_parity():
     say("tuned", tiles, ms)
     rows_uvp, rows_tc, waves_uvp, waves_tc = tiles
     assert rows_uvp in (0, 16, 24, 32, 40, 48, 64, 96, 128) and rows_tc in (0, 8, 12, 16, 24, 32, 64)
-    assert waves_uvp in (1, 2) and waves_tc in (1, 4)
+    assert waves_uvp in (1, 2, 3) and waves_tc in (1, 4)
     assert ms > 0 and ctx.get_tiles() == tiles
     assert ctx.step_counts() == (0, 0)
     for k in FIELDS:

@@ -1,7 +1,8 @@
 """CPU: the producers and consumers either side of the pseudo-time iteration, in the C
-library — reference mesh, porous-block flags, nodal velocities, state table and restart
-files — against the flang-built reference mesh routine (mesh51), the reference's own shipped
-final-state file (refout51x84) and closed forms."""
+library — reference mesh, porous-block flags, nodal velocities, state table, restart and
+ParaView files — against the flang-built reference mesh routine (mesh51), the reference's own
+shipped final-state and ParaView files (refout51x84, refvtk51x84) and closed forms."""
+import math
 import os
 
 import numpy as np
@@ -116,3 +117,51 @@ def test_nodal_velocity_closed_form():
     I = np.arange(1, im + 1)[:, None]; J = np.arange(1, jm + 1)[None, :]
     assert np.array_equal(u[1:, 1:], 2.0 * (I + 0.5) + 0.5 * J)
     assert np.array_equal(v[1:, 1:], -1.0 * I + 3.0 * (J + 0.5))
+
+
+def test_vtk_writer_reproduces_reference_paraview_file(tmp_path):
+    """transient.f90:55-103 from the reference's final state: the file written from the nodal
+    fields of data/output_variables.dat is byte for byte the reference's shipped
+    data/paraview_output100.vtk (SHA-256, size and sampled lines)."""
+    import hashlib
+    g = np.load(os.path.join(GOLDEN, "refvtk51x84.npz"), allow_pickle=False)
+    im, jm = g["u"].shape
+    x = np.concatenate([[0.0], g["x"]]); y = np.concatenate([[0.0], g["y"]])
+
+    def pl(a):
+        out = np.zeros((im + 1, jm + 1)); out[1:, 1:] = a
+        return out
+    path = str(tmp_path / "paraview_output100.vtk")
+    L.write_vtk(path, x, y, pl(g["u"]), pl(g["v"]), pl(g["p"]), pl(g["T"]))
+    raw = open(path, "rb").read()
+    lines = raw.decode().split("\n")
+    for k, line in zip(g["line_no"], g["lines"]):
+        assert lines[int(k)] == str(line), (int(k), lines[int(k)], str(line))
+    assert len(raw) == int(g["size"])
+    assert hashlib.sha256(raw).hexdigest() == str(g["sha256"])
+
+
+def test_vtk_writer_layout_small_grid(tmp_path):
+    """Order (i outer, j inner), sections and the velocity magnitude on a 2 x 3 grid."""
+    x = np.array([0.0, -1.0, 2.5]); y = np.array([0.0, 0.0, 0.25, 1.0])
+    u = np.zeros((3, 4)); v = np.zeros((3, 4)); p = np.zeros((3, 4)); T = np.zeros((3, 4))
+    u[1:, 1:] = [[3.0, 0.0, -1.0], [0.5, 0.25, 2.0]]
+    v[1:, 1:] = [[4.0, 1.0, 0.0], [0.0, 0.0, -2.0]]
+    p[1:, 1:] = [[1, 2, 3], [4, 5, 6]]
+    T[1:, 1:] = [[0.1, 0.2, 0.3], [0.4, 0.5, 0.6]]
+    path = str(tmp_path / "s.vtk")
+    L.write_vtk(path, x, y, u, v, p, T, title="small")
+    lines = open(path).read().split("\n")
+    assert lines[1] == "small" and lines[4] == "DIMENSIONS   3   2 1" and lines[5] == "POINTS      6 float"
+    pts = [tuple(float(t) for t in s.split()) for s in lines[6:12]]
+    assert pts == [(-1.0, 0.0, 0.0), (-1.0, 0.25, 0.0), (-1.0, 1.0, 0.0), (2.5, 0.0, 0.0), (2.5, 0.25, 0.0),
+                   (2.5, 1.0, 0.0)]
+    assert lines[12] == "POINT_DATA      6" and lines[13] == "SCALARS VEL_MAGNITUDE float"
+    mag = [float(s) for s in lines[15:21]]
+    assert mag == [5.0, 1.0, 1.0, 0.5, 0.25, round(math.sqrt(8.0), 4)]
+    assert lines[21] == "SCALARS T float" and [float(s) for s in lines[23:29]] == [0.1, 0.2, 0.3, 0.4, 0.5, 0.6]
+    assert lines[29] == "SCALARS P float" and [float(s) for s in lines[31:37]] == [1, 2, 3, 4, 5, 6]
+    assert lines[37:40] == ["", "", "VECTORS Vectors float"]
+    vec = [tuple(float(t) for t in s.split()) for s in lines[40:46]]
+    assert vec[0] == (3.0, 4.0, 0.0) and vec[5] == (2.0, -2.0, 0.0)
+    assert lines[46:] == [""]
