@@ -216,18 +216,95 @@ def bench_warmstart(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_fp32_steady(args):
+    """FP32 storage against FP64 on steady fields: the same seeded steady cases solved by the
+    FP64 batch to a tight tolerance, by the FP32 batch alone (stopped at its residual floor or
+    the step cap), and by the FP32 batch with the FP64 finish (hand-over below 5e-7)."""
+    from cfd_amd import warmstart as WS
+    n, H, itc = args.acc_cases, args.ws_size, args.acc_itc
+    cs = WS.steady_cases(n, seed=7, size=H, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
+    log("FP32 vs FP64 steady fields: %d cases on %dx%d" % (n, H, H))
+    b64 = WS.build_batch(cs, PREC_FP64)
+    (_, it64, _, st64), ms64 = b64.run_sliced(args.acc_tol, itc, log=log)
+    b32 = WS.build_batch(cs, PREC_FP32)
+    (_, it32, res32, st32), ms32 = b32.run_sliced(2.5e-7, itc, log=log)
+    hand = WS.solve(cs, args.acc_tol, itc, log=log)
+    fields = ("um_tau", "vm_tau", "p", "T_tau")
+    ok = np.nonzero((st64 == 1) & (hand["status"] == 1))[0]
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+    e32 = np.array([[rel(b32.download(int(i), f), b64.download(int(i), f)) for f in fields] for i in ok])
+    eh = np.array([[rel(hand["batch"].download(int(i), f), b64.download(int(i), f)) for f in fields] for i in ok])
+    worst32 = e32.max(axis=1) if len(ok) else np.zeros(0)
+    out = {
+        "metric": "FP32-storage steady fields vs FP64 (max relative difference per case, worst field)",
+        "value": float(np.median(worst32)) if len(ok) else None, "unit": "relative", "n_gpus": 1,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64 vs f64",
+        "data": "synthetic (seeded steady channel cases with porous blocks)",
+        "config": {"workload": "steady cases, %dx%d grids" % (H, H), "cases": n, "fp64_tol": args.acc_tol,
+                   "itc_max": itc},
+        "cases_compared": int(len(ok)),
+        "fp32_alone": {"median": float(np.median(worst32)) if len(ok) else None,
+                       "max": float(worst32.max()) if len(ok) else None,
+                       "within_1e-5": int((worst32 <= 1e-5).sum()),
+                       "per_field_max": dict(zip(fields, [float(v) for v in e32.max(axis=0)])) if len(ok) else None,
+                       "final_residual_median": float(np.median(np.max(res32, axis=1)))},
+        "fp32_then_fp64": {"max": float(eh.max()) if len(ok) else None,
+                           "steps_fp32_median": float(np.median(hand["iters32"][ok])) if len(ok) else None,
+                           "steps_fp64_median": float(np.median(hand["iters64"][ok])) if len(ok) else None,
+                           "seconds": hand["seconds"]},
+        "fp64_alone": {"steps_median": float(np.median(it64[ok])) if len(ok) else None, "seconds": ms64 * 1e-3},
+    }
+    for b in (b64, b32, hand["batch"]):
+        b.close()
+    print(json.dumps(out), flush=True)
+
+
+# resolution divisor of each U-Net convolution (the layer table is L.UNET_LAYERS)
+UNET_RES = [1, 1, 2, 2, 4, 4, 8, 8, 4, 4, 2, 2, 1, 1, 1]
+
+
 def unet_flops(H, W, B):
     """Multiply-add FLOPs (x2) of one forward pass of the U-Net on B H x W inputs."""
-    from oracle import unet_ref as U       # layer table only (channel counts)
-    spec = U.SPEC
-    res = [1, 1, 2, 2, 4, 4, 8, 8, 4, 4, 2, 2, 1, 1, 1]   # downsampling factor of each conv
-    return sum(2.0 * B * (H // r) * (W // r) * 9 * cin * cout for (cin, cout, _), r in zip(spec, res))
+    return sum(2.0 * B * (H // r) * (W // r) * 9 * cin * cout for (cin, cout, _), r in zip(L.UNET_LAYERS, UNET_RES))
+
+
+def conv_kernel_ms(H, W, B, cin, cout, reps=20):
+    """Mean device time (ms) of the forward and weight-gradient convolution kernels of one
+    layer on device buffers, between HIP events on the NULL stream, which the library's nn
+    operations use."""
+    from cfd_amd.hipmem import DeviceArray, time_null_stream
+    rng = np.random.default_rng(3)
+
+    def bf16_bits(a):
+        return (np.asarray(a, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    x = DeviceArray(bf16_bits(rng.standard_normal((B, H, W, cin))))
+    w = DeviceArray(bf16_bits(rng.standard_normal((cout, 9 * cin)) / np.sqrt(9 * cin)))
+    bias = DeviceArray(np.zeros(cout, np.float32))
+    y = DeviceArray(np.zeros((B, H, W, cout), np.uint16))
+    dy = DeviceArray(bf16_bits(rng.standard_normal((B, H, W, cout))))
+    dw = DeviceArray(np.zeros((cout, 9 * cin), np.float32))
+    lib = L.load()
+
+    def fwd():
+        L.check(lib.cfd_nn_conv3x3(x.ptr, B, H, W, cin, w.ptr, bias.ptr, cout, 1, y.ptr, None))
+
+    def wgrad():
+        L.check(lib.cfd_nn_conv3x3_wgrad(x.ptr, dy.ptr, B, H, W, cin, cout, dw.ptr))
+    fwd(); wgrad()
+    out = time_null_stream(fwd, reps), time_null_stream(wgrad, reps)
+    for a in (x, w, bias, y, dy, dw):
+        a.free()
+    return out
 
 
 def bench_unet(args):
     """U-Net training throughput with the library's kernels: Adam steps on a synthetic batch.
     One step = forward + input gradients + weight gradients (~3x the forward FLOPs), priced
-    against MI355X's dense BF16 MFMA peak."""
+    against MI355X's dense BF16 MFMA peak on the device time of the step (HIP events around
+    the forward, backward and update; the host-to-device copies of the batch are reported
+    separately as the wall time)."""
     H = W = args.ws_size
     B = 16
     net = L.UNet(H, W, B, seed=1)
@@ -237,19 +314,34 @@ def bench_unet(args):
     for _ in range(5):
         net.train_step(x, t, 1e-4)
     n = max(args.steps // 4, 20)
+    dev = np.zeros((n, 3))
     t0 = time.perf_counter()
-    for _ in range(n):
+    for k in range(n):
         net.train_step(x, t, 1e-4)
-    dt = (time.perf_counter() - t0) / n
-    fl = 3.0 * unet_flops(H, W, B)
+        dev[k] = net.last_timing()
+    wall = (time.perf_counter() - t0) / n
     net.close()
-    out = {"metric": "U-Net training step throughput", "value": fl / dt / 1e12, "unit": "TFLOP/s", "n_gpus": 1,
-           "steps": n, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+    dev_ms = float(dev.sum(axis=1).mean())
+    fl = 3.0 * unet_flops(H, W, B)
+    # the kernels that dominate the step: forward and weight gradient of the widest
+    # full-resolution layer (concat 96 -> 32 channels)
+    cin, cout = 96, 32
+    f_ms, w_ms = conv_kernel_ms(H, W, B, cin, cout)
+    lf = 2.0 * B * H * W * 9 * cin * cout
+    kern = {"layer": "%d->%d at %dx%d, batch %d" % (cin, cout, H, W, B), "flops_per_launch": lf,
+            "forward_ms": f_ms, "forward_tflops": lf / f_ms / 1e9, "wgrad_ms": w_ms, "wgrad_tflops": lf / w_ms / 1e9}
+    dom = "wgrad" if w_ms >= f_ms else "forward"
+    ach = kern[dom + "_tflops"]
+    out = {"metric": "U-Net training step throughput", "value": fl / dev_ms / 1e9, "unit": "TFLOP/s", "n_gpus": 1,
+           "steps": n, "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
            "config": {"workload": "U-Net 32/64/128/256, %dx%d, batch %d" % (H, W, B)},
-           "roofline": {"bound": "mfma", "achieved": fl / dt / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
-                        "frac": fl / dt / 1e12 / 2500.0, "traffic": None},
-           "note": "whole-step time including host-to-device batch copies and the elementwise kernels"}
+           "device_ms_forward_backward_update": [float(v) for v in dev.mean(axis=0)],
+           "wall_ms_per_step": wall * 1e3,
+           "roofline": {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0,
+                        "traffic": None, "kernel": "k_conv3x3_wgrad_t" if dom == "wgrad" else "k_conv3x3_t",
+                        "flops_per_launch": lf},
+           "kernels": kern}
     print(json.dumps(out), flush=True)
 
 
@@ -274,13 +366,16 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
-    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart", "unet"],
+    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart", "unet", "fp32steady"],
                     help="grid: one large grid (the cell-update rate); batch: time to residual of the "
                          "256-case mixed batch against the plain FP64 path")
     ap.add_argument("--batch-tol", type=float, default=1e-6)
     ap.add_argument("--batch-itc", type=int, default=50000)
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--batch-beta", type=float, default=200.0)
+    ap.add_argument("--acc-cases", type=int, default=64)
+    ap.add_argument("--acc-tol", type=float, default=1e-10)
+    ap.add_argument("--acc-itc", type=int, default=300000)
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
     ap.add_argument("--ws-size", type=int, default=64)
@@ -294,6 +389,8 @@ def main():
         return bench_warmstart(args)
     if args.workload == "unet":
         return bench_unet(args)
+    if args.workload == "fp32steady":
+        return bench_fp32_steady(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the pseudo-time "

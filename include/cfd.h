@@ -201,6 +201,13 @@ int  cfd_ctx_last_timing(const cfd_ctx *ctx, double *ms_per_step);
  * single-pass kernel). Mainly for tests that pin both against each other. */
 int  cfd_ctx_set_kernel(cfd_ctx *ctx, int32_t kernel);
 
+/* Two-stream pipeline of the fused path (default on for contexts from cfd_ctx_create):
+ * for unsteady problems the T/C kernel of pseudo-step n runs on a second stream while the
+ * u/v/p kernel of step n+1 runs, since the latter reads only u, v, p and the physical-level
+ * T. Results are identical either way; every call that steps joins the streams before it
+ * returns. Off = one stream. Steady problems always use one stream. */
+int  cfd_ctx_set_pipeline(cfd_ctx *ctx, int32_t on);
+
 /* Shapes of the fused kernels. Zeros select the built-in choice (one row block per CU,
  * no register-budget override). */
 typedef struct cfd_tiles {
@@ -347,6 +354,10 @@ int  cfd_unet_predict(cfd_unet *net, const float *x, int32_t batch, float *y);
 int  cfd_unet_loss_grad(cfd_unet *net, const float *x, const float *target, int32_t batch, double *loss, float *grad);
 /* One Adam step (beta 0.9, 0.999, eps 1e-8) on the batch; *loss before the update. */
 int  cfd_unet_train_step(cfd_unet *net, const float *x, const float *target, int32_t batch, float lr, double *loss);
+/* Device time of the last cfd_unet_train_step from HIP events on the NULL stream:
+ * ms[0] forward (with the loss), ms[1] backward, ms[2] Adam update and weight refresh
+ * (host-to-device input copies excluded). Waits for that step. */
+int  cfd_unet_last_timing(cfd_unet *net, double ms[3]);
 
 /* The U-Net's building blocks on device pointers (BF16 as uint16 bit patterns, NHWC,
  * NULL stream): 3x3 convolution (stride 1, zero padding 1; weights [Cout][9 Cin]) with

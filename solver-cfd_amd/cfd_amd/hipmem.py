@@ -1,5 +1,6 @@
-"""Minimal device-memory helper over libamdhip64 (ctypes) for tests of the device-pointer
-entry points. No torch: the tests drive the C-ABI exactly as a C caller would."""
+"""Minimal device-memory and event helper over libamdhip64 (ctypes), the HIP runtime the
+library itself links, for callers of the device-pointer entry points (the cfd_nn_* operations,
+the legacy functions): tests and the benchmark drive the C-ABI exactly as a C caller would."""
 import ctypes
 
 import numpy as np
@@ -16,6 +17,11 @@ def hip():
         L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         L.hipDeviceSynchronize.argtypes = []
         L.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        L.hipEventDestroy.argtypes = [ctypes.c_void_p]
         _hip = L
     return _hip
 
@@ -62,3 +68,22 @@ def device_count():
     if hip().hipGetDeviceCount(ctypes.byref(n)) != 0:
         return 0
     return n.value
+
+
+def time_null_stream(f, reps):
+    """Mean device ms of f() (work issued on the NULL stream) over `reps` calls, between two
+    HIP events recorded on the NULL stream."""
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip().hipEventCreate(ctypes.byref(e0)) == 0 and hip().hipEventCreate(ctypes.byref(e1)) == 0
+    try:
+        assert hip().hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            f()
+        assert hip().hipEventRecord(e1, None) == 0
+        assert hip().hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float(0)
+        assert hip().hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return ms.value / reps
+    finally:
+        hip().hipEventDestroy(e0)
+        hip().hipEventDestroy(e1)

@@ -23,13 +23,13 @@ EXPORTS_CFD = [
     "cfd_params_default", "cfd_mesh_1d", "cfd_geometry_derive", "cfd_ctx_create", "cfd_ctx_destroy",
     "cfd_ctx_get_params", "cfd_ctx_set_params", "cfd_ctx_set_mesh", "cfd_ctx_upload", "cfd_ctx_download",
     "cfd_ctx_device_ptr", "cfd_ctx_sync_levels", "cfd_pseudo_steps", "cfd_run_to_tol", "cfd_physical_step",
-    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_ctx_step_counts",
+    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_ctx_set_pipeline", "cfd_ctx_step_counts",
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
     "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_mesh_reference", "cfd_porous_flags_reference",
     "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_num_params", "cfd_unet_get_params", "cfd_unet_set_params",
-    "cfd_unet_predict", "cfd_unet_loss_grad", "cfd_unet_train_step", "cfd_nn_conv3x3", "cfd_nn_conv3x3_wgrad",
+    "cfd_unet_predict", "cfd_unet_loss_grad", "cfd_unet_train_step", "cfd_unet_last_timing", "cfd_nn_conv3x3", "cfd_nn_conv3x3_wgrad",
     "cfd_nn_flip_weights", "cfd_nn_maxpool2", "cfd_nn_maxpool2_bwd", "cfd_nn_upsample2", "cfd_nn_upsample2_bwd",
     "cfd_nodal_velocity", "cfd_write_state_table", "cfd_read_state_table", "cfd_write_restart", "cfd_read_restart",
     "cfd_write_vtk",
@@ -40,6 +40,13 @@ EXPORTS_LEGACY = [
     "solve_Z", "solve_C", "bcUV", "bcP", "bcZ", "bcC", "max_reduce", "alocar_globais", "desalocar_globais",
     "cfd_legacy_init", "cfd_legacy_set_params", "cfd_legacy_set_mesh", "cfd_legacy_context",
 ]
+
+
+# U-Net convolutions (input channels, output channels, ReLU), in the library's order
+# (cfd_unet.hip build_convs): encoder 1-3, bottom, decoder 3-1 (after concatenation), head
+UNET_LAYERS = [(5, 32, 1), (32, 32, 1), (32, 64, 1), (64, 64, 1), (64, 128, 1), (128, 128, 1), (128, 256, 1),
+               (256, 256, 1), (384, 128, 1), (128, 128, 1), (192, 64, 1), (64, 64, 1), (96, 32, 1), (32, 32, 1),
+               (32, 4, 0)]
 
 
 class Tiles(ctypes.Structure):
@@ -93,6 +100,7 @@ def load():
         "cfd_pseudo_steps_graph": (ctypes.c_int, [_vp, i32, i32, _dp]),
         "cfd_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
         "cfd_ctx_set_kernel": (ctypes.c_int, [_vp, i32]),
+        "cfd_ctx_set_pipeline": (ctypes.c_int, [_vp, i32]),
         "cfd_ctx_step_counts": (ctypes.c_int, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_ctx_set_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(Tiles)]),
         "cfd_pseudo_steps_timed": (ctypes.c_int, [_vp, i32, _dp, _dp]),
@@ -128,6 +136,7 @@ def load():
         "cfd_unet_predict": (ctypes.c_int, [_vp, _fp, i32, _fp]),
         "cfd_unet_loss_grad": (ctypes.c_int, [_vp, _fp, _fp, i32, _dp, _fp]),
         "cfd_unet_train_step": (ctypes.c_int, [_vp, _fp, _fp, i32, ctypes.c_float, _dp]),
+        "cfd_unet_last_timing": (ctypes.c_int, [_vp, _dp]),
         "cfd_nn_conv3x3": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp, _vp, i32, i32, _vp, _vp]),
         "cfd_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, i32, i32, i32, i32, i32, _vp]),
         "cfd_nn_flip_weights": (ctypes.c_int, [_vp, i32, i32, _vp]),
@@ -239,6 +248,10 @@ class Context:
 
     def set_kernel(self, k):
         check(load().cfd_ctx_set_kernel(self.h, int(k)))
+
+    def set_pipeline(self, on):
+        """Two-stream pipeline of the fused path (unsteady problems): on (default) or off."""
+        check(load().cfd_ctx_set_pipeline(self.h, 1 if on else 0))
 
     def pseudo_steps(self, n, history=False):
         res = np.zeros(3)
@@ -534,6 +547,12 @@ class UNet:
         check(load().cfd_unet_train_step(self.h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), x.shape[0],
                                          ctypes.c_float(lr), ctypes.byref(loss)))
         return loss.value
+
+    def last_timing(self):
+        """Device ms of the last train_step: (forward, backward, update)."""
+        ms = (ctypes.c_double * 3)()
+        check(load().cfd_unet_last_timing(self.h, ms))
+        return tuple(ms)
 
     def close(self):
         if self.h:

@@ -884,9 +884,12 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
                 q_t = a.Tt[k]; q_c = a.Ct[k]; q_e = a.eps[k]; q_L = a.lp[k];
                 if (!STEADY) { q_T = a.T[k]; q_C = a.C[k]; }
                 if (j == 2) q_v = chan ? R(0) : R(a.v_i);
-                else if (t >= 2 && t <= im - 1 && j <= jm - 1) q_v = a.vn[(size_t)t * sv + j];
+                else if (t >= 2 && t <= im - 1 && j >= 3 && j <= jm - 1) q_v = a.vn[(size_t)t * sv + j];
             }
-            if (t > 2 && t < im) q_u = a.un[(size_t)t * su + j];
+            // interior cells only: the boundary cells of u and v may still be being written
+            // (the two-stream step runs K2 beside the u/v/p boundary pass) and no stage here
+            // reads them
+            if (t > 2 && t < im && j >= 2 && j <= jm - 1) q_u = a.un[(size_t)t * su + j];
         }
     };
     __syncthreads();
@@ -1052,11 +1055,18 @@ __device__ __forceinline__ void materialize_cell(const Geo &g, const cfd_params 
     }
 }
 
+// boundary cells t0 .. t1-1 of the materialize_cell enumeration (u, v, p first, then T, C)
 template <typename F>
-__global__ void k_materialize(Geo g, cfd_params P, F *u, F *v, double *p, F *T, F *C, const F *vprev)
+__global__ void k_materialize(Geo g, cfd_params P, F *u, F *v, double *p, F *T, F *C, const F *vprev, int t0, int t1)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    materialize_cell(g, P, u, v, p, T, C, vprev, t);
+    const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < t1) materialize_cell(g, P, u, v, p, T, C, vprev, t);
+}
+
+// first T/C cell of the materialize_cell enumeration
+__host__ __device__ inline int materialize_tc_start(int im, int jm)
+{
+    return 2 * (im + 1) + 4 * (jm - 2) + 4 * im + 2 * (jm - 3) + (2 * im + 2 * (jm - 2));
 }
 
 }  // namespace
@@ -1159,23 +1169,41 @@ int fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P
                hipStream_t s)
 {
     const bool timed = a->ev[0] != nullptr;
+    // Two-stream form (a->s2): K2 and the T/C boundary cells run on s2 after K1, so that the
+    // next step's K1 (which reads only u, v, p and the physical level) overlaps them. The
+    // caller passes the event of the K2 two steps back, whose inputs this K1 overwrites.
+    const bool two = a->s2 != nullptr && !timed;
+    const int im = g->imax, jm = g->jmax;
+    const int n = materialize_count(im, jm), ntc = materialize_tc_start(im, jm);
     K1Args<R> k1;
-    fill_k1<R>(k1, g->imax, g->jmax, f->pack, P, a);
+    fill_k1<R>(k1, im, jm, f->pack, P, a);
+    if (two && a->ev_k2_wait) (void)hipStreamWaitEvent(s, a->ev_k2_wait, 0);
     if (timed) (void)hipEventRecord(a->ev[0], s);
     launch_k1<R>(k1, P, a->occ1, s);
     int r = launched();
     if (r != CFD_OK) return r;
     if (timed) (void)hipEventRecord(a->ev[1], s);
+    const hipStream_t s2 = two ? a->s2 : s;
+    if (two) {
+        (void)hipEventRecord(a->ev_k1, s);
+        hipLaunchKernelGGL((k_materialize<R>), dim3(cdiv(ntc, 256)), dim3(256), 0, s, mk(g), *P, (R *)a->um_n,
+                           (R *)a->vm_n, a->pn, (R *)a->T_n, (R *)a->C_n, (const R *)a->vm_tau, 0, ntc);
+        r = launched();
+        if (r != CFD_OK) return r;
+        (void)hipStreamWaitEvent(s2, a->ev_k1, 0);
+    }
     K2Args<R> k2;
-    fill_k2<R>(k2, g->imax, g->jmax, g->y, f->pack, P, a);
-    launch_k2<R>(k2, P, a->occ2, s);
+    fill_k2<R>(k2, im, jm, g->y, f->pack, P, a);
+    launch_k2<R>(k2, P, a->occ2, s2);
     r = launched();
     if (r != CFD_OK) return r;
     if (timed) (void)hipEventRecord(a->ev[2], s);
-    const int n = materialize_count(g->imax, g->jmax);
-    hipLaunchKernelGGL((k_materialize<R>), dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), *P, (R *)a->um_n,
-                       (R *)a->vm_n, a->pn, (R *)a->T_n, (R *)a->C_n, (const R *)a->vm_tau);
+    const int t0 = two ? ntc : 0;
+    hipLaunchKernelGGL((k_materialize<R>), dim3(cdiv(n - t0, 256)), dim3(256), 0, s2, mk(g), *P, (R *)a->um_n,
+                       (R *)a->vm_n, a->pn, (R *)a->T_n, (R *)a->C_n, (const R *)a->vm_tau, t0, n);
     r = launched();
+    if (r != CFD_OK) return r;
+    if (two) (void)hipEventRecord(a->ev_k2_rec, s2);
     if (timed) (void)hipEventRecord(a->ev[3], s);
     return r;
 }
@@ -1261,7 +1289,7 @@ int cfdk_materialize_bc(const cfd_devgeo *g, const cfd_params *P, double *um_n, 
 {
     const int n = materialize_count(g->imax, g->jmax);
     hipLaunchKernelGGL((k_materialize<double>), dim3(cdiv(n, 256)), dim3(256), 0, s, mk(g), *P, um_n, vm_n, pn,
-                       T_n, C_n, vm_prev);
+                       T_n, C_n, vm_prev, 0, n);
     return launched();
 }
 

@@ -173,6 +173,15 @@ struct cfd_ctx {
     float *eps32, *lp32;
     int live32, geo32_stale;
     int tiles_set;         /* chosen by the caller or the autotuner */
+    /* two-stream pipeline of the fused path (unsteady problems, context stream): K2 of step
+     * n on s2 overlaps K1 of step n+1 on s. ek2[n % 2] marks the end of K2(n); K1(n+2)
+     * waits for it (it overwrites K2(n)'s u, v input). Every call that steps joins s2 back
+     * into s before it returns, so other calls only ever see the main stream. */
+    int pipe;              /* enabled (cfd_ctx_set_pipeline) */
+    hipStream_t s2;
+    hipEvent_t ek1, ek2[2];
+    int ek2_live[2], ek2_last;
+    int64_t pipe_n;
 };
 
 static void tiles_from_cache(cfd_ctx *c);
@@ -238,6 +247,13 @@ static int ctx_alloc(cfd_ctx *c)
     CHK(ensure_hist(c, 64));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    if (c->owns_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->ek1, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ek2[0], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ek2[1], hipEventDisableTiming));
+        c->pipe = 1;
+    }
     /* eps = 1 everywhere until a porosity field is uploaded */
     CHK(cfdk_fill(g->eps, 1.0, (long)c->np, c->s));
     return CFD_OK;
@@ -260,6 +276,9 @@ static void ctx_free(cfd_ctx *c)
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     for (int k = 0; k < 4; ++k) if (c->evk[k]) hipEventDestroy(c->evk[k]);
+    if (c->ek1) hipEventDestroy(c->ek1);
+    for (int k = 0; k < 2; ++k) if (c->ek2[k]) hipEventDestroy(c->ek2[k]);
+    if (c->s2) hipStreamDestroy(c->s2);
     if (c->owns_stream && c->s) hipStreamDestroy(c->s);
 }
 
@@ -319,6 +338,14 @@ int cfd_ctx_set_params(cfd_ctx *c, const cfd_params *p)
     if (p->imax != c->P.imax || p->jmax != c->P.jmax)
         return fail(CFD_ERR_ARG, "grid size cannot change (%dx%d -> %dx%d)", c->P.imax, c->P.jmax, p->imax, p->jmax);
     c->P = *p;
+    return CFD_OK;
+}
+
+int cfd_ctx_set_pipeline(cfd_ctx *c, int32_t on)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    if (on && !c->s2) return fail(CFD_ERR_UNSUPPORTED, "the pipeline needs a context stream (not the legacy default stream)");
+    c->pipe = on ? 1 : 0;
     return CFD_OK;
 }
 
@@ -624,6 +651,17 @@ static int step_exact(cfd_ctx *c, uint64_t *slot)
     return CFD_OK;
 }
 
+/* Joins the pipeline's second stream into the main stream: work queued on s after this
+ * sees every K2 launched so far. */
+static int pipe_join(cfd_ctx *c)
+{
+    if (c->ek2_live[0] || c->ek2_live[1]) {
+        HIPCHK(hipStreamWaitEvent(c->s, c->ek2[c->ek2_last], 0));
+        c->ek2_live[0] = c->ek2_live[1] = 0;
+    }
+    return CFD_OK;
+}
+
 /* Fused step: boundary-consistent iterates in, outputs with boundaries written last.
  * With `timed`, the context's per-kernel events bracket each launch. */
 static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
@@ -654,8 +692,23 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     a.hb1 = c->tiles.rows_uvp; a.hb2 = c->tiles.rows_tc;
     a.occ1 = c->tiles.waves_uvp; a.occ2 = c->tiles.waves_tc;
     if (timed) for (int k = 0; k < 4; ++k) a.ev[k] = c->evk[k];
+    /* unsteady problems only: in steady mode K1 reads the T iterate that K2 writes */
+    const int two = c->pipe && c->s2 && !timed && !st;
+    if (two) {
+        const int q = (int)(c->pipe_n & 1);
+        a.s2 = c->s2; a.ev_k1 = c->ek1;
+        a.ev_k2_wait = c->ek2_live[q] ? c->ek2[q] : NULL;
+        a.ev_k2_rec = c->ek2[q];
+    } else {
+        CHK(pipe_join(c));
+    }
     CHK(cfdk_zero_u64(slot, 3, c->s));
     CHK(cfdk_fused_step(&c->g, &c->fac, &c->P, &a, c->kernel, c->s));
+    if (two) {
+        const int q = (int)(c->pipe_n & 1);
+        c->ek2_live[q] = 1; c->ek2_last = q;
+        c->pipe_n++;
+    }
     c->bc_clean = 1;          /* cfdk_fused_step writes the boundary cells last */
     c->first_step = 0;
     c->n_fused++;
@@ -714,6 +767,7 @@ static int one_step(cfd_ctx *c, uint64_t *slot)
 {
     if (fused_ok(c))
         return step_fused(c, slot);
+    CHK(pipe_join(c));
     CHK(materialize(c));
     return step_exact(c, slot);
 }
@@ -745,6 +799,7 @@ int cfd_pseudo_steps(cfd_ctx *c, int32_t nsteps, double res[3], double *history)
         CHK(one_step(c, slot));
         tau_swap(c);
     }
+    CHK(pipe_join(c));
     HIPCHK(hipEventRecord(c->ev1, c->s));
     HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, (size_t)nslots * 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -774,6 +829,7 @@ int cfd_run_to_tol(cfd_ctx *c, double tol, int64_t itc_max, int32_t check_every,
             CHK(one_step(c, c->d_hist + 3 * k));
             tau_swap(c);
         }
+        CHK(pipe_join(c));
         HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, (size_t)n * 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));
         for (int32_t k = 0; k < n; ++k) {
@@ -827,6 +883,7 @@ int cfd_physical_step(cfd_ctx *c, double eps, int64_t itc_max, int32_t first, in
     while (itc < itc_max) {
         CHK(one_step(c, c->d_hist));
         ++taken;
+        CHK(pipe_join(c));
         HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));
         for (int q = 0; q < 3; ++q) r[q] = u64_to_res(c->h_hist[q]);
@@ -881,12 +938,15 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
     CHK(ensure_hist(c, 1));
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
+    CHK(pipe_join(c));            /* no wait inside the graph on work from before it */
     HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     int r = CFD_OK;
     for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) {
         r = one_step(c, c->d_hist);
         tau_swap(c);
     }
+    if (r == CFD_OK) r = pipe_join(c);    /* the second stream rejoins before the capture ends */
+    c->ek2_live[0] = c->ek2_live[1] = 0;
     hipError_t e = hipStreamEndCapture(c->s, &graph);
     if (r != CFD_OK) { if (graph) hipGraphDestroy(graph); return r; }
     if (e != hipSuccess) return fail(CFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -988,6 +1048,7 @@ static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_
         CHK(step_fused(c, c->d_hist));
         tau_swap(c);
     }
+    CHK(pipe_join(c));
     HIPCHK(hipEventRecord(c->ev1, c->s));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -999,8 +1060,8 @@ static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_
 /* Candidate shapes. The u/v/p kernel holds ~300 registers per lane: one wave per SIMD
  * without spills, or two with the excess spilled to scratch (then more, shorter row blocks
  * fill the 2 x 256 wave slots); variant 3 runs the u and v chains in separate waves of a
- * 512-lane workgroup, each with half the register windows. The T/C kernel runs 3 waves per SIMD as compiled, or 4 with
- * spills. */
+ * 512-lane workgroup, each with half the register windows. The T/C kernel runs 3 waves per
+ * SIMD as compiled, or 4 with spills. Shapes are timed with the pipeline as configured. */
 static const cfd_tiles CAND_UVP[] = {
     {0, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0}, {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
     {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
@@ -1061,6 +1122,7 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)
     }
     /* restore the state exactly as it was (the FP64 buffers hold it; float copies are
      * rebuilt by the next fused step) */
+    if (pipe_join(c) != CFD_OK && r == CFD_OK) r = CFD_ERR_HIP;
     memcpy(c->st, roles, sizeof roles);
     memcpy(c->s32, roles32, sizeof roles32);
     c->live32 = 0;

@@ -71,6 +71,11 @@ typedef struct cfd_fused_args {
     int32_t hb1, hb2, occ1, occ2;
     /* optional events recorded before K1, after K1, after K2 and after the boundary pass */
     hipEvent_t ev[4];
+    /* optional second stream: K2 and the T/C boundary cells run on s2 after ev_k1 (recorded
+     * on the main stream after K1), ev_k2_rec is recorded on s2 at the end, and K1 first
+     * waits for ev_k2_wait (the K2 two steps back, whose input buffer K1 overwrites) */
+    hipStream_t s2;
+    hipEvent_t ev_k1, ev_k2_wait, ev_k2_rec;
 } cfd_fused_args;
 
 /* Precomputed 1-D geometry factors for the fused kernels (allocated by the host). */

@@ -6,8 +6,11 @@
 // index grid, from which the solver's staggered initial fields are taken.
 //
 // Numerics: activations and their gradients are stored in BF16 (NHWC), every convolution
-// is an implicit GEMM on the CDNA matrix cores (v_mfma_f32_32x32x8bf16_1k: BF16 operands,
-// FP32 accumulation), master weights, weight gradients, biases and the Adam state are FP32.
+// is an implicit GEMM on the CDNA matrix cores (BF16 operands, FP32 accumulation): layers
+// whose channel counts are multiples of 32 run spatially tiled kernels on the gfx950
+// v_mfma_f32_32x32x16_bf16, the first layer and the head a generic im2col kernel on
+// v_mfma_f32_32x32x8bf16_1k. Master weights, weight gradients, biases and the Adam state
+// are FP32.
 //
 //   forward   y[m][n]  = sum_k A[m][k] W[n][k] + b[n]     m = pixel, n = cout,
 //                                                          k = (dy, dx, cin), A = im2col(x)
@@ -24,6 +27,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <vector>
 
 #include "../../include/cfd.h"
@@ -33,7 +37,6 @@ extern "C" int cfd__fail(int code, const char *fmt, ...);
 namespace {
 
 typedef uint16_t bf16;
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __host__ inline float bf2f(bf16 h)
@@ -54,130 +57,306 @@ __device__ __host__ inline bf16 f2bf(float f)        // round to nearest even
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 inline int launched() { return hipGetLastError() == hipSuccess ? CFD_OK : CFD_ERR_HIP; }
 
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;   // LDS row pitch avoids bank conflicts
-
 // ---------------------------------------------------------------------------------------
-// implicit-GEMM 3x3 convolution, stride 1, zero padding 1, NHWC
-// ---------------------------------------------------------------------------------------
+// 3x3 convolution, stride 1, zero padding 1, NHWC, as an implicit GEMM on the matrix cores
 // x: [B][H][W][Cin] bf16; w: [Cout][9*Cin] bf16 (k = (dy*3+dx)*Cin + ci); y: [B][H][W][Cout]
-template <bool RELU>
-__global__ __launch_bounds__(256) void k_conv3x3(const bf16 *__restrict__ x, int B, int H, int W, int Cin,
-                                                 const bf16 *__restrict__ w, const float *__restrict__ bias,
-                                                 int Cout, bf16 *__restrict__ y, float *__restrict__ y32)
+//
+// Spatial tiling: a 16 x 16 pixel tile plus its one-pixel halo is staged in LDS once per
+// 32-channel chunk and serves all nine taps, so each input value leaves L2 about 1.3 times
+// instead of nine. Operands go to v_mfma_f32_32x32x16_bf16 (gfx950: 16 k per instruction,
+// 8 consecutive channels per lane); the next chunk's loads are issued into registers before
+// the current chunk's MFMAs. RAGGED kernels (channel counts off the 32 grid: the first
+// layer's 5 inputs, the head's 4 outputs) load element by element and zero-fill the chunk.
+// ---------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TT = 16;                 // output tile edge (pixels)
+constexpr int TH2 = TT + 2;            // halo tile edge
+constexpr int CK = 32;                 // channels per chunk
+constexpr int CP = CK + 8;             // LDS pitch per pixel / per weight row: 80 B, conflict-free 16 B reads
+
+// 8 consecutive channels c0..c0+7 of one pixel row `p` (channel count C): one 16 B load, or
+// element loads with zeros past C
+template <bool RAGGED>
+__device__ __forceinline__ uint4 load8(const bf16 *p, int c0, int C)
 {
-    __shared__ bf16 As[BM][LDK];
-    __shared__ bf16 Bs[BN][LDK];
-    const int M = B * H * W, K = 9 * Cin;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    if (!RAGGED) return *(const uint4 *)(p + c0);
+    union { uint4 v; bf16 e[8]; } u;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) u.e[q] = c0 + q < C ? p[c0 + q] : (bf16)0;
+    return u.v;
+}
+
+template <int BNT>
+struct __align__(16) ConvSmem {     // 16 B aligned: operand reads are ds_read_b128
+    bf16 xs[TH2 * TH2][CP];            // halo tile, one chunk of channels
+    bf16 ws[9][BNT][CP];               // weights of the chunk for all taps
+};
+
+template <int BNT, bool RELU, bool RAGGED>
+__global__ __launch_bounds__(256, 2) void k_conv3x3_t(const bf16 *__restrict__ x, int B, int H, int W, int Cin,
+                                                      const bf16 *__restrict__ w, const float *__restrict__ bias,
+                                                      int Cout, bf16 *__restrict__ y, float *__restrict__ y32)
+{
+    constexpr int NB = BNT / 32;                        // 32-wide cout blocks per wave
+    constexpr int NXV = TH2 * TH2 * (CK / 8);           // 16 B vectors of the halo tile
+    constexpr int NWV = 9 * BNT * (CK / 8);             // 16 B vectors of the weights
+    constexpr int RX = (NXV + 255) / 256, RW = (NWV + 255) / 256;
+    __shared__ ConvSmem<BNT> sm;
+    const int tw = (W + TT - 1) / TT, th = (H + TT - 1) / TT;
+    int tile = blockIdx.x;
+    const int tx = tile % tw; tile /= tw;
+    const int ty = tile % th;
+    const int b = tile / th;
+    const int h0 = ty * TT, w0 = tx * TT, n0 = blockIdx.y * BNT;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
-    // this thread's A row and k slice
-    const int ar = t >> 2, ak = (t & 3) * 8;
-    const int am = m0 + ar;
-    int ab = 0, ah = 0, aw = 0;
-    if (am < M) { ab = am / (H * W); const int rem = am - ab * H * W; ah = rem / W; aw = rem - ah * W; }
-    f32x16 acc;
-    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        // A tile: im2col gather with zero padding
+    const int half = lane >> 5, l32 = lane & 31;
+    const size_t K = (size_t)9 * Cin;
+    const int nchunk = (Cin + CK - 1) / CK;
+
+    uint4 rx[RX], rw[RW];
+    auto fetch = [&](int c) {
+        const int ci0 = c * CK;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int k = k0 + ak + q;
-            bf16 v = 0;
-            if (am < M && k < K) {
-                const int r = k / Cin, ci = k - r * Cin;
-                const int hh = ah + r / 3 - 1, ww = aw + r % 3 - 1;
-                if (hh >= 0 && hh < H && ww >= 0 && ww < W) v = x[((size_t)(ab * H + hh) * W + ww) * Cin + ci];
+        for (int i = 0; i < RX; ++i) {
+            const int v = t + 256 * i;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (v < NXV) {
+                const int pix = v >> 2, qd = v & 3;
+                const int hh = h0 - 1 + pix / TH2, ww = w0 - 1 + pix % TH2;
+                if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+                    val = load8<RAGGED>(x + ((size_t)(b * H + hh) * W + ww) * Cin, ci0 + 8 * qd, Cin);
             }
-            As[ar][ak + q] = v;
+            rx[i] = val;
         }
-        // B tile: Bs[n][k] = w[n][k]
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int n = n0 + ar, k = k0 + ak + q;
-            Bs[ar][ak + q] = (n < Cout && k < K) ? w[(size_t)n * K + k] : (bf16)0;
+        for (int i = 0; i < RW; ++i) {
+            const int v = t + 256 * i;
+            const int n = v / 36, rem = v - n * 36, tap = rem >> 2, qd = rem & 3;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (v < NWV && (!RAGGED || n0 + n < Cout))
+                val = load8<RAGGED>(w + (size_t)(n0 + n) * K + (size_t)tap * Cin, ci0 + 8 * qd, Cin);
+            rw[i] = val;
         }
-        __syncthreads();
+    };
+    auto stash = [&]() {
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 8) {
-            const int kk = ks + 4 * (lane >> 5);
-            bf16x4 a, b;
-            const bf16 *pa = &As[wm + (lane & 31)][kk];
-            const bf16 *pb = &Bs[wn + (lane & 31)][kk];
-            a[0] = (short)pa[0]; a[1] = (short)pa[1]; a[2] = (short)pa[2]; a[3] = (short)pa[3];
-            b[0] = (short)pb[0]; b[1] = (short)pb[1]; b[2] = (short)pb[2]; b[3] = (short)pb[3];
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        for (int i = 0; i < RX; ++i) {
+            const int v = t + 256 * i;
+            if (v < NXV) *(uint4 *)&sm.xs[v >> 2][8 * (v & 3)] = rx[i];
         }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const int v = t + 256 * i;
+            const int n = v / 36, rem = v - n * 36, tap = rem >> 2, qd = rem & 3;
+            if (v < NWV) *(uint4 *)&sm.ws[tap][n][8 * qd] = rw[i];
+        }
+    };
+
+    f32x16 acc[2][NB];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[mb][nb][q] = 0.f;
+
+    // this lane's A pixel of m-block mb: tile row 4 wv + 2 mb + l32 / 16, column l32 % 16
+    const int prow = 4 * wv + (l32 >> 4), pcol = l32 & 15;
+    fetch(0);
+    for (int c = 0; c < nchunk; ++c) {
+        __syncthreads();                 // the previous chunk's reads are done
+        stash();
         __syncthreads();
+        if (c + 1 < nchunk) fetch(c + 1);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int kc = 16 * s + 8 * half;
+                bf16x8 a[2], bb[NB];
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb)
+                    a[mb] = *(const bf16x8 *)&sm.xs[(prow + 2 * mb + dy) * TH2 + pcol + dx][kc];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) bb[nb] = *(const bf16x8 *)&sm.ws[tap][32 * nb + l32][kc];
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mb], bb[nb], acc[mb][nb], 0, 0, 0);
+            }
+        }
     }
     // D layout of the 32x32 MFMA: lane l, element r -> row 8 (r/4) + 4 (l/32) + r%4, col l%32
-    const int n = n0 + wn + (lane & 31);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (m < M && n < Cout) {
-            float v = acc[r] + (bias ? bias[n] : 0.f);
-            if (RELU) v = v > 0.f ? v : 0.f;
-            if (y) y[(size_t)m * Cout + n] = f2bf(v);
-            if (y32) y32[(size_t)m * Cout + n] = v;
-        }
+    for (int nb = 0; nb < NB; ++nb) {
+        const int n = n0 + 32 * nb + l32;
+        if (RAGGED && n >= Cout) continue;
+        const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = 8 * (r >> 2) + 4 * half + (r & 3);
+                const int hh = h0 + 4 * wv + 2 * mb + (m >> 4), ww = w0 + (m & 15);
+                if (hh < H && ww < W) {
+                    float v = acc[mb][nb][r] + bn;
+                    if (RELU) v = v > 0.f ? v : 0.f;
+                    const size_t o = ((size_t)(b * H + hh) * W + ww) * Cout + n;
+                    if (y) y[o] = f2bf(v);
+                    if (y32) y32[o] = v;
+                }
+            }
     }
 }
 
-// dW[n][k] += sum_m dy[m][n] * im2col(x)[m][k] over this block's slice of m
-__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
-                                                       int B, int H, int W, int Cin, int Cout, int m_per_block,
-                                                       float *__restrict__ dw)
+// ---------------------------------------------------------------------------------------
+// spatially tiled weight gradient:
+//   dW[n][tap][ci] = sum over pixels p of dy[p][n] * x[p + offset(tap)][ci]
+// The reduction runs over pixels, which are strided in NHWC, so the block stages both
+// operands transposed in LDS: dy of a 16 x 16 pixel tile as dyT[n][pixel], and the halo tile
+// of x as three copies shifted by the tap column (xs[dx][ci][row][col] = x(row, col + dx)),
+// which keeps every MFMA operand read an aligned 16 B load. Block (tile group, ci chunk,
+// n block): the four waves take four pixel rows each for all nine taps and loop over their
+// group's tiles, the next tile's loads in flight during the current tile's MFMAs; the waves'
+// sums meet in LDS and the block writes them to its own slice of a partial buffer, which
+// k_wsum reduces over the groups (no atomics on the gradient).
+// ---------------------------------------------------------------------------------------
+constexpr int PPD = TT * TT + 8;            // dyT pitch (bf16): 528 B, conflict-free 16 B reads
+constexpr int PPX = TH2 * TT + 8;           // xs pitch per channel (bf16): 592 B
+
+struct __align__(16) WgradSmem {
+    union {
+        struct {
+            bf16 dyT[32][PPD];
+            bf16 xs[3][CK][PPX];
+        } t;
+        float red[9][32][32];
+    };
+};
+
+template <bool RAGGED>
+__global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
+                                                            int B, int H, int W, int Cin, int Cout,
+                                                            float *__restrict__ part)
 {
-    __shared__ bf16 Ys[BN][LDK];     // [n][m]
-    __shared__ bf16 Xs[BM][LDK];     // [k][m]
-    const int M = B * H * W, K = 9 * Cin;
-    const int n0 = blockIdx.x * BN, kt0 = blockIdx.y * BM;
-    const int mb = blockIdx.z * m_per_block, me = min(M, mb + m_per_block);
+    constexpr int NXP = TH2 * TH2, RX = (NXP + 63) / 64;
+    __shared__ WgradSmem sm;
+    const int tw = (W + TT - 1) / TT, th = (H + TT - 1) / TT, ntiles = B * th * tw;
+    const int ci0 = blockIdx.y * CK, n0 = blockIdx.z * 32;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;        // wave tile: rows n, cols k
-    const int lr = t >> 2, lm = (t & 3) * 8;                   // loader: row (n or k), m slice
-    f32x16 acc;
-    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-    for (int m0 = mb; m0 < me; m0 += BK) {
+    const int half = lane >> 5, l32 = lane & 31;
+    f32x16 acc[9];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int m = m0 + lm + q, n = n0 + lr;
-            Ys[lr][lm + q] = (m < me && n < Cout) ? dy[(size_t)m * Cout + n] : (bf16)0;
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[q][e] = 0.f;
+
+    // Each wave moves one group of 8 channels (qd = wave) for 64 consecutive pixels per
+    // step, so the transposed 2 B stores of a wave land in one LDS row (few bank conflicts).
+    const int qd = wv;
+    uint4 ry[4], rx[RX];
+    auto fetch = [&](int tile) {
+        const int tx = tile % tw, ty = (tile / tw) % th, b = tile / (tw * th);
+        const int h0 = ty * TT, w0 = tx * TT;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {            // dy: 256 pixels x 32 couts
+            const int pix = 64 * i + lane;
+            const int hh = h0 + (pix >> 4), ww = w0 + (pix & 15);
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (hh < H && ww < W) val = load8<RAGGED>(dy + ((size_t)(b * H + hh) * W + ww) * Cout, n0 + 8 * qd, Cout);
+            ry[i] = val;
         }
-        const int k = kt0 + lr;
-        int r = 0, ci = 0;
-        if (k < K) { r = k / Cin; ci = k - r * Cin; }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int m = m0 + lm + q;
-            bf16 v = 0;
-            if (m < me && k < K) {
-                const int b = m / (H * W), rem = m - b * H * W, h = rem / W, wq = rem - h * W;
-                const int hh = h + r / 3 - 1, ww = wq + r % 3 - 1;
-                if (hh >= 0 && hh < H && ww >= 0 && ww < W) v = x[((size_t)(b * H + hh) * W + ww) * Cin + ci];
+        for (int i = 0; i < RX; ++i) {           // x: 18 x 18 halo pixels x 32 channels
+            const int pix = 64 * i + lane;
+            const int hh = h0 - 1 + pix / TH2, ww = w0 - 1 + pix % TH2;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (pix < NXP && hh >= 0 && hh < H && ww >= 0 && ww < W)
+                val = load8<RAGGED>(x + ((size_t)(b * H + hh) * W + ww) * Cin, ci0 + 8 * qd, Cin);
+            rx[i] = val;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int pix = 64 * i + lane;
+            union { uint4 v; bf16 e[8]; } u;
+            u.v = ry[i];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sm.t.dyT[8 * qd + q][pix] = u.e[q];
+        }
+#pragma unroll
+        for (int i = 0; i < RX; ++i) {
+            const int pix = 64 * i + lane;
+            if (pix >= NXP) continue;
+            const int hr = pix / TH2, hc = pix - hr * TH2;
+            union { uint4 v; bf16 e[8]; } u;
+            u.v = rx[i];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int c = hc - dx;
+                if (c >= 0 && c < TT) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) sm.t.xs[dx][8 * qd + q][hr * TT + c] = u.e[q];
+                }
             }
-            Xs[lr][lm + q] = v;
         }
+    };
+
+    if (blockIdx.x < ntiles) fetch(blockIdx.x);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();                    // the previous tile's operand reads are done
+        stash();
         __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 8) {
-            const int kk = ks + 4 * (lane >> 5);
-            bf16x4 a, b;
-            const bf16 *pa = &Ys[wr + (lane & 31)][kk];
-            const bf16 *pb = &Xs[wc + (lane & 31)][kk];
-            a[0] = (short)pa[0]; a[1] = (short)pa[1]; a[2] = (short)pa[2]; a[3] = (short)pa[3];
-            b[0] = (short)pb[0]; b[1] = (short)pb[1]; b[2] = (short)pb[2]; b[3] = (short)pb[3];
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        for (int ks = 0; ks < 4; ++ks) {
+            const int r = 4 * wv + ks;      // tile row: pixels 16 r .. 16 r + 15 are this k step
+            const bf16x8 a = *(const bf16x8 *)&sm.t.dyT[l32][16 * r + 8 * half];
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int dyy = tap / 3, dxx = tap - 3 * (tap / 3);
+                const bf16x8 bb = *(const bf16x8 *)&sm.t.xs[dxx][l32][(r + dyy) * TT + 8 * half];
+                acc[tap] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[tap], 0, 0, 0);
+            }
         }
-        __syncthreads();
     }
-    const int kc = kt0 + wc + (lane & 31);
+    // sum the four waves' partials in LDS; the block's sums go to its own slice of the
+    // partial buffer part[blockIdx.x][Cout][9 Cin] (reduced over the tile groups later)
+    __syncthreads();
+    for (int v = t; v < 9 * 32 * 32; v += 256) (&sm.red[0][0][0])[v] = 0.f;
+    __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int n = n0 + wr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (n < Cout && kc < K && acc[r] != 0.f) atomicAdd(&dw[(size_t)n * K + kc], acc[r]);
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int n = 8 * (e >> 2) + 4 * half + (e & 3);      // D row (A row = cout)
+            atomicAdd(&sm.red[tap][n][l32], acc[tap][e]);
+        }
+    __syncthreads();
+    const size_t K = (size_t)9 * Cin;
+    float *out = part + (size_t)blockIdx.x * K * Cout;
+    for (int v = t; v < 9 * 32 * 32; v += 256) {
+        const int tap = v >> 10, n = (v >> 5) & 31, ci = v & 31;
+        if (RAGGED && (n0 + n >= Cout || ci0 + ci >= Cin)) continue;
+        out[(size_t)(n0 + n) * K + (size_t)tap * Cin + ci0 + ci] = sm.red[tap][n][ci];
     }
+}
+
+// dw[i] += sum over g < G of part[g][i]; blockIdx.y takes a slice of the groups (atomic
+// adds when there are several slices)
+__global__ void k_wsum(const float *__restrict__ part, int G, int gper, long total, float *__restrict__ dw)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int g0 = blockIdx.y * gper, g1 = min(G, g0 + gper);
+    float s = 0.f;
+    for (int g = g0; g < g1; ++g) s += part[(size_t)g * total + i];
+    if (gridDim.y == 1) dw[i] += s;
+    else atomicAdd(&dw[i], s);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -189,15 +368,32 @@ __global__ void k_relu_bwd(bf16 *g, const bf16 *y, long n)          // g *= (y >
         if (!(bf2f(y[i]) > 0.f)) g[i] = 0;
 }
 
-__global__ void k_bias_grad(const bf16 *g, long M, int C, float *db)
+// db[c] += sum over m of g[m][c], C a power of two <= 2048: each thread reads 16 B vectors
+// at a stride that is a multiple of C, so its eight lanes keep fixed channels; per-block sums
+// meet in LDS and leave with one atomic per channel.
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ g, long M, int C, float *__restrict__ db)
 {
-    // one block per channel chunk of 64 lanes x 4 row groups
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int grp = threadIdx.x >> 6;
-    float s = 0.f;
-    if (c < C)
-        for (long m = (long)blockIdx.y * 4 + grp; m < M; m += 4L * gridDim.y) s += bf2f(g[m * C + c]);
-    if (c < C && s != 0.f) atomicAdd(&db[c], s);
+    __shared__ float red[2048];
+    for (int c = threadIdx.x; c < C; c += 256) red[c] = 0.f;
+    __syncthreads();
+    const long n = M * C, nvec = n / 8;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const long v0 = (long)blockIdx.x * 256 + threadIdx.x, stride = (long)gridDim.x * 256;
+    for (long v = v0; v < nvec; v += stride) {
+        const uint4 q = *(const uint4 *)(g + 8 * v);
+        const bf16 *e = (const bf16 *)&q;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += bf2f(e[k]);
+    }
+    if (v0 < nvec) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) atomicAdd(&red[(8 * v0 + k) & (C - 1)], s[k]);
+    }
+    if (blockIdx.x == 0)                                   // tail elements (n not a multiple of 8)
+        for (long e = 8 * nvec + threadIdx.x; e < n; e += 256) atomicAdd(&red[e & (C - 1)], bf2f(g[e]));
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += 256)
+        if (red[c] != 0.f) atomicAdd(&db[c], red[c]);
 }
 
 __global__ void k_maxpool2(const bf16 *x, int B, int H, int W, int C, bf16 *y)
@@ -341,6 +537,28 @@ __global__ void k_adam(float *w, const float *g, float *m, float *v, long n, flo
 
 inline int eb(long n) { const long b = (n + 255) / 256; return (int)(b < 8192 ? (b > 0 ? b : 1) : 8192); }
 
+// Partial sums of the weight gradients: one device buffer for the process, grown on
+// demand. The nn operations run in order on the NULL stream, so consecutive calls share it;
+// callers that issue them from several host threads at once must serialise them.
+std::mutex g_ws_mu;
+float *g_ws = nullptr;
+size_t g_ws_n = 0;
+
+float *workspace(size_t n)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (n > g_ws_n) {
+        if (g_ws) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(g_ws);
+            g_ws = nullptr; g_ws_n = 0;
+        }
+        if (hipMalloc((void **)&g_ws, n * sizeof(float)) != hipSuccess) { g_ws = nullptr; return nullptr; }
+        g_ws_n = n;
+    }
+    return g_ws;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -351,27 +569,51 @@ extern "C" {
 int cfd_nn_conv3x3(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t Cin, const uint16_t *w,
                    const float *bias, int32_t Cout, int32_t relu, uint16_t *y, float *y32)
 {
-    const int M = B * H * W;
-    const dim3 g(cdiv(M, BM), cdiv(Cout, BN));
-    if (relu) hipLaunchKernelGGL(k_conv3x3<true>, g, dim3(256), 0, 0, x, B, H, W, Cin, w, bias, Cout, y, y32);
-    else hipLaunchKernelGGL(k_conv3x3<false>, g, dim3(256), 0, 0, x, B, H, W, Cin, w, bias, Cout, y, y32);
+    if (B < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return cfd__fail(CFD_ERR_ARG, "cfd_nn_conv3x3: bad shape");
+    const int tiles = B * cdiv(H, TT) * cdiv(W, TT);
+    const bool ragged = (Cin % CK) != 0 || (Cout % 32) != 0;
+#define CONV_LAUNCH(BNT, RAG)                                                                               \
+    {                                                                                                       \
+        const dim3 g(tiles, cdiv(Cout, BNT));                                                               \
+        if (relu) hipLaunchKernelGGL((k_conv3x3_t<BNT, true, RAG>), g, dim3(256), 0, 0, x, B, H, W, Cin, w, bias, \
+                                     Cout, y, y32);                                                         \
+        else hipLaunchKernelGGL((k_conv3x3_t<BNT, false, RAG>), g, dim3(256), 0, 0, x, B, H, W, Cin, w, bias, \
+                                Cout, y, y32);                                                              \
+    }
+    if (ragged) CONV_LAUNCH(32, true)
+    else if (Cout % 64 == 0) CONV_LAUNCH(64, false)
+    else CONV_LAUNCH(32, false)
+#undef CONV_LAUNCH
     return launched();
 }
 
 int cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin,
                          int32_t Cout, float *dw)
 {
-    const int M = B * H * W, K = 9 * Cin;
-    // split m so that the grid has a few hundred blocks
-    const int tiles = cdiv(Cout, BN) * cdiv(K, BM);
-    int splits = cdiv(512, tiles);
-    const int maxs = cdiv(M, BK);
-    if (splits > maxs) splits = maxs;
-    if (splits < 1) splits = 1;
-    const int mpb = cdiv(cdiv(M, splits), BK) * BK;
-    splits = cdiv(M, mpb);
-    hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(cdiv(Cout, BN), cdiv(K, BM), splits), dim3(256), 0, 0, x, dy, B, H, W,
-                       Cin, Cout, mpb, dw);
+    if (B < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return cfd__fail(CFD_ERR_ARG, "cfd_nn_conv3x3_wgrad: bad shape");
+    const long total = 9L * Cin * Cout;
+    const bool ragged = (Cin % CK) != 0 || (Cout % 32) != 0;
+    // tile groups so that the grid holds about 1024 blocks (two rounds of two per CU)
+    const int ntiles = B * cdiv(H, TT) * cdiv(W, TT);
+    const int per = cdiv(Cin, CK) * cdiv(Cout, 32);
+    int G = cdiv(1024, per);
+    if (G > ntiles) G = ntiles;
+    if (G < 1) G = 1;
+    float *part = workspace((size_t)G * total);
+    if (!part) return cfd__fail(CFD_ERR_HIP, "wgrad workspace of %ld floats", (long)G * total);
+    const dim3 grid(G, cdiv(Cin, CK), cdiv(Cout, 32));
+    if (ragged) hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part);
+    else hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part);
+    int r = launched();
+    if (r != CFD_OK) return r;
+    // reduce over the groups: about 2048 blocks, each summing a slice of the groups
+    const int nbx = cdiv(total, 256);
+    int gy = 2048 / nbx;
+    if (gy > G) gy = G;
+    if (gy < 1) gy = 1;
+    const int gper = cdiv(G, gy);
+    gy = cdiv(G, gper);
+    hipLaunchKernelGGL(k_wsum, dim3(nbx, gy), dim3(256), 0, 0, (const float *)part, G, gper, total, dw);
     return launched();
 }
 
@@ -430,7 +672,7 @@ struct Dev {
         n = count;
         return hipMalloc((void **)&p, count * sizeof(T) + 256) == hipSuccess ? CFD_OK : CFD_ERR_HIP;
     }
-    void release() { if (p) hipFree(p); p = nullptr; }
+    void release() { if (p) (void)hipFree(p); p = nullptr; }
 };
 
 }  // namespace
@@ -452,6 +694,7 @@ struct cfd_unet {
     Dev<float> xin32;
     Dev<bf16> xin;
     long step = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // inputs on the device, forward, backward, update
 };
 
 namespace {
@@ -567,9 +810,11 @@ int conv_bwd(cfd_unet *u, int q, int B, int ain, const bf16 *gy)
     const long M = (long)B * s.h * s.w;
     int r = cfd_nn_conv3x3_wgrad(u->act[ain].p, gy, B, s.h, s.w, c.cin, c.cout, u->g32.p + c.woff);
     if (r != CFD_OK) return r;
-    const int by = (int)((M + 255) / 256 < 64 ? (M + 255) / 256 : 64);
-    hipLaunchKernelGGL(k_bias_grad, dim3(cdiv(c.cout, 64), by > 0 ? by : 1), dim3(256), 0, 0, gy, M, c.cout,
-                       u->g32.p + c.boff);
+    if (c.cout > 2048 || (c.cout & (c.cout - 1)))
+        return cfd__fail(CFD_ERR_UNSUPPORTED, "bias gradient needs a power-of-two channel count, got %d", c.cout);
+    const long nvec = M * c.cout / 8;
+    const int nb = (int)(cdiv(nvec, 256) < 1024 ? (cdiv(nvec, 256) > 0 ? cdiv(nvec, 256) : 1) : 1024);
+    hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, 0, gy, M, c.cout, u->g32.p + c.boff);
     r = launched();
     if (r != CFD_OK || ain == A_IN) return r;
     return cfd_nn_conv3x3(gy, B, s.h, s.w, c.cout, u->wf.p + u->wf_off[q], nullptr, c.cin, 0, u->gbuf[ain].p, nullptr);
@@ -687,6 +932,8 @@ int cfd_unet_create(int32_t height, int32_t width, int32_t batch, uint64_t seed,
     u->H = height; u->W = width; u->Bmax = batch;
     build_convs(u);
     int r = CFD_OK;
+    for (int k = 0; k < 4 && r == CFD_OK; ++k)
+        if (hipEventCreate(&u->ev[k]) != hipSuccess) r = CFD_ERR_HIP;
     r = u->w32.alloc(u->nparam); if (r == CFD_OK) r = u->g32.alloc(u->nparam);
     if (r == CFD_OK) r = u->m32.alloc(u->nparam); if (r == CFD_OK) r = u->v32.alloc(u->nparam);
     if (r == CFD_OK) r = u->wb.alloc(u->nparam);
@@ -726,10 +973,12 @@ int cfd_unet_create(int32_t height, int32_t width, int32_t batch, uint64_t seed,
 int cfd_unet_destroy(cfd_unet *u)
 {
     if (!u) return CFD_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     u->w32.release(); u->g32.release(); u->m32.release(); u->v32.release(); u->wb.release(); u->wf.release();
     for (int a = 0; a < 40; ++a) { u->act[a].release(); u->gacc[a].release(); u->gbuf[a].release(); }
     u->out32.release(); u->target.release(); u->gout.release(); u->loss.release(); u->xin32.release(); u->xin.release();
+    for (int k = 0; k < 4; ++k)
+        if (u->ev[k]) (void)hipEventDestroy(u->ev[k]);
     delete u;
     return CFD_OK;
 }
@@ -783,6 +1032,7 @@ int cfd_unet_loss_grad(cfd_unet *u, const float *x, const float *t, int32_t B, d
     const long no = (long)B * u->H * u->W * 4;
     int r = load_inputs(u, x, B);
     if (r == CFD_OK && hipMemcpy(u->target.p, t, no * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) r = CFD_ERR_HIP;
+    if (r == CFD_OK && u->ev[0]) (void)hipEventRecord(u->ev[0], 0);
     if (r == CFD_OK) r = forward(u, B);
     if (r == CFD_OK && hipMemset(u->loss.p, 0, sizeof(double)) != hipSuccess) r = CFD_ERR_HIP;
     if (r == CFD_OK) {
@@ -790,7 +1040,9 @@ int cfd_unet_loss_grad(cfd_unet *u, const float *x, const float *t, int32_t B, d
                            u->gout.p, u->loss.p);
         r = launched();
     }
+    if (r == CFD_OK && u->ev[1]) (void)hipEventRecord(u->ev[1], 0);
     if (r == CFD_OK) r = backward(u, B);
+    if (r == CFD_OK && u->ev[2]) (void)hipEventRecord(u->ev[2], 0);
     double l = 0.0;
     if (r == CFD_OK && hipMemcpy(&l, u->loss.p, sizeof l, hipMemcpyDeviceToHost) != hipSuccess) r = CFD_ERR_HIP;
     if (r == CFD_OK && grad_host &&
@@ -812,7 +1064,20 @@ int cfd_unet_train_step(cfd_unet *u, const float *x, const float *t, int32_t B, 
                        (long)u->nparam, lr, b1, b2, 1e-8f, c1, c2);
     r = launched();
     if (r == CFD_OK) r = refresh_weights(u);
+    if (r == CFD_OK && u->ev[3]) (void)hipEventRecord(u->ev[3], 0);
     return r == CFD_OK ? CFD_OK : cfd__fail(r, "U-Net update failed");
+}
+
+int cfd_unet_last_timing(cfd_unet *u, double ms[3])
+{
+    if (!u || !ms) return cfd__fail(CFD_ERR_ARG, "NULL argument");
+    if (!u->ev[3] || u->step == 0) return cfd__fail(CFD_ERR_STATE, "no training step timed yet");
+    float a = 0.f, b = 0.f, c = 0.f;
+    if (hipEventSynchronize(u->ev[3]) != hipSuccess || hipEventElapsedTime(&a, u->ev[0], u->ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&b, u->ev[1], u->ev[2]) != hipSuccess || hipEventElapsedTime(&c, u->ev[2], u->ev[3]) != hipSuccess)
+        return cfd__fail(CFD_ERR_HIP, "U-Net timing events");
+    ms[0] = a; ms[1] = b; ms[2] = c;
+    return CFD_OK;
 }
 
 }  // extern "C"

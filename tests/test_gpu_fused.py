@@ -259,3 +259,31 @@ def test_autotune_restores_state_and_keeps_parity():
     ctx2 = L.Context(prm, x, y, eps, lp)
     assert ctx2.get_tiles() == tiles
     ctx2.close()
+
+
+@pytest.mark.parametrize("waves", [1, 3])
+def test_pipeline_matches_single_stream_bitwise(waves):
+    """The two-stream pipeline (T/C kernel of step n beside the u/v/p kernel of step n+1)
+    runs the same kernels on the same data: eager steps, run_to_tol chunks and graph replays
+    agree bit for bit with the one-stream order."""
+    say("case: pipeline vs one stream, waves", waves)
+    prm, x, y, eps, lp, st = _multiblock_case(
+        140, 520, bc=BC_CHANNEL, invfr2=0.0, buoyancy=BUOY_BOUSSINESQ, ri=2.0, t_ref=0.5, slot_lo=0.4,
+        slot_hi=0.6, slot_value=1.0, mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3), t_bottom=1.0, t_top=0.0)
+    outs = []
+    for pipe in (0, 1):
+        ctx = L.Context(prm, x, y, eps, lp)
+        ctx.set_pipeline(pipe)
+        ctx.set_tiles(24, 12, waves, 1)
+        ctx.set_state(st)
+        res, hist = ctx.pseudo_steps(21, history=True)
+        code, iters, res2 = ctx.run_to_tol(1e-30, 30, check_every=7, raise_on_fail=False)
+        g = ctx.pseudo_steps_graph(8, 3)
+        fields = {k: ctx.download(k) for k in FIELDS}
+        ctx.close()
+        outs.append((hist, res2, g, fields))
+        say("pipe", pipe, "done")
+    (h0, r0, g0, f0), (h1, r1, g1, f1) = outs
+    assert np.array_equal(h0, h1) and np.array_equal(r0, r1) and np.array_equal(g0, g1)
+    for k in FIELDS:
+        assert np.array_equal(f0[k], f1[k]), (k, max_rel(f1[k], f0[k]))

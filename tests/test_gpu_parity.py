@@ -17,7 +17,7 @@ from cfd_amd import cases as C
 from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, COMPAT_CUDA, BUOY_BOUSSINESQ, MU_POLY4,
                             LAYOUT_FORTRAN)
 from helpers import load_golden, max_rel, field_rel, random_state, small_case
-from hipmem import DeviceArray
+from cfd_amd.hipmem import DeviceArray
 
 pytestmark = pytest.mark.gpu
 

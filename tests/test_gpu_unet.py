@@ -13,7 +13,7 @@ import pytest
 
 from cfd_amd import lib as L
 from oracle import unet_ref as U
-from hipmem import DeviceArray
+from cfd_amd.hipmem import DeviceArray
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,11 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-CONV_SHAPES = [(2, 8, 12, 5, 7), (1, 16, 16, 40, 70), (3, 6, 10, 96, 32), (1, 8, 8, 256, 130)]
+# the generic implicit-GEMM kernel (channel counts off the 32 grid) and the spatially tiled
+# one (Cin, Cout multiples of 32) with partial tiles at the image edges and images smaller
+# than one tile
+CONV_SHAPES = [(2, 8, 12, 5, 7), (1, 16, 16, 40, 70), (3, 6, 10, 96, 32), (1, 8, 8, 256, 130),
+               (2, 20, 36, 64, 128), (1, 33, 17, 32, 64), (2, 8, 8, 128, 256), (1, 16, 48, 192, 64)]
 
 
 @pytest.mark.parametrize("B,H,W,cin,cout", CONV_SHAPES)

@@ -64,3 +64,9 @@ def test_pool_upsample_and_bf16_closed_forms():
     assert U.bf16(1.0 + 2 ** -9) == 1.0                        # tie to even
     assert U.bf16(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
     assert U.bf16(-2.5) == -2.5
+
+
+def test_library_layer_table_matches_oracle():
+    """The package's layer table (used for FLOP counts) is the oracle's network."""
+    from cfd_amd import lib as L
+    assert [tuple(s) for s in L.UNET_LAYERS] == [tuple(s) for s in U.SPEC]
