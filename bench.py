@@ -459,8 +459,11 @@ def main():
     if os.path.exists(args.traffic):
         with open(args.traffic) as f:
             t = json.load(f)
+        tt = t.get("tiles") or {}
+        # the recorded figure is the u/v/p kernel's: it depends on that kernel's shape only
         if (t.get("imax") == prm.imax and t.get("jmax") == prm.jmax and t.get("kernel") == args.kernel
-                and t.get("tiles") == tiles and prec == PREC_FP64):
+                and tiles is not None and prec == PREC_FP64
+                and tt.get("rows_uvp") == tiles["rows_uvp"] and tt.get("waves_uvp") == tiles["waves_uvp"]):
             traffic = t.get("uvp_bytes_per_launch")
     if args.copy_probe:
         # plain copy kernels of known size (physical level -> pseudo-time level of u, v, T,
