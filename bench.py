@@ -364,6 +364,7 @@ def main():
     ap.add_argument("--tiles", default="",
                     help="rows_uvp,rows_tc[,waves_uvp,waves_tc]: set the kernel shapes instead of autotuning")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per kernel with events")
+    ap.add_argument("--pipeline", type=int, default=1, help="two-stream pipelined pseudo-steps (1) or one stream (0)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
     ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart", "unet", "fp32steady"],
@@ -407,6 +408,7 @@ def main():
     log("workload %s %dx%d %s, building context" % (cfg["name"], prm.imax, prm.jmax, args.precision))
     ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], precision=prec)
     ctx.set_kernel(args.kernel)
+    ctx.set_pipeline(args.pipeline)
     ctx.set_state(cfg["state"])
     tiles = None
     if args.tiles:
@@ -480,7 +482,7 @@ def main():
         "data": "synthetic (seeded SplitMix64 inputs: tanh mesh, porous block, perturbed state)",
         "config": {"workload": "%s fused pseudo-step, %s" % (cfg["name"], args.precision),
                    "imax": prm.imax, "jmax": prm.jmax, "kernel": args.kernel,
-                   "graph_steps": args.graph, "tiles": tiles,
+                   "graph_steps": args.graph, "tiles": tiles, "pipeline": args.pipeline,
                    "steps_fused": n_fused, "steps_exact": n_exact},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

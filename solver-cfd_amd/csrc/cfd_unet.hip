@@ -216,26 +216,35 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_t(const bf16 *__restrict__ x
 // spatially tiled weight gradient:
 //   dW[n][tap][ci] = sum over pixels p of dy[p][n] * x[p + offset(tap)][ci]
 // The reduction runs over pixels, which are strided in NHWC, so the block stages both
-// operands transposed in LDS: dy of a 16 x 16 pixel tile as dyT[n][pixel], and the halo tile
-// of x as three copies shifted by the tap column (xs[dx][ci][row][col] = x(row, col + dx)),
-// which keeps every MFMA operand read an aligned 16 B load. Block (tile group, ci chunk,
+// operands transposed in LDS: dy of a 16 x 16 pixel tile as dyT[n][pixel], and the 18 x 18
+// halo tile of x as xs[ci][halo pixel]. A tap's operand is then 8 consecutive halo pixels
+// from column 8h + dx of a halo row: a 16 B LDS read at a 2 B-aligned address, which gfx950
+// serves in one ds_read_b128 (unaligned LDS access). Block (tile group, ci chunk,
 // n block): the four waves take four pixel rows each for all nine taps and loop over their
 // group's tiles, the next tile's loads in flight during the current tile's MFMAs; the waves'
 // sums meet in LDS and the block writes them to its own slice of a partial buffer, which
 // k_wsum reduces over the groups (no atomics on the gradient).
 // ---------------------------------------------------------------------------------------
 constexpr int PPD = TT * TT + 8;            // dyT pitch (bf16): 528 B, conflict-free 16 B reads
-constexpr int PPX = TH2 * TT + 8;           // xs pitch per channel (bf16): 592 B
+constexpr int PPX = TH2 * TH2 + 4;          // xs pitch per channel (bf16): 656 B = 164 dwords
 
 struct __align__(16) WgradSmem {
     union {
         struct {
             bf16 dyT[32][PPD];
-            bf16 xs[3][CK][PPX];
+            bf16 xs[CK][PPX];
         } t;
         float red[9][32][32];
     };
 };
+
+// 16 B from LDS at a 2 B-aligned address
+__device__ __forceinline__ bf16x8 lds_read8_unaligned(const bf16 *p)
+{
+    bf16x8 v;
+    __builtin_memcpy(&v, __builtin_assume_aligned(p, 2), 16);
+    return v;
+}
 
 template <bool RAGGED>
 __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
@@ -292,17 +301,10 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restri
         for (int i = 0; i < RX; ++i) {
             const int pix = 64 * i + lane;
             if (pix >= NXP) continue;
-            const int hr = pix / TH2, hc = pix - hr * TH2;
             union { uint4 v; bf16 e[8]; } u;
             u.v = rx[i];
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const int c = hc - dx;
-                if (c >= 0 && c < TT) {
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) sm.t.xs[dx][8 * qd + q][hr * TT + c] = u.e[q];
-                }
-            }
+            for (int q = 0; q < 8; ++q) sm.t.xs[8 * qd + q][pix] = u.e[q];
         }
     };
 
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restri
 #pragma unroll
             for (int tap = 0; tap < 9; ++tap) {
                 const int dyy = tap / 3, dxx = tap - 3 * (tap / 3);
-                const bf16x8 bb = *(const bf16x8 *)&sm.t.xs[dxx][l32][(r + dyy) * TT + 8 * half];
+                const bf16x8 bb = lds_read8_unaligned(&sm.t.xs[l32][(r + dyy) * TH2 + 8 * half + dxx]);
                 acc[tap] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[tap], 0, 0, 0);
             }
         }
@@ -362,10 +364,35 @@ __global__ void k_wsum(const float *__restrict__ part, int G, int gper, long tot
 // ---------------------------------------------------------------------------------------
 // elementwise and data-movement kernels
 // ---------------------------------------------------------------------------------------
-__global__ void k_relu_bwd(bf16 *g, const bf16 *y, long n)          // g *= (y > 0)
+// G channels per thread: 8 (one 16 B vector of BF16, two of FP32) when the channel counts
+// and offsets are multiples of 8, else 1. Index math is 32-bit: activation tensors are
+// limited to 2^31 elements (checked by the callers).
+template <int G>
+struct VB {                                      // G BF16 values
+    union { uint4 v; bf16 e[8]; } u;
+    __device__ __forceinline__ void load(const bf16 *p)
+    {
+        if (G == 8) u.v = *(const uint4 *)p;
+        else u.e[0] = *p;
+    }
+    __device__ __forceinline__ void store(bf16 *p) const
+    {
+        if (G == 8) *(uint4 *)p = u.v;
+        else *p = u.e[0];
+    }
+};
+
+template <int G>
+__global__ void k_relu_bwd(bf16 *g, const bf16 *y, int n)          // g *= (y > 0)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        if (!(bf2f(y[i]) > 0.f)) g[i] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n / G; i += gridDim.x * blockDim.x) {
+        VB<G> a, m;
+        a.load(g + G * i); m.load(y + G * i);
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (!(bf2f(m.u.e[k]) > 0.f)) a.u.e[k] = 0;
+        a.store(g + G * i);
+    }
 }
 
 // db[c] += sum over m of g[m][c], C a power of two <= 2048: each thread reads 16 B vectors
@@ -396,93 +423,137 @@ __global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ g, l
         if (red[c] != 0.f) atomicAdd(&db[c], red[c]);
 }
 
+template <int G>
 __global__ void k_maxpool2(const bf16 *x, int B, int H, int W, int C, bf16 *y)
 {
-    const long n = (long)B * (H / 2) * (W / 2) * C;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        long p = i / C;
-        const int wo = (int)(p % (W / 2)); p /= (W / 2);
-        const int ho = (int)(p % (H / 2));
-        const int b = (int)(p / (H / 2));
-        const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + c;
-        float m = bf2f(x[base]);
-        m = fmaxf(m, bf2f(x[base + C]));
-        m = fmaxf(m, bf2f(x[base + (size_t)W * C]));
-        m = fmaxf(m, bf2f(x[base + (size_t)W * C + C]));
-        y[i] = f2bf(m);
+    const int Ho = H / 2, Wo = W / 2, CG = C / G, n = B * Ho * Wo * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int cg = i % CG;
+        int p = i / CG;
+        const int wo = p % Wo; p /= Wo;
+        const int ho = p % Ho, b = p / Ho;
+        const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + G * cg;
+        VB<G> a0, a1, a2, a3, o;
+        a0.load(x + base); a1.load(x + base + C); a2.load(x + base + (size_t)W * C); a3.load(x + base + (size_t)W * C + C);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            float m = bf2f(a0.u.e[k]);
+            m = fmaxf(m, bf2f(a1.u.e[k]));
+            m = fmaxf(m, bf2f(a2.u.e[k]));
+            m = fmaxf(m, bf2f(a3.u.e[k]));
+            o.u.e[k] = f2bf(m);
+        }
+        o.store(y + (size_t)G * i);
     }
 }
 
 // gradient to the first of the four inputs that holds the maximum (row-major order)
+template <int G>
 __global__ void k_maxpool2_bwd(const bf16 *x, const bf16 *y, const bf16 *gy, int B, int H, int W, int C, float *gx)
 {
-    const long n = (long)B * (H / 2) * (W / 2) * C;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        long p = i / C;
-        const int wo = (int)(p % (W / 2)); p /= (W / 2);
-        const int ho = (int)(p % (H / 2));
-        const int b = (int)(p / (H / 2));
-        const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + c;
+    const int Ho = H / 2, Wo = W / 2, CG = C / G, n = B * Ho * Wo * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int cg = i % CG;
+        int p = i / CG;
+        const int wo = p % Wo; p /= Wo;
+        const int ho = p % Ho, b = p / Ho;
+        const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + G * cg;
         const size_t off[4] = {0, (size_t)C, (size_t)W * C, (size_t)W * C + C};
-        const float mv = bf2f(y[i]);           // compared as values: -0 equals +0
-        for (int q = 0; q < 4; ++q)
-            if (bf2f(x[base + off[q]]) == mv) { gx[base + off[q]] += bf2f(gy[i]); break; }
+        VB<G> a[4], m, gg;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q].load(x + base + off[q]);
+        m.load(y + (size_t)G * i); gg.load(gy + (size_t)G * i);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const float mv = bf2f(m.u.e[k]);           // compared as values: -0 equals +0
+            for (int q = 0; q < 4; ++q)
+                if (bf2f(a[q].u.e[k]) == mv) { gx[base + off[q] + k] += bf2f(gg.u.e[k]); break; }
+        }
     }
 }
 
+// x: [B][H][W][C] -> y[B][2H][2W][Cy] channels cofs..cofs+C-1
+template <int G>
 __global__ void k_upsample2(const bf16 *x, int B, int H, int W, int C, bf16 *y, int Cy, int cofs)
 {
-    // x: [B][H][W][C] -> y[B][2H][2W][Cy] channels cofs..cofs+C-1
-    const long n = (long)B * (2 * H) * (2 * W) * C;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        long p = i / C;
-        const int wo = (int)(p % (2 * W)); p /= (2 * W);
-        const int ho = (int)(p % (2 * H));
-        const int b = (int)(p / (2 * H));
-        y[((size_t)(b * 2 * H + ho) * 2 * W + wo) * Cy + cofs + c] = x[((size_t)(b * H + ho / 2) * W + wo / 2) * C + c];
+    const int CG = C / G, n = B * (2 * H) * (2 * W) * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int cg = i % CG;
+        int p = i / CG;
+        const int wo = p % (2 * W); p /= 2 * W;
+        const int ho = p % (2 * H), b = p / (2 * H);
+        VB<G> a;
+        a.load(x + ((size_t)(b * H + ho / 2) * W + wo / 2) * C + G * cg);
+        a.store(y + ((size_t)(b * 2 * H + ho) * 2 * W + wo) * Cy + cofs + G * cg);
     }
 }
 
-// gx[B][H][W][C] = sum over the 2x2 children of gy channels cofs.. (gy: [B][2H][2W][Cy])
+// gx[B][H][W][C] += sum over the 2x2 children of gy channels cofs.. (gy: [B][2H][2W][Cy])
+template <int G>
 __global__ void k_upsample2_bwd(const bf16 *gy, int B, int H, int W, int C, int Cy, int cofs, float *gx)
 {
-    const long n = (long)B * H * W * C;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        long p = i / C;
-        const int w = (int)(p % W); p /= W;
-        const int h = (int)(p % H);
-        const int b = (int)(p / H);
-        float s = 0.f;
+    const int CG = C / G, n = B * H * W * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int cg = i % CG;
+        int p = i / CG;
+        const int w = p % W; p /= W;
+        const int h = p % H, b = p / H;
+        float s[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) s[k] = 0.f;
         for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx)
-                s += bf2f(gy[((size_t)(b * 2 * H + 2 * h + dy) * 2 * W + 2 * w + dx) * Cy + cofs + c]);
-        gx[i] += s;
+            for (int dx = 0; dx < 2; ++dx) {
+                VB<G> a;
+                a.load(gy + ((size_t)(b * 2 * H + 2 * h + dy) * 2 * W + 2 * w + dx) * Cy + cofs + G * cg);
+#pragma unroll
+                for (int k = 0; k < G; ++k) s[k] += bf2f(a.u.e[k]);
+            }
+        float *o = gx + (size_t)G * i;
+#pragma unroll
+        for (int k = 0; k < G; ++k) o[k] += s[k];
     }
 }
 
 // copy channels: y[..][cofs + c] = x[..][c], for npix pixels
-__global__ void k_chan_copy(const bf16 *x, long npix, int C, bf16 *y, int Cy, int cofs)
+template <int G>
+__global__ void k_chan_copy(const bf16 *x, int npix, int C, bf16 *y, int Cy, int cofs)
 {
-    const long n = npix * C;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        y[(i / C) * Cy + cofs + (i % C)] = x[i];
+    const int CG = C / G, n = npix * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int p = i / CG, cg = i % CG;
+        VB<G> a;
+        a.load(x + (size_t)G * i);
+        a.store(y + (size_t)p * Cy + cofs + G * cg);
+    }
 }
 
 // gx[..][c] += g[..][cofs + c]
-__global__ void k_chan_add(const bf16 *g, long npix, int C, int Cg, int cofs, float *gx)
+template <int G>
+__global__ void k_chan_add(const bf16 *g, int npix, int C, int Cg, int cofs, float *gx)
 {
-    const long n = npix * C;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        gx[i] += bf2f(g[(i / C) * Cg + cofs + (i % C)]);
+    const int CG = C / G, n = npix * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int p = i / CG, cg = i % CG;
+        VB<G> a;
+        a.load(g + (size_t)p * Cg + cofs + G * cg);
+        float *o = gx + (size_t)G * i;
+#pragma unroll
+        for (int k = 0; k < G; ++k) o[k] += bf2f(a.u.e[k]);
+    }
 }
 
-__global__ void k_f32_to_bf16(const float *x, long n, bf16 *y)
+// y[i] = bf16(x[i]) for i in [i0, n): vectors of 8 from i0 (a multiple of 8) when G == 8
+template <int G>
+__global__ void k_f32_to_bf16(const float *x, long i0, long n, bf16 *y)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = f2bf(x[i]);
+    const long nv = (n - i0) / G;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (long)gridDim.x * blockDim.x) {
+        const long e = i0 + G * i;
+        VB<G> o;
+#pragma unroll
+        for (int k = 0; k < G; ++k) o.u.e[k] = f2bf(x[e + k]);
+        o.store(y + e);
+    }
 }
 
 __global__ void k_fill_f32(float *x, long n, float v)
@@ -623,30 +694,47 @@ int cfd_nn_flip_weights(const float *w, int32_t Cin, int32_t Cout, uint16_t *wf)
     return launched();
 }
 
+#define ELEMS_OK(n, what)                                                                                  \
+    if ((n) <= 0 || (n) >= 2147483648L) return cfd__fail(CFD_ERR_ARG, what ": %ld elements (1 .. 2^31-1)", (long)(n))
+
 int cfd_nn_maxpool2(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t *y)
 {
-    hipLaunchKernelGGL(k_maxpool2, dim3(eb((long)B * H * W * C / 4)), dim3(256), 0, 0, x, B, H, W, C, y);
+    const long n = (long)B * H * W * C;
+    ELEMS_OK(n, "cfd_nn_maxpool2");
+    if (C % 8 == 0) hipLaunchKernelGGL(k_maxpool2<8>, dim3(eb(n / 32)), dim3(256), 0, 0, x, B, H, W, C, y);
+    else hipLaunchKernelGGL(k_maxpool2<1>, dim3(eb(n / 4)), dim3(256), 0, 0, x, B, H, W, C, y);
     return launched();
 }
 
 int cfd_nn_maxpool2_bwd(const uint16_t *x, const uint16_t *y, const uint16_t *gy, int32_t B, int32_t H, int32_t W,
                         int32_t C, float *gx)
 {
-    hipLaunchKernelGGL(k_maxpool2_bwd, dim3(eb((long)B * H * W * C / 4)), dim3(256), 0, 0, x, y, gy, B, H, W, C, gx);
+    const long n = (long)B * H * W * C;
+    ELEMS_OK(n, "cfd_nn_maxpool2_bwd");
+    if (C % 8 == 0) hipLaunchKernelGGL(k_maxpool2_bwd<8>, dim3(eb(n / 32)), dim3(256), 0, 0, x, y, gy, B, H, W, C, gx);
+    else hipLaunchKernelGGL(k_maxpool2_bwd<1>, dim3(eb(n / 4)), dim3(256), 0, 0, x, y, gy, B, H, W, C, gx);
     return launched();
 }
 
 int cfd_nn_upsample2(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t *y, int32_t Cy,
                      int32_t cofs)
 {
-    hipLaunchKernelGGL(k_upsample2, dim3(eb((long)B * H * W * C * 4)), dim3(256), 0, 0, x, B, H, W, C, y, Cy, cofs);
+    const long n = (long)B * 4 * H * W * Cy;
+    ELEMS_OK(n, "cfd_nn_upsample2");
+    if (C % 8 == 0 && Cy % 8 == 0 && cofs % 8 == 0)
+        hipLaunchKernelGGL(k_upsample2<8>, dim3(eb(n / 8)), dim3(256), 0, 0, x, B, H, W, C, y, Cy, cofs);
+    else hipLaunchKernelGGL(k_upsample2<1>, dim3(eb(n)), dim3(256), 0, 0, x, B, H, W, C, y, Cy, cofs);
     return launched();
 }
 
 int cfd_nn_upsample2_bwd(const uint16_t *gy, int32_t B, int32_t H, int32_t W, int32_t C, int32_t Cy, int32_t cofs,
                          float *gx)
 {
-    hipLaunchKernelGGL(k_upsample2_bwd, dim3(eb((long)B * H * W * C)), dim3(256), 0, 0, gy, B, H, W, C, Cy, cofs, gx);
+    const long n = (long)B * 4 * H * W * Cy;
+    ELEMS_OK(n, "cfd_nn_upsample2_bwd");
+    if (C % 8 == 0 && Cy % 8 == 0 && cofs % 8 == 0)
+        hipLaunchKernelGGL(k_upsample2_bwd<8>, dim3(eb(n / 32)), dim3(256), 0, 0, gy, B, H, W, C, Cy, cofs, gx);
+    else hipLaunchKernelGGL(k_upsample2_bwd<1>, dim3(eb(n / 4)), dim3(256), 0, 0, gy, B, H, W, C, Cy, cofs, gx);
     return launched();
 }
 
@@ -771,8 +859,9 @@ int forward(cfd_unet *u, int B)
         const Shape sl = act_shape(u, low), ss = act_shape(u, skip), sc = act_shape(u, cat);
         int rr = cfd_nn_upsample2(u->act[low].p, B, sl.h, sl.w, sl.c, u->act[cat].p, sc.c, 0);
         if (rr == CFD_OK) {
-            hipLaunchKernelGGL(k_chan_copy, dim3(eb((long)B * ss.h * ss.w * ss.c)), dim3(256), 0, 0, u->act[skip].p,
-                               (long)B * ss.h * ss.w, ss.c, u->act[cat].p, sc.c, sl.c);
+            const int npix = B * ss.h * ss.w;      // channel counts 32..256, offsets 64..256: multiples of 8
+            hipLaunchKernelGGL(k_chan_copy<8>, dim3(eb((long)npix * ss.c / 8)), dim3(256), 0, 0, u->act[skip].p, npix,
+                               ss.c, u->act[cat].p, sc.c, sl.c);
             rr = launched();
         }
         return rr;
@@ -822,14 +911,23 @@ int conv_bwd(cfd_unet *u, int q, int B, int ain, const bf16 *gy)
 
 int relu_mask(cfd_unet *u, int a, long n)
 {
-    hipLaunchKernelGGL(k_relu_bwd, dim3(eb(n)), dim3(256), 0, 0, u->gbuf[a].p, u->act[a].p, n);
+    // every masked activation has 32..256 channels: n is a multiple of 8
+    hipLaunchKernelGGL(k_relu_bwd<8>, dim3(eb(n / 8)), dim3(256), 0, 0, u->gbuf[a].p, u->act[a].p, (int)n);
+    return launched();
+}
+
+// y = bf16(x) over n values: 16 B vectors, then the tail
+int f32_to_bf16(const float *x, long n, bf16 *y)
+{
+    const long n8 = n & ~7L;
+    if (n8) hipLaunchKernelGGL(k_f32_to_bf16<8>, dim3(eb(n8 / 8)), dim3(256), 0, 0, x, 0L, n8, y);
+    if (n > n8) hipLaunchKernelGGL(k_f32_to_bf16<1>, dim3(1), dim3(256), 0, 0, x, n8, n, y);
     return launched();
 }
 
 int to_bf(cfd_unet *u, int a, long n)
 {
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(eb(n)), dim3(256), 0, 0, u->gacc[a].p, n, u->gbuf[a].p);
-    return launched();
+    return f32_to_bf16(u->gacc[a].p, n, u->gbuf[a].p);
 }
 
 long numel(cfd_unet *u, int a, int B)
@@ -856,8 +954,9 @@ int backward(cfd_unet *u, int B)
         const Shape sl = act_shape(u, low), ss = act_shape(u, skip), sc = act_shape(u, cat);
         int rr = cfd_nn_upsample2_bwd(u->gbuf[cat].p, B, sl.h, sl.w, sl.c, sc.c, 0, u->gacc[low].p);
         if (rr == CFD_OK) {
-            hipLaunchKernelGGL(k_chan_add, dim3(eb((long)B * ss.h * ss.w * ss.c)), dim3(256), 0, 0, u->gbuf[cat].p,
-                               (long)B * ss.h * ss.w, ss.c, sc.c, sl.c, u->gacc[skip].p);
+            const int npix = B * ss.h * ss.w;
+            hipLaunchKernelGGL(k_chan_add<8>, dim3(eb((long)npix * ss.c / 8)), dim3(256), 0, 0, u->gbuf[cat].p, npix,
+                               ss.c, sc.c, sl.c, u->gacc[skip].p);
             rr = launched();
         }
         return rr;
@@ -892,8 +991,7 @@ int backward(cfd_unet *u, int B)
 
 int refresh_weights(cfd_unet *u)
 {
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(eb((long)u->nparam)), dim3(256), 0, 0, u->w32.p, (long)u->nparam, u->wb.p);
-    int r = launched();
+    int r = f32_to_bf16(u->w32.p, (long)u->nparam, u->wb.p);
     for (int q = 0; q < 15 && r == CFD_OK; ++q) {
         const Conv &c = u->convs[q];
         r = cfd_nn_flip_weights(u->w32.p + c.woff, c.cin, c.cout, u->wf.p + u->wf_off[q]);
@@ -905,8 +1003,7 @@ int load_inputs(cfd_unet *u, const float *x, int B)
 {
     const long n = (long)B * u->H * u->W * 5;
     if (hipMemcpy(u->xin32.p, x, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return CFD_ERR_HIP;
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(eb(n)), dim3(256), 0, 0, u->xin32.p, n, u->act[A_IN].p);
-    return launched();
+    return f32_to_bf16(u->xin32.p, n, u->act[A_IN].p);
 }
 
 uint64_t sm64(uint64_t *s)
@@ -928,6 +1025,9 @@ int cfd_unet_create(int32_t height, int32_t width, int32_t batch, uint64_t seed,
     if (height < 8 || width < 8 || height % 8 || width % 8 || batch < 1)
         return cfd__fail(CFD_ERR_ARG, "U-Net grid must be a multiple of 8 (>= 8) and batch >= 1, got %dx%d, %d",
                          height, width, batch);
+    // the widest activation (96 channels at full resolution) must index in 32 bits
+    if ((long)batch * height * width * 96 >= 2147483648L)
+        return cfd__fail(CFD_ERR_ARG, "U-Net batch x grid too large: %d x %dx%d", batch, height, width);
     cfd_unet *u = new cfd_unet();
     u->H = height; u->W = width; u->Bmax = batch;
     build_convs(u);
