@@ -220,22 +220,19 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_t(const bf16 *__restrict__ x
 // halo tile of x as xs[ci][halo pixel]. A tap's operand is then 8 consecutive halo pixels
 // from column 8h + dx of a halo row: a 16 B LDS read at a 2 B-aligned address, which gfx950
 // serves in one ds_read_b128 (unaligned LDS access). Block (tile group, ci chunk,
-// n block): the four waves take four pixel rows each for all nine taps and loop over their
-// group's tiles, the next tile's loads in flight during the current tile's MFMAs; the waves'
-// sums meet in LDS and the block writes them to its own slice of a partial buffer, which
-// k_wsum reduces over the groups (no atomics on the gradient).
+// n block): wave w owns taps w, w+4 (and wave 0 tap 8) over all pixels of its group's tiles,
+// the next tile's loads in flight during the current tile's MFMAs; the waves' outputs are
+// disjoint, so each writes its sums straight to the block's slice of a partial buffer, which
+// k_wsum reduces over the groups (no atomics, in LDS or on the gradient).
 // ---------------------------------------------------------------------------------------
 constexpr int PPD = TT * TT + 8;            // dyT pitch (bf16): 528 B, conflict-free 16 B reads
 constexpr int PPX = TH2 * TH2 + 4;          // xs pitch per channel (bf16): 656 B = 164 dwords
 
 struct __align__(16) WgradSmem {
-    union {
-        struct {
-            bf16 dyT[32][PPD];
-            bf16 xs[CK][PPX];
-        } t;
-        float red[9][32][32];
-    };
+    struct {
+        bf16 dyT[32][PPD];
+        bf16 xs[CK][PPX];
+    } t;
 };
 
 // 16 B from LDS at a 2 B-aligned address
@@ -247,7 +244,7 @@ __device__ __forceinline__ bf16x8 lds_read8_unaligned(const bf16 *p)
 }
 
 template <bool RAGGED>
-__global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
+__global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
                                                             int B, int H, int W, int Cin, int Cout,
                                                             float *__restrict__ part)
 {
@@ -257,9 +254,12 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restri
     const int ci0 = blockIdx.y * CK, n0 = blockIdx.z * 32;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int half = lane >> 5, l32 = lane & 31;
-    f32x16 acc[9];
+    // wave w owns taps w, w + 4 and (wave 0 only) 8, over all 16 pixel rows of each tile:
+    // the waves' outputs are disjoint, so nothing is reduced across waves
+    const int ntap = wv == 0 ? 3 : 2;
+    f32x16 acc[3];
 #pragma unroll
-    for (int q = 0; q < 9; ++q)
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[q][e] = 0.f;
 
@@ -314,37 +314,34 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_t(const bf16 *__restri
         stash();
         __syncthreads();
         if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const int r = 4 * wv + ks;      // tile row: pixels 16 r .. 16 r + 15 are this k step
+#pragma unroll 4
+        for (int r = 0; r < TT; ++r) {      // tile row r: pixels 16 r .. 16 r + 15 are this k step
             const bf16x8 a = *(const bf16x8 *)&sm.t.dyT[l32][16 * r + 8 * half];
 #pragma unroll
-            for (int tap = 0; tap < 9; ++tap) {
-                const int dyy = tap / 3, dxx = tap - 3 * (tap / 3);
-                const bf16x8 bb = lds_read8_unaligned(&sm.t.xs[l32][(r + dyy) * TH2 + 8 * half + dxx]);
-                acc[tap] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[tap], 0, 0, 0);
+            for (int q = 0; q < 3; ++q) {
+                if (q < ntap) {
+                    const int tap = wv + 4 * q, dyy = tap / 3, dxx = tap - 3 * (tap / 3);
+                    const bf16x8 bb = lds_read8_unaligned(&sm.t.xs[l32][(r + dyy) * TH2 + 8 * half + dxx]);
+                    acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[q], 0, 0, 0);
+                }
             }
         }
     }
-    // sum the four waves' partials in LDS; the block's sums go to its own slice of the
-    // partial buffer part[blockIdx.x][Cout][9 Cin] (reduced over the tile groups later)
-    __syncthreads();
-    for (int v = t; v < 9 * 32 * 32; v += 256) (&sm.red[0][0][0])[v] = 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int n = 8 * (e >> 2) + 4 * half + (e & 3);      // D row (A row = cout)
-            atomicAdd(&sm.red[tap][n][l32], acc[tap][e]);
-        }
-    __syncthreads();
+    // the block's sums go to its own slice of the partial buffer part[blockIdx.x][Cout][9 Cin]
+    // (reduced over the tile groups later); D rows are couts, columns input channels
     const size_t K = (size_t)9 * Cin;
     float *out = part + (size_t)blockIdx.x * K * Cout;
-    for (int v = t; v < 9 * 32 * 32; v += 256) {
-        const int tap = v >> 10, n = (v >> 5) & 31, ci = v & 31;
-        if (RAGGED && (n0 + n >= Cout || ci0 + ci >= Cin)) continue;
-        out[(size_t)(n0 + n) * K + (size_t)tap * Cin + ci0 + ci] = sm.red[tap][n][ci];
+    const int ci = ci0 + l32;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (q >= ntap) continue;
+        const int tap = wv + 4 * q;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int n = n0 + 8 * (e >> 2) + 4 * half + (e & 3);
+            if (RAGGED && (n >= Cout || ci >= Cin)) continue;
+            out[(size_t)n * K + (size_t)tap * Cin + ci] = acc[q][e];
+        }
     }
 }
 
@@ -664,10 +661,10 @@ int cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32
     if (B < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return cfd__fail(CFD_ERR_ARG, "cfd_nn_conv3x3_wgrad: bad shape");
     const long total = 9L * Cin * Cout;
     const bool ragged = (Cin % CK) != 0 || (Cout % 32) != 0;
-    // tile groups so that the grid holds about 1024 blocks (two rounds of two per CU)
+    // tile groups so that the grid holds about 1536 blocks (two rounds of three per CU)
     const int ntiles = B * cdiv(H, TT) * cdiv(W, TT);
     const int per = cdiv(Cin, CK) * cdiv(Cout, 32);
-    int G = cdiv(1024, per);
+    int G = cdiv(1536, per);
     if (G > ntiles) G = ntiles;
     if (G < 1) G = 1;
     float *part = workspace((size_t)G * total);
