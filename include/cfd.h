@@ -217,7 +217,8 @@ typedef struct cfd_tiles {
                             2 = two waves per SIMD (excess registers spilled to scratch),
                             3 = u and v chains in separate waves of a 512-lane workgroup */
     int32_t waves_tc;    /* register budget of the T/C kernel: 0|1 = as compiled (3 waves
-                            per SIMD), 4 = four waves per SIMD (with spills)              */
+                            per SIMD), 4 = four waves per SIMD (with spills),
+                            2 = T and C chains in separate waves of a 512-lane workgroup */
 } cfd_tiles;
 
 int  cfd_ctx_set_tiles(cfd_ctx *ctx, const cfd_tiles *tiles);

@@ -323,7 +323,8 @@ int cfd_batch_set_tiles(cfd_batch *b, const cfd_tiles *t)
     if (!b || !t) return cfd__fail(CFD_ERR_ARG, "NULL argument");
     if (t->rows_uvp < 0 || t->rows_tc < 0) return cfd__fail(CFD_ERR_ARG, "row-block heights must be >= 0");
     if (!(t->waves_uvp >= 0 && t->waves_uvp <= 3)) return cfd__fail(CFD_ERR_ARG, "waves_uvp must be 0, 1, 2 or 3");
-    if (!(t->waves_tc == 0 || t->waves_tc == 1 || t->waves_tc == 4)) return cfd__fail(CFD_ERR_ARG, "waves_tc must be 0, 1 or 4");
+    if (!(t->waves_tc == 0 || t->waves_tc == 1 || t->waves_tc == 2 || t->waves_tc == 4))
+        return cfd__fail(CFD_ERR_ARG, "waves_tc must be 0, 1, 2 or 4");
     b->tiles = *t;
     b->args_dirty = 1;
     return CFD_OK;
@@ -436,7 +437,7 @@ static int make_launch(cfd_batch *b, cfd_batch_launch *L, double tol, int64_t it
     memset(L, 0, sizeof *L);
     L->prec = b->prec; L->steady = b->steady; L->mupoly = b->mupoly;
     L->occ1 = b->tiles.waves_uvp >= 2 ? b->tiles.waves_uvp : 1;
-    L->occ2 = b->tiles.waves_tc == 4 ? 4 : 1;
+    L->occ2 = (b->tiles.waves_tc == 2 || b->tiles.waves_tc == 4) ? b->tiles.waves_tc : 1;
     L->ncases = b->n; L->max_mat = b->max_mat;
     L->k1 = b->d_k1; L->k2 = b->d_k2; L->mat = b->d_mat; L->tiles1 = b->d_t1; L->tiles2 = b->d_t2;
     L->ntiles1 = b->ntiles1; L->ntiles2 = b->ntiles2;
@@ -609,7 +610,8 @@ int cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *out_
                                       {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
                                       {16, 0, 3, 0}, {32, 0, 3, 0}, {48, 0, 3, 0}, {64, 0, 3, 0}};
     static const cfd_tiles cand2[] = {{0, 8, 0, 1}, {0, 16, 0, 1}, {0, 24, 0, 1}, {0, 32, 0, 1}, {0, 48, 0, 1},
-                                      {0, 8, 0, 4}, {0, 16, 0, 4}, {0, 24, 0, 4}, {0, 32, 0, 4}};
+                                      {0, 8, 0, 4}, {0, 16, 0, 4}, {0, 24, 0, 4}, {0, 32, 0, 4},
+                                      {0, 8, 0, 2}, {0, 16, 0, 2}, {0, 32, 0, 2}, {0, 48, 0, 2}};
     void *snap = NULL;
     BHIP(hipStreamSynchronize(b->s));
     BHIP(hipMalloc(&snap, b->state_bytes));

@@ -972,6 +972,169 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// K2, split chains: T in waves 0-3, C in waves 4-7 of a 512-lane workgroup over the same 256
+// columns. T and C do not interact, so each chain keeps only its own windows (about half
+// the registers of scal_tile: more waves per SIMD, and room beside K1 in the two-stream
+// step). Both chains read u, v, eps and L; the T chain publishes the v row for the j+1
+// neighbour. Same expressions as scal_tile.
+// ---------------------------------------------------------------------------------------
+constexpr int NT2S = 2 * NT2;
+
+template <typename R, bool STEADY>
+__device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int bj, Smem2<R> &sm)
+{
+    const int im = a.imax, jm = a.jmax;
+    const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);
+    const int jlo = 2 + bj * W2, jhi = min(jlo + W2, jm);
+    const int x = threadIdx.x & (NT2 - 1);
+    const bool isC = threadIdx.x >= NT2;
+    const int j = jlo - HL2 + x;
+    const bool chan = a.chan != 0;
+    const R dt = R(a.dt), dtau = R(a.dtau), kt = R(a.kt), kc = R(a.kc);
+    const R tinf = R(a.tinf), tb = R(a.t_bottom), ttop = R(a.t_top);
+    const int RR = im + 2, Cc = jm + 2;
+    const double *pr = a.pk, *pc = a.pk + 4 * RR;
+    auto sTT = sm.sTT; auto sCT = sm.sCT; auto sV = sm.sV;
+    auto sZ1 = sm.sZ1; auto sZ2 = sm.sZ2; auto sC1 = sm.sC1; auto sC2 = sm.sC2;
+
+    for (int k = threadIdx.x; k < LROW2; k += NT2S) {
+        for (int q = 0; q < 3; ++q) { sTT[q][k] = R(0); sCT[q][k] = R(0); }
+        for (int q = 0; q < 2; ++q) { sV[q][k] = R(0); sZ1[q][k] = R(0); sZ2[q][k] = R(0); sC1[q][k] = R(0); sC2[q][k] = R(0); }
+    }
+    const bool jvalid = j >= 1 && j <= jm;
+    const int jc = min(max(j, 2), jm - 1);
+    const R aue_j = R(pc[min(max(j, 1), jm)]);
+    const R rdy_j = R(pc[2 * Cc + jc]), rdy_n = R(pc[2 * Cc + jc + 1]);
+    const R rdym_n = R(pc[3 * Cc + jc + 1]);
+    const bool scol = j >= 2 && j <= jm - 1;
+    const bool out_col = j >= jlo && j < jhi;
+    const double yj = a.y[min(max(j, 1), jm)];
+    const R cslot = chan ? ((yj >= a.slot_lo && yj <= a.slot_hi) ? R(a.slot_value) : R(0)) : R(0);
+    const R kap = isC ? kc : kt;
+    R *const outp = isC ? a.Cn : a.Tn;
+    auto own_ring = isC ? sCT : sTT;
+    auto ring1 = isC ? sC1 : sZ1;
+    auto ring2 = isC ? sC2 : sZ2;
+
+    Win<4, R> zt, uw, vw, vnw, id, oml, dZ;
+    Win<3, R> y1, y2, y3;
+    zt.fill(R(0)); uw.fill(R(0)); vw.fill(R(0)); vnw.fill(R(0)); id.fill(R(1)); oml.fill(R(1));
+    if (!STEADY) dZ.fill(R(0));
+    y1.fill(R(0)); y2.fill(R(0)); y3.fill(R(0));
+
+    const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
+    R q_t, q_u, q_v, q_Z, q_e, q_L;
+    auto fetch = [&](int t) {
+        q_t = R(0); q_u = R(0); q_v = R(0); q_Z = R(0); q_e = R(1); q_L = R(0);
+        if (jvalid && t >= 1 && t <= im + 1) {
+            if (t <= im) {
+                const size_t k = (size_t)t * su + j;
+                q_t = isC ? a.Ct[k] : a.Tt[k];
+                q_e = a.eps[k]; q_L = a.lp[k];
+                if (!STEADY) q_Z = isC ? a.C[k] : a.T[k];
+                if (j == 2) q_v = chan ? R(0) : R(a.v_i);
+                else if (t >= 2 && t <= im - 1 && j >= 3 && j <= jm - 1) q_v = a.vn[(size_t)t * sv + j];
+            }
+            // interior cells only (see scal_tile)
+            if (t > 2 && t < im && j >= 2 && j <= jm - 1) q_u = a.un[(size_t)t * su + j];
+        }
+    };
+    __syncthreads();
+    fetch(ilo - 3);
+
+    for (int t = ilo - 3; t <= ihi + 2; ++t) {
+        const R znew = q_t, vnew = q_v, Zp = q_Z, e = q_e, Lp = q_L;
+        R unew = q_u;
+        fetch(t + 1);
+        if (t <= 2) unew = chan ? R(a.u_in) : R(0);
+        else if (t >= im) unew = chan ? uw[0] : R(0);
+        zt.push(znew); uw.push(unew); vw.push(vnew);
+        own_ring[s3(t)][PAD + x] = znew;
+        if (!isC) sV[s2(t)][PAD + x] = vnew;
+        __syncthreads();
+        {
+            R vn_ = sV[s2(t)][PAD + x + 1];
+            if (j + 1 == jm) vn_ = chan ? R(0) : R(2) * sV[s2(t)][PAD + x - 1] - sV[s2(t)][PAD + x];
+            vnw.push(vn_);
+            id.push(Lp != R(0) ? R(1) / (Lp * (e - R(1)) + R(1)) : R(1));
+            oml.push(R(1) - Lp);
+            if (!STEADY) dZ.push(Zp - znew);
+        }
+#define SCS_STAGE(K, WP, CROW, WOUT, RINGW, OUTP)                                                \
+        {                                                                                         \
+            const int r = t - K;                                                                  \
+            const int bq = K;                                                                     \
+            R val = R(0);                                                                         \
+            bool interior = false;                                                                \
+            if (r >= ilo - (3 - K) && r <= ihi - 1 + (3 - K)) {                                   \
+                if (r >= 2 && r <= im - 1) {                                                      \
+                    if (scol) {                                                                   \
+                        interior = true;                                                          \
+                        const R zE = WP[0], zP = WP[1], zW = WP[2];                               \
+                        const R *c = CROW;                                                        \
+                        const R zS = isC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, tinf, tb, ttop); \
+                        const R zN = isC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, tinf, tb, ttop);   \
+                        const R avn_r = R(pr[RR + r]);                                            \
+                        const R diff = (kap * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
+                                     + (kap * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP));   \
+                        const R uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];          \
+                        const R conv = R(0.5) * ((zE + zP) * uE - (zW + zP) * uP) * aue_j         \
+                                     + R(0.5) * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;        \
+                        const R vols = R(pr[3 * RR + r + 1]) * rdym_n;                            \
+                        const R Rr = isC ? vols * (diff - conv * id[bq])                          \
+                                         : vols * (diff - oml[bq] * conv) * id[bq];               \
+                        const R rs = ((STEADY ? R(0) : dZ[bq]) + Rr * dt) * dtau;                 \
+                        const R tau = zt[bq];                                                     \
+                        if (K == 1) val = tau + rs;                                               \
+                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (zP + rs);               \
+                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (zP + rs);                 \
+                        if (r == 2) WOUT.set(0, (isC && chan) ? cslot : val);                     \
+                    }                                                                             \
+                } else if (r == im) {                                                             \
+                    val = scol ? WOUT[0] : R(0);                                                  \
+                } else if (r == 1) {                                                              \
+                    val = (isC && chan) ? cslot : R(0);                                           \
+                }                                                                                 \
+                RINGW                                                                             \
+            }                                                                                     \
+            WOUT.push(val);                                                                       \
+            if (OUTP != nullptr && interior && out_col && r >= ilo && r < ihi)                    \
+                OUTP[(size_t)r * su + j] = val;                                                   \
+        }
+        R *const noout = nullptr;
+        SCS_STAGE(1, zt, own_ring[s3(t - 1)], y1, { ring1[s2(t - 1)][PAD + x] = val; }, noout)
+        SCS_STAGE(2, y1, ring1[s2(t - 2)], y2, { ring2[s2(t - 2)][PAD + x] = val; }, noout)
+        SCS_STAGE(3, y2, ring2[s2(t - 3)], y3, {}, outp)
+#undef SCS_STAGE
+    }
+}
+
+template <typename R, bool STEADY>
+__global__ __launch_bounds__(NT2S, 4) void k_scal_split(const K2Args<R> a)
+{
+    __shared__ Smem2<R> sm;
+    const int G = a.nbi * a.nbj, per = (G + 7) / 8;
+    const int b = blockIdx.x, L = (b % 8) * per + b / 8;
+    if (L >= G) return;
+    scal_tile_split<R, STEADY>(a, L / a.nbj, L % a.nbj, sm);
+}
+
+template <typename R, bool STEADY>
+__global__ __launch_bounds__(NT2S, 4) void k_scal_split_batch(const K2Args<R> *cases, const BatchTile *tiles,
+                                                              int ntiles, const int *ctl)
+{
+    __shared__ Smem2<R> sm;
+    for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        const BatchTile tl = tiles[w];
+        const int cs = ctl[tl.c];
+        if (!(cs & 1)) continue;
+        const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
+        scal_tile_split<R, STEADY>(a, tl.bi, tl.bj, sm);
+        __syncthreads();
+    }
+}
+
 template <typename R, bool STEADY, int OCC>
 __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args<R> a)
 {
@@ -1155,7 +1318,10 @@ void launch_k2(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
 {
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
     const dim3 g2(grid2), b2(NT2);
-    if (occ == 4) {
+    if (occ == 2) {           // split chains: T and C in separate waves
+        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true>), g2, dim3(NT2S), 0, s, k2);
+        else hipLaunchKernelGGL((k_scal_split<R, false>), g2, dim3(NT2S), 0, s, k2);
+    } else if (occ == 4) {
         if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4>), g2, b2, 0, s, k2);
         else hipLaunchKernelGGL((k_scal<R, false, 4>), g2, b2, 0, s, k2);
     } else {
@@ -1405,7 +1571,9 @@ void batch_launch_k2(const cfd_batch_launch *L, hipStream_t s)
     const dim3 g2(L->grid2), b2(NT2);
     const K2Args<R> *a = (const K2Args<R> *)L->k2;
     const BatchTile *t = (const BatchTile *)L->tiles2;
-    if (L->occ2 == 4) hipLaunchKernelGGL((k_scal_batch<R, STEADY, 4>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
+    if (L->occ2 == 2)
+        hipLaunchKernelGGL((k_scal_split_batch<R, STEADY>), g2, dim3(NT2S), 0, s, a, t, L->ntiles2, L->ctl);
+    else if (L->occ2 == 4) hipLaunchKernelGGL((k_scal_batch<R, STEADY, 4>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
     else hipLaunchKernelGGL((k_scal_batch<R, STEADY, 1>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
 }
 
@@ -1518,8 +1686,9 @@ int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t o
      : occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NT1, 0)     \
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NT1, 0))
 #define OCC2(R, S)                                                                                         \
-    (occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NT2, 0)         \
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NT2, 0))
+    (occ2 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_split_batch<R, S>, NT2S, 0)     \
+     : occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NT2, 0)       \
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NT2, 0))
     if (prec == CFD_PREC_FP32) {
         if (steady) { e1 = mupoly ? OCC1(float, true, true) : OCC1(float, true, false); e2 = OCC2(float, true); }
         else { e1 = mupoly ? OCC1(float, false, true) : OCC1(float, false, false); e2 = OCC2(float, false); }

@@ -1014,8 +1014,8 @@ static int tiles_valid(const cfd_ctx *c, const cfd_tiles *t)
         return fail(CFD_ERR_ARG, "row-block heights must lie in [0, imax], got %d, %d", t->rows_uvp, t->rows_tc);
     if (!(t->waves_uvp >= 0 && t->waves_uvp <= 3))
         return fail(CFD_ERR_ARG, "waves_uvp must be 0, 1, 2 or 3, got %d", t->waves_uvp);
-    if (!(t->waves_tc == 0 || t->waves_tc == 1 || t->waves_tc == 4))
-        return fail(CFD_ERR_ARG, "waves_tc must be 0, 1 or 4, got %d", t->waves_tc);
+    if (!(t->waves_tc == 0 || t->waves_tc == 1 || t->waves_tc == 2 || t->waves_tc == 4))
+        return fail(CFD_ERR_ARG, "waves_tc must be 0, 1, 2 or 4, got %d", t->waves_tc);
     return CFD_OK;
 }
 
@@ -1061,7 +1061,8 @@ static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_
  * without spills, or two with the excess spilled to scratch (then more, shorter row blocks
  * fill the 2 x 256 wave slots); variant 3 runs the u and v chains in separate waves of a
  * 512-lane workgroup, each with half the register windows. The T/C kernel runs 3 waves per
- * SIMD as compiled, or 4 with spills. Shapes are timed with the pipeline as configured. */
+ * SIMD as compiled, 4 with spills, or (variant 2) the T and C chains in separate waves of a
+ * 512-lane workgroup. Shapes are timed with the pipeline as configured. */
 static const cfd_tiles CAND_UVP[] = {
     {0, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0}, {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
     {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
@@ -1070,6 +1071,7 @@ static const cfd_tiles CAND_UVP[] = {
 static const cfd_tiles CAND_TC[] = {
     {0, 0, 0, 1}, {0, 8, 0, 1}, {0, 12, 0, 1}, {0, 16, 0, 1}, {0, 24, 0, 1}, {0, 32, 0, 1}, {0, 64, 0, 1},
     {0, 8, 0, 4}, {0, 12, 0, 4}, {0, 16, 0, 4}, {0, 24, 0, 4}, {0, 32, 0, 4},
+    {0, 0, 0, 2}, {0, 8, 0, 2}, {0, 16, 0, 2}, {0, 32, 0, 2}, {0, 64, 0, 2},
 };
 
 int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)

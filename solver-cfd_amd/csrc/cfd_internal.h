@@ -66,7 +66,8 @@ typedef struct cfd_fused_args {
     uint64_t *res_max;
     int32_t prec;
     /* row-block heights of K1 / K2 (0: built-in heuristic) and the register-budget variants
-     * (occ1 in {1, 2} waves per SIMD or 3 = split u/v chains, occ2 in {1, 4} waves per SIMD);
+     * (occ1 in {1, 2} waves per SIMD or 3 = split u/v chains, occ2 in {1, 4} waves per SIMD or
+     * 2 = split T/C chains);
      * chosen by the autotuner */
     int32_t hb1, hb2, occ1, occ2;
     /* optional events recorded before K1, after K1, after K2 and after the boundary pass */

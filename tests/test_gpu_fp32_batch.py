@@ -141,18 +141,19 @@ def _make_batch(cases, prec):
     return b
 
 
-@pytest.mark.parametrize("prec,steady,waves", [(PREC_FP64, 1, 1), (PREC_FP64, 0, 1), (PREC_FP32, 1, 1),
-                                               (PREC_FP64, 1, 3), (PREC_FP32, 0, 3)])
-def test_batch_matches_single_contexts_bitwise(prec, steady, waves):
-    say("case: batch vs contexts", "fp32" if prec == PREC_FP32 else "fp64", "steady", steady, "waves", waves)
+@pytest.mark.parametrize("prec,steady,waves,waves_tc", [(PREC_FP64, 1, 1, 1), (PREC_FP64, 0, 1, 1), (PREC_FP32, 1, 1, 1),
+                                                        (PREC_FP64, 1, 3, 1), (PREC_FP32, 0, 3, 1), (PREC_FP64, 0, 1, 2),
+                                                        (PREC_FP32, 1, 3, 2)])
+def test_batch_matches_single_contexts_bitwise(prec, steady, waves, waves_tc):
+    say("case: batch vs contexts", "fp32" if prec == PREC_FP32 else "fp64", "steady", steady, "waves", waves, waves_tc)
     cases = _bc_consistent(prec, steady)
     b = _make_batch(cases, prec)
-    b.set_tiles(24, 12, waves, 1)
+    b.set_tiles(24, 12, waves, waves_tc)
     code, iters, res, status = b.run(0.0, 25, steps_per_graph=10)
     say("batch", code, iters, status)
     assert code == 0 and list(iters) == [25] * len(cases) and list(status) == [4] * len(cases)
     for i, (prm, x, y, eps, lp, st1, ctx) in enumerate(cases):
-        ctx.set_tiles(24, 12, waves, 1)
+        ctx.set_tiles(24, 12, waves, waves_tc)
         r = ctx.pseudo_steps(25)
         for k in FIELDS:
             a, c = b.download(i, k), ctx.download(k)

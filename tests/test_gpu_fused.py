@@ -156,7 +156,7 @@ def test_graph_replays_match_eager():
 
 
 @pytest.mark.parametrize("tiles", [(16, 8, 1, 1), (37, 11, 2, 4), (64, 64, 2, 1), (24, 5, 1, 4), (29, 8, 3, 1),
-                                   (0, 0, 3, 4)])
+                                   (0, 0, 3, 4), (24, 13, 1, 2), (0, 0, 3, 2)])
 def test_fused_explicit_tiles_match_oracle(tiles):
     """Row-block heights other than the built-in choice (including ones that do not divide
     the row count) and both register-budget variants of each kernel give the same
@@ -198,14 +198,15 @@ def test_fused_two_wave_variants_channel_mu():
     ctx.close()
 
 
+@pytest.mark.parametrize("kernel", ["uvp", "tc"])
 @pytest.mark.parametrize("variant", ["reference_bc", "channel_mu_bouss", "unsteady_upwind"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
-def test_split_chain_kernel_matches_one_wave_kernel(variant, prec):
+def test_split_chain_kernel_matches_one_wave_kernel(variant, prec, kernel):
     """The split-chain u/v/p kernel (u and v chains in separate waves) evaluates the same
     expressions as the one-wave kernel; with FMA contraction the compiler may pair products
     and sums differently in the two kernels, so trajectories agree to rounding (a few ulps
     after 25 steps), far inside the oracle bar that both meet."""
-    say("case: split-chain kernel", variant, "fp32" if prec == PREC_FP32 else "fp64")
+    say("case: split-chain kernel", kernel, variant, "fp32" if prec == PREC_FP32 else "fp64")
     if variant == "reference_bc":
         prm, x, y, eps, lp, st = _multiblock_case(170, 530, bc=0, invfr2=0.7, g=1.5)
     elif variant == "channel_mu_bouss":
@@ -216,9 +217,12 @@ def test_split_chain_kernel_matches_one_wave_kernel(variant, prec):
     else:
         prm, x, y, eps, lp, st = _multiblock_case(90, 300, bc=BC_CHANNEL, invfr2=0.0, scheme=SCHEME_UPWIND, steady=0)
     out = []
-    for waves in (1, 3):
+    for split in (False, True):
         ctx = L.Context(prm, x, y, eps, lp, precision=prec)
-        ctx.set_tiles(24, 12, waves, 1)
+        if kernel == "uvp":
+            ctx.set_tiles(24, 12, 3 if split else 1, 1)
+        else:
+            ctx.set_tiles(24, 12, 1, 2 if split else 1)
         ctx.set_state(st)
         res, hist = ctx.pseudo_steps(25, history=True)
         assert ctx.step_counts() == (24, 1)
@@ -246,7 +250,7 @@ def test_autotune_restores_state_and_keeps_parity():
     say("tuned", tiles, ms)
     rows_uvp, rows_tc, waves_uvp, waves_tc = tiles
     assert rows_uvp in (0, 16, 24, 32, 40, 48, 64, 96, 128) and rows_tc in (0, 8, 12, 16, 24, 32, 64)
-    assert waves_uvp in (1, 2, 3) and waves_tc in (1, 4)
+    assert waves_uvp in (1, 2, 3) and waves_tc in (1, 2, 4)
     assert ms > 0 and ctx.get_tiles() == tiles
     assert ctx.step_counts() == (0, 0)
     for k in FIELDS:
