@@ -319,6 +319,7 @@ int cfd_ctx_destroy(cfd_ctx *c)
 {
     if (!c) return CFD_OK;
     if (c->s || c->owns_stream) hipStreamSynchronize(c->s);
+    if (c->s2) hipStreamSynchronize(c->s2);
     ctx_free(c);
     free(c);
     return CFD_OK;
