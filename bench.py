@@ -285,16 +285,17 @@ def conv_kernel_ms(H, W, B, cin, cout, reps=20):
     y = DeviceArray(np.zeros((B, H, W, cout), np.uint16))
     dy = DeviceArray(bf16_bits(rng.standard_normal((B, H, W, cout))))
     dw = DeviceArray(np.zeros((cout, 9 * cin), np.float32))
+    db = DeviceArray(np.zeros(cout, np.float32))
     lib = L.load()
 
     def fwd():
         L.check(lib.cfd_nn_conv3x3(x.ptr, B, H, W, cin, w.ptr, bias.ptr, cout, 1, y.ptr, None))
 
     def wgrad():
-        L.check(lib.cfd_nn_conv3x3_wgrad(x.ptr, dy.ptr, B, H, W, cin, cout, dw.ptr))
+        L.check(lib.cfd_nn_conv3x3_wgrad(x.ptr, dy.ptr, B, H, W, cin, cout, dw.ptr, db.ptr))
     fwd(); wgrad()
     out = time_null_stream(fwd, reps), time_null_stream(wgrad, reps)
-    for a in (x, w, bias, y, dy, dw):
+    for a in (x, w, bias, y, dy, dw, db):
         a.free()
     return out
 

@@ -363,14 +363,15 @@ int  cfd_unet_last_timing(cfd_unet *net, double ms[3]);
 /* The U-Net's building blocks on device pointers (BF16 as uint16 bit patterns, NHWC,
  * NULL stream): 3x3 convolution (stride 1, zero padding 1; weights [Cout][9 Cin]) with
  * optional bias and ReLU into BF16 y and/or FP32 y32; its weight gradient accumulated into
- * FP32 dw [Cout][9 Cin]; the flipped weights that turn the forward kernel into the input
+ * FP32 dw [Cout][9 Cin] and, when db is not NULL, its bias gradient (the sum of dy over all
+ * pixels) accumulated into FP32 db [Cout] in the same pass; the flipped weights that turn the forward kernel into the input
  * gradient; 2x2 max pooling and its backward (gradient to the first maximal input, added
  * into FP32 gx); nearest x2 upsampling into channels cofs.. of a Cy-channel tensor, and its
  * backward (sum of the 2x2 children, added into FP32 gx). */
 int  cfd_nn_conv3x3(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t Cin, const uint16_t *w,
                     const float *bias, int32_t Cout, int32_t relu, uint16_t *y, float *y32);
 int  cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin,
-                          int32_t Cout, float *dw);
+                          int32_t Cout, float *dw, float *db);
 int  cfd_nn_flip_weights(const float *w, int32_t Cin, int32_t Cout, uint16_t *wf);
 int  cfd_nn_maxpool2(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t *y);
 int  cfd_nn_maxpool2_bwd(const uint16_t *x, const uint16_t *y, const uint16_t *gy, int32_t B, int32_t H, int32_t W,

@@ -138,7 +138,7 @@ def load():
         "cfd_unet_train_step": (ctypes.c_int, [_vp, _fp, _fp, i32, ctypes.c_float, _dp]),
         "cfd_unet_last_timing": (ctypes.c_int, [_vp, _dp]),
         "cfd_nn_conv3x3": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp, _vp, i32, i32, _vp, _vp]),
-        "cfd_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, i32, i32, i32, i32, i32, _vp]),
+        "cfd_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, i32, i32, i32, i32, i32, _vp, _vp]),
         "cfd_nn_flip_weights": (ctypes.c_int, [_vp, i32, i32, _vp]),
         "cfd_nn_maxpool2": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp]),
         "cfd_nn_maxpool2_bwd": (ctypes.c_int, [_vp, _vp, _vp, i32, i32, i32, i32, _vp]),

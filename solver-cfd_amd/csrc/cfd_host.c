@@ -692,6 +692,7 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     a.res_max = slot;
     a.hb1 = c->tiles.rows_uvp; a.hb2 = c->tiles.rows_tc;
     a.occ1 = c->tiles.waves_uvp; a.occ2 = c->tiles.waves_tc;
+    a.lone = 1;
     if (timed) for (int k = 0; k < 4; ++k) a.ev[k] = c->evk[k];
     /* unsteady problems only: in steady mode K1 reads the T iterate that K2 writes */
     const int two = c->pipe && c->s2 && !timed && !st;

@@ -246,8 +246,12 @@ __device__ __forceinline__ bf16x8 lds_read8_unaligned(const bf16 *p)
 template <bool RAGGED>
 __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
                                                             int B, int H, int W, int Cin, int Cout,
-                                                            float *__restrict__ part)
+                                                            float *__restrict__ part, float *__restrict__ part_b)
 {
+    // with part_b, the blocks of the first channel chunk also sum dy over their pixels (the
+    // bias gradient): thread t adds pixels 32 (t / 32) .. + 31 of cout t % 32 of each tile
+    const bool do_bias = part_b != nullptr && blockIdx.y == 0;
+    float bsum = 0.f;
     constexpr int NXP = TH2 * TH2, RX = (NXP + 63) / 64;
     __shared__ WgradSmem sm;
     const int tw = (W + TT - 1) / TT, th = (H + TT - 1) / TT, ntiles = B * th * tw;
@@ -314,6 +318,16 @@ __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restri
         stash();
         __syncthreads();
         if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        if (do_bias) {
+            const uint4 *row = (const uint4 *)&sm.t.dyT[t & 31][32 * (t >> 5)];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint4 q = row[v];
+                const bf16 *e = (const bf16 *)&q;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) bsum += bf2f(e[k]);
+            }
+        }
 #pragma unroll 4
         for (int r = 0; r < TT; ++r) {      // tile row r: pixels 16 r .. 16 r + 15 are this k step
             const bf16x8 a = *(const bf16x8 *)&sm.t.dyT[l32][16 * r + 8 * half];
@@ -325,6 +339,18 @@ __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restri
                     acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[q], 0, 0, 0);
                 }
             }
+        }
+    }
+    if (do_bias) {                          // the eight pixel groups of each cout meet in LDS
+        __syncthreads();
+        float *red = (float *)&sm.t.dyT[0][0];
+        red[t] = bsum;
+        __syncthreads();
+        if (t < 32 && (!RAGGED || n0 + t < Cout)) {
+            float sb = 0.f;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) sb += red[32 * g + t];
+            part_b[(size_t)blockIdx.x * Cout + n0 + t] = sb;
         }
     }
     // the block's sums go to its own slice of the partial buffer part[blockIdx.x][Cout][9 Cin]
@@ -390,34 +416,6 @@ __global__ void k_relu_bwd(bf16 *g, const bf16 *y, int n)          // g *= (y > 
             if (!(bf2f(m.u.e[k]) > 0.f)) a.u.e[k] = 0;
         a.store(g + G * i);
     }
-}
-
-// db[c] += sum over m of g[m][c], C a power of two <= 2048: each thread reads 16 B vectors
-// at a stride that is a multiple of C, so its eight lanes keep fixed channels; per-block sums
-// meet in LDS and leave with one atomic per channel.
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ g, long M, int C, float *__restrict__ db)
-{
-    __shared__ float red[2048];
-    for (int c = threadIdx.x; c < C; c += 256) red[c] = 0.f;
-    __syncthreads();
-    const long n = M * C, nvec = n / 8;
-    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const long v0 = (long)blockIdx.x * 256 + threadIdx.x, stride = (long)gridDim.x * 256;
-    for (long v = v0; v < nvec; v += stride) {
-        const uint4 q = *(const uint4 *)(g + 8 * v);
-        const bf16 *e = (const bf16 *)&q;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) s[k] += bf2f(e[k]);
-    }
-    if (v0 < nvec) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) atomicAdd(&red[(8 * v0 + k) & (C - 1)], s[k]);
-    }
-    if (blockIdx.x == 0)                                   // tail elements (n not a multiple of 8)
-        for (long e = 8 * nvec + threadIdx.x; e < n; e += 256) atomicAdd(&red[e & (C - 1)], bf2f(g[e]));
-    __syncthreads();
-    for (int c = threadIdx.x; c < C; c += 256)
-        if (red[c] != 0.f) atomicAdd(&db[c], red[c]);
 }
 
 template <int G>
@@ -655,8 +653,13 @@ int cfd_nn_conv3x3(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C
     return launched();
 }
 
-int cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin,
-                         int32_t Cout, float *dw)
+}  // extern "C"
+
+namespace {
+
+// dw += weight gradient; with db, db += bias gradient (sum of dy over pixels) in the same pass
+int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+          float *dw, float *db)
 {
     if (B < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return cfd__fail(CFD_ERR_ARG, "cfd_nn_conv3x3_wgrad: bad shape");
     const long total = 9L * Cin * Cout;
@@ -667,13 +670,21 @@ int cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32
     int G = cdiv(1536, per);
     if (G > ntiles) G = ntiles;
     if (G < 1) G = 1;
-    float *part = workspace((size_t)G * total);
-    if (!part) return cfd__fail(CFD_ERR_HIP, "wgrad workspace of %ld floats", (long)G * total);
+    float *part = workspace((size_t)G * (total + Cout));
+    if (!part) return cfd__fail(CFD_ERR_HIP, "wgrad workspace of %ld floats", (long)G * (total + Cout));
+    float *part_b = db ? part + (size_t)G * total : nullptr;
     const dim3 grid(G, cdiv(Cin, CK), cdiv(Cout, 32));
-    if (ragged) hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part);
-    else hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part);
+    if (ragged) hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, part_b);
+    else hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, part_b);
     int r = launched();
     if (r != CFD_OK) return r;
+    if (db) {
+        const int gyb = G < 64 ? G : 64, gperb = cdiv(G, gyb);
+        hipLaunchKernelGGL(k_wsum, dim3(cdiv(Cout, 256), cdiv(G, gperb)), dim3(256), 0, 0, (const float *)part_b, G,
+                           gperb, (long)Cout, db);
+        r = launched();
+        if (r != CFD_OK) return r;
+    }
     // reduce over the groups: about 2048 blocks, each summing a slice of the groups
     const int nbx = cdiv(total, 256);
     int gy = 2048 / nbx;
@@ -683,6 +694,16 @@ int cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32
     gy = cdiv(G, gper);
     hipLaunchKernelGGL(k_wsum, dim3(nbx, gy), dim3(256), 0, 0, (const float *)part, G, gper, total, dw);
     return launched();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfd_nn_conv3x3_wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin,
+                         int32_t Cout, float *dw, float *db)
+{
+    return wgrad(x, dy, B, H, W, Cin, Cout, dw, db);
 }
 
 int cfd_nn_flip_weights(const float *w, int32_t Cin, int32_t Cout, uint16_t *wf)
@@ -893,15 +914,7 @@ int conv_bwd(cfd_unet *u, int q, int B, int ain, const bf16 *gy)
 {
     const Conv &c = u->convs[q];
     const Shape s = act_shape(u, ain);
-    const long M = (long)B * s.h * s.w;
-    int r = cfd_nn_conv3x3_wgrad(u->act[ain].p, gy, B, s.h, s.w, c.cin, c.cout, u->g32.p + c.woff);
-    if (r != CFD_OK) return r;
-    if (c.cout > 2048 || (c.cout & (c.cout - 1)))
-        return cfd__fail(CFD_ERR_UNSUPPORTED, "bias gradient needs a power-of-two channel count, got %d", c.cout);
-    const long nvec = M * c.cout / 8;
-    const int nb = (int)(cdiv(nvec, 256) < 1024 ? (cdiv(nvec, 256) > 0 ? cdiv(nvec, 256) : 1) : 1024);
-    hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, 0, gy, M, c.cout, u->g32.p + c.boff);
-    r = launched();
+    int r = wgrad(u->act[ain].p, gy, B, s.h, s.w, c.cin, c.cout, u->g32.p + c.woff, u->g32.p + c.boff);
     if (r != CFD_OK || ain == A_IN) return r;
     return cfd_nn_conv3x3(gy, B, s.h, s.w, c.cout, u->wf.p + u->wf_off[q], nullptr, c.cin, 0, u->gbuf[ain].p, nullptr);
 }

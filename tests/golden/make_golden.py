@@ -214,6 +214,32 @@ def case_uniform51_pe100():
     case_steps("uniform51_pe100", "c51", ov, x, y, eps, lp, st, 2000, hd)
 
 
+def case_steady51_converged(tol=1e-8, nmax=400000, every=1000):
+    """The uniform51_pe100 problem in steady mode (the physical level tracks the iterate,
+    dtau dt = 2e-3) iterated until max(r_u, r_v, r_p) first falls below 1e-8 after the first
+    step: the converged steady fields, the exit step, and every 1000th history row."""
+    n = 51
+    ov = dict(imax=n, jmax=n, Re=100.0, Pe=100.0, Sc=1.0, Darcy_number=1.0e-2, Cf=0.4,
+              g=0.0, Fr=1.0, v_i=0.5)
+    x = C.uniform_nodes(n); y = C.uniform_nodes(n)
+    eps = C.zeros("p", n, n); eps[1:, 1:] = 1.0
+    lp = C.zeros("p", n, n)
+    st = C.perturbed_state(n, n, seed=1, amp=1e-3, T0=1.0, C0=0.0)
+    hd = dict(dtau=0.05, dt=0.04, beta=1.0, b_art=0.0, tinf=1.0)
+    b = build("c51", **ov)
+    prm = prm_from(ov, steady=1, **hd)
+    buf = run(b, [2, nmax, 0, 0, 1], [hd["dtau"], hd["dt"], hd["beta"], hd["b_art"], hd["tinf"], tol],
+              geometry_payload(x, y, eps, lp) + state_payload(st, n, n))
+    r = Reader(buf)
+    hist = np.ascontiguousarray(r.f64((3, nmax)).T)
+    taken = int(np.max(np.nonzero(hist.max(axis=1) > 0)[0])) + 1
+    assert taken < nmax and hist[taken - 1].max() < tol, "not converged"
+    out = {"hist_sub": hist[every - 1:taken:every], "hist_last": hist[taken - 1]}
+    for nm, kind in (("um_tau", "u"), ("vm_tau", "v"), ("p", "p"), ("T_tau", "p"), ("C_tau", "p")):
+        out[nm] = r.field(kind, n, n)
+    save("steady51_converged", prm, x, y, eps, lp, st, dict(nsteps=taken, tol=tol, every=every), out)
+
+
 def case_quick_porous48():
     """QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2), Da=1e-3, Cf=0.55,
     buoyancy on, at 48x48 with the reference BC set and buoyancy form."""
@@ -286,7 +312,7 @@ def case_refvtk51x84():
 
 
 CASES = dict(refout51x84=case_refout51x84, refvtk51x84=case_refvtk51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
-             phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
+             phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, steady51_converged=case_steady51_converged, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)

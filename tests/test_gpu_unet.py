@@ -37,11 +37,12 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-# the generic implicit-GEMM kernel (channel counts off the 32 grid) and the spatially tiled
-# one (Cin, Cout multiples of 32) with partial tiles at the image edges and images smaller
-# than one tile
+# channel counts off the 32 grid (the element-wise loading kernels) and on it, partial tiles at
+# the image edges, images smaller than one tile, and (2, 48, 80, 256, 256): more pixel tiles
+# (30) than weight-gradient blocks per channel block (24), so blocks loop over tiles
 CONV_SHAPES = [(2, 8, 12, 5, 7), (1, 16, 16, 40, 70), (3, 6, 10, 96, 32), (1, 8, 8, 256, 130),
-               (2, 20, 36, 64, 128), (1, 33, 17, 32, 64), (2, 8, 8, 128, 256), (1, 16, 48, 192, 64)]
+               (2, 20, 36, 64, 128), (1, 33, 17, 32, 64), (2, 8, 8, 128, 256), (1, 16, 48, 192, 64),
+               (2, 48, 80, 256, 256), (1, 40, 40, 32, 4)]
 
 
 @pytest.mark.parametrize("B,H,W,cin,cout", CONV_SHAPES)
@@ -71,11 +72,16 @@ def test_conv3x3_forward_and_gradients_match_oracle(B, H, W, cin, cout):
     assert lib.cfd_nn_conv3x3(dg.ptr, B, H, W, cout, dwf.ptr, None, cin, 0, None, dgx32.ptr) == 0
     ref = U.conv_input_grad(g, w)
     assert rel(dgx32.get(), ref) <= 1e-5
-    # weight gradient
+    # weight gradient, alone and with the bias gradient (both accumulate into what is there)
     dgw = DeviceArray(np.zeros((cout, 9 * cin), np.float32))
-    assert lib.cfd_nn_conv3x3_wgrad(dx.ptr, dg.ptr, B, H, W, cin, cout, dgw.ptr) == 0
+    assert lib.cfd_nn_conv3x3_wgrad(dx.ptr, dg.ptr, B, H, W, cin, cout, dgw.ptr, None) == 0
     ref = U.conv_weight_grad(x, g).reshape(cout, -1)
     assert rel(dgw.get(), ref) <= 1e-5
+    dgb = DeviceArray(np.full(cout, 0.5, np.float32))
+    assert lib.cfd_nn_conv3x3_wgrad(dx.ptr, dg.ptr, B, H, W, cin, cout, dgw.ptr, dgb.ptr) == 0
+    assert rel(dgw.get(), 2 * ref) <= 1e-5
+    refb = g.sum(axis=(0, 1, 2)) + 0.5
+    assert rel(dgb.get(), refb) <= 1e-5, rel(dgb.get(), refb)
     say("ok")
 
 

@@ -74,6 +74,20 @@ def test_trajectories_match_reference_exactly(name):
         assert np.array_equal(s[k], out[k]), k
 
 
+def test_converged_steady_fixture_is_a_fixed_point_of_the_oracle():
+    """steady51_converged: the oracle reproduces the reference's first 1000 steady-mode steps
+    exactly, and one more step from the reference's converged fields keeps the residual (the
+    largest increment of u, v and p) below the tolerance."""
+    o, prm, st, out, meta = _oracle("steady51_converged")
+    s, cs = o.make_state(st)
+    h = o.pseudo_steps(s, cs, meta["every"])
+    assert np.array_equal(h[-1], out["hist_sub"][0])
+    conv = {"um": out["um_tau"], "vm": out["vm_tau"], "p": out["p"], "T": out["T_tau"], "C": out["C_tau"]}
+    s, cs = o.make_state(conv)
+    h = o.pseudo_steps(s, cs, 1)
+    assert 0 < h[0].max() < meta["tol"] and h[0].max() <= out["hist_last"].max()
+
+
 def test_physical_steps_match_reference_exactly():
     o, prm, st, out, meta = _oracle("phys24")
     s, cs = o.make_state(st)
