@@ -246,7 +246,8 @@ __device__ __forceinline__ bf16x8 lds_read8_unaligned(const bf16 *p)
 template <bool RAGGED>
 __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
                                                             int B, int H, int W, int Cin, int Cout,
-                                                            float *__restrict__ part, float *__restrict__ part_b)
+                                                            float *__restrict__ part, long pstride,
+                                                            float *__restrict__ part_b, long bstride)
 {
     // with part_b, the blocks of the first channel chunk also sum dy over their pixels (the
     // bias gradient): thread t adds pixels 32 (t / 32) .. + 31 of cout t % 32 of each tile
@@ -350,13 +351,14 @@ __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restri
             float sb = 0.f;
 #pragma unroll
             for (int g = 0; g < 8; ++g) sb += red[32 * g + t];
-            part_b[(size_t)blockIdx.x * Cout + n0 + t] = sb;
+            part_b[(size_t)blockIdx.x * bstride + n0 + t] = sb;
         }
     }
-    // the block's sums go to its own slice of the partial buffer part[blockIdx.x][Cout][9 Cin]
-    // (reduced over the tile groups later); D rows are couts, columns input channels
+    // the block's sums go to its own slice of the partial buffer, part + blockIdx.x pstride,
+    // laid out [Cout][9 Cin] (reduced over the tile groups later); D rows are couts, columns
+    // input channels
     const size_t K = (size_t)9 * Cin;
-    float *out = part + (size_t)blockIdx.x * K * Cout;
+    float *out = part + (size_t)blockIdx.x * pstride;
     const int ci = ci0 + l32;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -371,15 +373,15 @@ __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restri
     }
 }
 
-// dw[i] += sum over g < G of part[g][i]; blockIdx.y takes a slice of the groups (atomic
-// adds when there are several slices)
-__global__ void k_wsum(const float *__restrict__ part, int G, int gper, long total, float *__restrict__ dw)
+// dw[i] += sum over g < G of part[g stride + i], i < count; blockIdx.y takes a slice of the
+// groups (atomic adds when there are several slices)
+__global__ void k_wsum(const float *__restrict__ part, int G, int gper, long count, long stride, float *__restrict__ dw)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+    if (i >= count) return;
     const int g0 = blockIdx.y * gper, g1 = min(G, g0 + gper);
     float s = 0.f;
-    for (int g = g0; g < g1; ++g) s += part[(size_t)g * total + i];
+    for (int g = g0; g < g1; ++g) s += part[(size_t)g * stride + i];
     if (gridDim.y == 1) dw[i] += s;
     else atomicAdd(&dw[i], s);
 }
@@ -509,31 +511,89 @@ __global__ void k_upsample2_bwd(const bf16 *gy, int B, int H, int W, int C, int 
     }
 }
 
-// copy channels: y[..][cofs + c] = x[..][c], for npix pixels
+// cat[B][2H][2W][Cl + Cs] = [upsample2(low) | skip]: low [B][H][W][Cl], skip [B][2H][2W][Cs]
 template <int G>
-__global__ void k_chan_copy(const bf16 *x, int npix, int C, bf16 *y, int Cy, int cofs)
+__global__ void k_up_cat(const bf16 *low, const bf16 *skip, int B, int H, int W, int Cl, int Cs, bf16 *cat)
 {
-    const int CG = C / G, n = npix * CG;
+    const int Cc = Cl + Cs, CG = Cc / G, n = B * (2 * H) * (2 * W) * CG;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int p = i / CG, cg = i % CG;
+        const int c = G * (i % CG);
+        const int p = i / CG;                          // output pixel (b, ho, wo)
+        const int wo = p % (2 * W), q = p / (2 * W);
+        const int ho = q % (2 * H), b = q / (2 * H);
         VB<G> a;
-        a.load(x + (size_t)G * i);
-        a.store(y + (size_t)p * Cy + cofs + G * cg);
+        if (c < Cl) a.load(low + ((size_t)(b * H + ho / 2) * W + wo / 2) * Cl + c);
+        else a.load(skip + (size_t)p * Cs + (c - Cl));
+        a.store(cat + (size_t)G * i);
     }
 }
 
-// gx[..][c] += g[..][cofs + c]
+// Backward of [pool -> ... -> upsample, concat] around a skip connection, in one pass:
+// g[B][H][W][C] = relu'(x) * (gc[..][cofs + c] + the pooled gradient gy routed to the first of
+// the four inputs that holds the maximum), rounded once to BF16. x is the ReLU output that
+// was pooled (y = maxpool(x)) and concatenated (channels cofs.. of the Cc-channel gc).
 template <int G>
-__global__ void k_chan_add(const bf16 *g, int npix, int C, int Cg, int cofs, float *gx)
+__global__ void k_pool_skip_bwd(const bf16 *x, const bf16 *y, const bf16 *gy, const bf16 *gc, int Cc, int cofs, int B,
+                                int H, int W, int C, bf16 *g)
 {
-    const int CG = C / G, n = npix * CG;
+    const int Ho = H / 2, Wo = W / 2, CG = C / G, n = B * Ho * Wo * CG;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int p = i / CG, cg = i % CG;
-        VB<G> a;
-        a.load(g + (size_t)p * Cg + cofs + G * cg);
-        float *o = gx + (size_t)G * i;
+        const int cg = i % CG;
+        int p = i / CG;
+        const int wo = p % Wo; p /= Wo;
+        const int ho = p % Ho, b = p / Ho;
+        const int pix = (b * H + 2 * ho) * W + 2 * wo;
+        const int dp[4] = {0, 1, W, W + 1};
+        VB<G> a[4], sk[4], m, gg;
 #pragma unroll
-        for (int k = 0; k < G; ++k) o[k] += bf2f(a.u.e[k]);
+        for (int q = 0; q < 4; ++q) {
+            a[q].load(x + (size_t)(pix + dp[q]) * C + G * cg);
+            sk[q].load(gc + (size_t)(pix + dp[q]) * Cc + cofs + G * cg);
+        }
+        m.load(y + (size_t)G * i); gg.load(gy + (size_t)G * i);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const float mv = bf2f(m.u.e[k]);           // compared as values: -0 equals +0
+            bool routed = false;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float xv = bf2f(a[q].u.e[k]);
+                float v = bf2f(sk[q].u.e[k]);
+                if (!routed && xv == mv) { v += bf2f(gg.u.e[k]); routed = true; }
+                sk[q].u.e[k] = xv > 0.f ? f2bf(v) : (bf16)0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sk[q].store(g + (size_t)(pix + dp[q]) * C + G * cg);
+    }
+}
+
+// g[B][H][W][C] = relu'(x) * (sum over the 2x2 children of gy channels cofs..), rounded once
+// to BF16 (gy: [B][2H][2W][Cy]; x the ReLU output that was upsampled)
+template <int G>
+__global__ void k_up_relu_bwd(const bf16 *gy, int B, int H, int W, int C, int Cy, int cofs, const bf16 *x, bf16 *g)
+{
+    const int CG = C / G, n = B * H * W * CG;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int cg = i % CG;
+        int p = i / CG;
+        const int w = p % W; p /= W;
+        const int h = p % H, b = p / H;
+        float s[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) s[k] = 0.f;
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                VB<G> a;
+                a.load(gy + ((size_t)(b * 2 * H + 2 * h + dy) * 2 * W + 2 * w + dx) * Cy + cofs + G * cg);
+#pragma unroll
+                for (int k = 0; k < G; ++k) s[k] += bf2f(a.u.e[k]);
+            }
+        VB<G> m;
+        m.load(x + (size_t)G * i);
+#pragma unroll
+        for (int k = 0; k < G; ++k) m.u.e[k] = bf2f(m.u.e[k]) > 0.f ? f2bf(s[k]) : (bf16)0;
+        m.store(g + (size_t)G * i);
     }
 }
 
@@ -569,6 +629,28 @@ __global__ void k_flip_weights(const float *w, int Cin, int Cout, bf16 *wf)
     }
 }
 
+// The flipped weights of every convolution of the network in one launch: layer q covers
+// elements foff[q] .. foff[q+1] of wf and reads its FP32 master at woff[q]
+struct FlipTable {
+    long foff[16], woff[15];
+    int cin[15], cout[15];
+};
+
+__global__ void k_flip_all(const float *w, FlipTable t, bf16 *wf)
+{
+    const long n = t.foff[15];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int q = 0;
+        while (i >= t.foff[q + 1]) ++q;
+        const long j = i - t.foff[q];
+        const int Cin = t.cin[q], Cout = t.cout[q];
+        const int ci = (int)(j / (9 * Cout));
+        const int rem = (int)(j % (9 * Cout));
+        const int r = rem / Cout, co = rem % Cout;
+        wf[i] = f2bf(w[t.woff[q] + ((size_t)co * 9 + (8 - r)) * Cin + ci]);
+    }
+}
+
 // loss = mean (p - t)^2 (partial sums into *loss), g = 2 (p - t) / n in bf16
 __global__ void k_mse(const float *p, const float *t, long n, bf16 *g, double *loss)
 {
@@ -589,15 +671,18 @@ __global__ void k_mse(const float *p, const float *t, long n, bf16 *g, double *l
     if (threadIdx.x == 0) atomicAdd(loss, part[0] / (double)n);
 }
 
+// Adam on the FP32 masters, which also leave as the BF16 weights of the forward pass
 __global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1, float b2,
-                       float eps, float c1, float c2)
+                       float eps, float c1, float c2, bf16 *wb)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const float gi = g[i];
         const float mi = b1 * m[i] + (1.f - b1) * gi;
         const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
         m[i] = mi; v[i] = vi;
-        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        wb[i] = f2bf(wi);
     }
 }
 
@@ -657,7 +742,9 @@ int cfd_nn_conv3x3(const uint16_t *x, int32_t B, int32_t H, int32_t W, int32_t C
 
 namespace {
 
-// dw += weight gradient; with db, db += bias gradient (sum of dy over pixels) in the same pass
+// dw += weight gradient; with db, db += bias gradient (sum of dy over pixels) in the same pass.
+// When db follows dw in memory (db == dw + 9 Cin Cout, the network's parameter layout), each
+// block's weight and bias partials share one slice and one reduction launch covers both.
 int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
           float *dw, float *db)
 {
@@ -672,27 +759,35 @@ int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W
     if (G < 1) G = 1;
     float *part = workspace((size_t)G * (total + Cout));
     if (!part) return cfd__fail(CFD_ERR_HIP, "wgrad workspace of %ld floats", (long)G * (total + Cout));
-    float *part_b = db ? part + (size_t)G * total : nullptr;
+    const bool joint = db && db == dw + total;
+    const long pstride = joint ? total + Cout : total;
+    float *part_b = !db ? nullptr : (joint ? part + total : part + (size_t)G * total);
+    const long bstride = joint ? pstride : Cout;
     const dim3 grid(G, cdiv(Cin, CK), cdiv(Cout, 32));
-    if (ragged) hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, part_b);
-    else hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, part_b);
+    if (ragged)
+        hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, pstride,
+                           part_b, bstride);
+    else
+        hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, pstride,
+                           part_b, bstride);
     int r = launched();
     if (r != CFD_OK) return r;
-    if (db) {
+    if (db && !joint) {
         const int gyb = G < 64 ? G : 64, gperb = cdiv(G, gyb);
         hipLaunchKernelGGL(k_wsum, dim3(cdiv(Cout, 256), cdiv(G, gperb)), dim3(256), 0, 0, (const float *)part_b, G,
-                           gperb, (long)Cout, db);
+                           gperb, (long)Cout, (long)Cout, db);
         r = launched();
         if (r != CFD_OK) return r;
     }
     // reduce over the groups: about 2048 blocks, each summing a slice of the groups
-    const int nbx = cdiv(total, 256);
+    const long count = joint ? total + Cout : total;
+    const int nbx = cdiv(count, 256);
     int gy = 2048 / nbx;
     if (gy > G) gy = G;
     if (gy < 1) gy = 1;
     const int gper = cdiv(G, gy);
     gy = cdiv(G, gper);
-    hipLaunchKernelGGL(k_wsum, dim3(nbx, gy), dim3(256), 0, 0, (const float *)part, G, gper, total, dw);
+    hipLaunchKernelGGL(k_wsum, dim3(nbx, gy), dim3(256), 0, 0, (const float *)part, G, gper, count, pstride, dw);
     return launched();
 }
 
@@ -792,7 +887,6 @@ struct cfd_unet {
     size_t wf_off[15];
     // activations (bf16) per stage, and their gradients
     Dev<bf16> act[40];
-    Dev<float> gacc[40];                  // FP32 gradient accumulators where two paths meet
     Dev<bf16> gbuf[40];
     Dev<float> out32, target;
     Dev<bf16> gout;
@@ -852,6 +946,12 @@ Shape act_shape(const cfd_unet *u, int a)
     return {0, 0, 0};
 }
 
+long numel(cfd_unet *u, int a, int B)
+{
+    const Shape s = act_shape(u, a);
+    return (long)B * s.h * s.w * s.c;
+}
+
 int conv_fwd(cfd_unet *u, int q, int B, int ain, int aout, float *out32)
 {
     const Conv &c = u->convs[q];
@@ -874,15 +974,11 @@ int forward(cfd_unet *u, int B)
         return cfd_nn_maxpool2(u->act[a].p, B, s.h, s.w, s.c, u->act[b].p);
     };
     auto up_cat = [&](int low, int skip, int cat) {
-        const Shape sl = act_shape(u, low), ss = act_shape(u, skip), sc = act_shape(u, cat);
-        int rr = cfd_nn_upsample2(u->act[low].p, B, sl.h, sl.w, sl.c, u->act[cat].p, sc.c, 0);
-        if (rr == CFD_OK) {
-            const int npix = B * ss.h * ss.w;      // channel counts 32..256, offsets 64..256: multiples of 8
-            hipLaunchKernelGGL(k_chan_copy<8>, dim3(eb((long)npix * ss.c / 8)), dim3(256), 0, 0, u->act[skip].p, npix,
-                               ss.c, u->act[cat].p, sc.c, sl.c);
-            rr = launched();
-        }
-        return rr;
+        const Shape sl = act_shape(u, low), ss = act_shape(u, skip);
+        // channel counts 32..256: multiples of 8
+        hipLaunchKernelGGL(k_up_cat<8>, dim3(eb(numel(u, cat, B) / 8)), dim3(256), 0, 0, u->act[low].p,
+                           u->act[skip].p, B, sl.h, sl.w, sl.c, ss.c, u->act[cat].p);
+        return launched();
     };
     if (r == CFD_OK) r = conv_fwd(u, 0, B, A_IN, A_E1A, nullptr);
     if (r == CFD_OK) r = conv_fwd(u, 1, B, A_E1A, A_E1, nullptr);
@@ -910,19 +1006,19 @@ int forward(cfd_unet *u, int B)
 
 // Backward through conv q: gy (bf16, [M][cout]) -> dW, db accumulated; gx (bf16 input grad)
 // written into gbuf[ain] unless ain == A_IN. For ReLU convs gy must already be masked.
-int conv_bwd(cfd_unet *u, int q, int B, int ain, const bf16 *gy)
+// With relu_in (ain is a ReLU output), the input gradient is then masked by relu'(ain): a pass
+// of its own, since a masked store in the convolution's epilogue costs the convolution more
+// (a 2 B gather per output element after the last MFMA) than the pass does.
+int conv_bwd(cfd_unet *u, int q, int B, int ain, const bf16 *gy, bool relu_in)
 {
     const Conv &c = u->convs[q];
     const Shape s = act_shape(u, ain);
     int r = wgrad(u->act[ain].p, gy, B, s.h, s.w, c.cin, c.cout, u->g32.p + c.woff, u->g32.p + c.boff);
     if (r != CFD_OK || ain == A_IN) return r;
-    return cfd_nn_conv3x3(gy, B, s.h, s.w, c.cout, u->wf.p + u->wf_off[q], nullptr, c.cin, 0, u->gbuf[ain].p, nullptr);
-}
-
-int relu_mask(cfd_unet *u, int a, long n)
-{
-    // every masked activation has 32..256 channels: n is a multiple of 8
-    hipLaunchKernelGGL(k_relu_bwd<8>, dim3(eb(n / 8)), dim3(256), 0, 0, u->gbuf[a].p, u->act[a].p, (int)n);
+    r = cfd_nn_conv3x3(gy, B, s.h, s.w, c.cout, u->wf.p + u->wf_off[q], nullptr, c.cin, 0, u->gbuf[ain].p, nullptr);
+    if (r != CFD_OK || !relu_in) return r;
+    const long n = numel(u, ain, B);                   // 32..256 channels: a multiple of 8
+    hipLaunchKernelGGL(k_relu_bwd<8>, dim3(eb(n / 8)), dim3(256), 0, 0, u->gbuf[ain].p, u->act[ain].p, (int)n);
     return launched();
 }
 
@@ -935,76 +1031,66 @@ int f32_to_bf16(const float *x, long n, bf16 *y)
     return launched();
 }
 
-int to_bf(cfd_unet *u, int a, long n)
-{
-    return f32_to_bf16(u->gacc[a].p, n, u->gbuf[a].p);
-}
 
-long numel(cfd_unet *u, int a, int B)
-{
-    const Shape s = act_shape(u, a);
-    return (long)B * s.h * s.w * s.c;
-}
-
+// Every activation gradient is BF16 and multiplied by the ReLU derivative of its activation
+// before it is used: after the input-gradient convolution by a mask pass, or, where two
+// branches meet, inside the fused upsample/pool/skip kernels (one FP32 sum, one rounding).
 int backward(cfd_unet *u, int B)
 {
     int r = fill0(u->g32.p, (long)u->nparam);
-    // head: gy = gout (no activation)
-    if (r == CFD_OK) r = conv_bwd(u, 14, B, A_D1, u->gout.p);
-    // dec1: D1 (relu) <- conv13 <- D1A (relu) <- conv12 <- C1
-    auto seq = [&](int qb, int qa, int aout, int amid, int ain) {
-        int rr = relu_mask(u, aout, numel(u, aout, B));
-        if (rr == CFD_OK) rr = conv_bwd(u, qb, B, amid, u->gbuf[aout].p);
-        if (rr == CFD_OK) rr = relu_mask(u, amid, numel(u, amid, B));
-        if (rr == CFD_OK) rr = conv_bwd(u, qa, B, ain, u->gbuf[amid].p);
+    // head (no activation on its output): the gradient of D1 leaves masked
+    if (r == CFD_OK) r = conv_bwd(u, 14, B, A_D1, u->gout.p, true);
+    // two convolutions of a level: aout = conv qb (amid), amid = conv qa (ain)
+    auto seq = [&](int qb, int qa, int aout, int amid, int ain, bool relu_in) {
+        int rr = conv_bwd(u, qb, B, amid, u->gbuf[aout].p, true);
+        if (rr == CFD_OK) rr = conv_bwd(u, qa, B, ain, u->gbuf[amid].p, relu_in);
         return rr;
     };
-    // split the concat gradient: low part -> upsample bwd into gacc[low]; skip part -> gacc[skip]
-    auto split = [&](int cat, int low, int skip) {
-        const Shape sl = act_shape(u, low), ss = act_shape(u, skip), sc = act_shape(u, cat);
-        int rr = cfd_nn_upsample2_bwd(u->gbuf[cat].p, B, sl.h, sl.w, sl.c, sc.c, 0, u->gacc[low].p);
-        if (rr == CFD_OK) {
-            const int npix = B * ss.h * ss.w;
-            hipLaunchKernelGGL(k_chan_add<8>, dim3(eb((long)npix * ss.c / 8)), dim3(256), 0, 0, u->gbuf[cat].p, npix,
-                               ss.c, sc.c, sl.c, u->gacc[skip].p);
-            rr = launched();
-        }
-        return rr;
+    // the low (upsampled) part of a concat gradient -> gradient of the level below
+    auto up_bwd = [&](int cat, int low) {
+        const Shape sl = act_shape(u, low), sc = act_shape(u, cat);
+        const long n = numel(u, low, B);               // channel counts 64..256: multiples of 8
+        hipLaunchKernelGGL(k_up_relu_bwd<8>, dim3(eb(n / 8)), dim3(256), 0, 0, u->gbuf[cat].p, B, sl.h, sl.w, sl.c,
+                           sc.c, 0, u->act[low].p, u->gbuf[low].p);
+        return launched();
     };
-    auto pool_bwd = [&](int a, int p) {
-        const Shape s = act_shape(u, a);
-        return cfd_nn_maxpool2_bwd(u->act[a].p, u->act[p].p, u->gbuf[p].p, B, s.h, s.w, s.c, u->gacc[a].p);
+    // skip part of the concat gradient plus the pooled gradient -> gradient of the encoder level
+    auto pool_skip_bwd = [&](int a, int p, int cat, int low) {
+        const Shape s = act_shape(u, a), sc = act_shape(u, cat), sl = act_shape(u, low);
+        const long n = numel(u, p, B);                 // 32..128 channels
+        hipLaunchKernelGGL(k_pool_skip_bwd<8>, dim3(eb(n / 8)), dim3(256), 0, 0, u->act[a].p, u->act[p].p,
+                           u->gbuf[p].p, u->gbuf[cat].p, sc.c, sl.c, B, s.h, s.w, s.c, u->gbuf[a].p);
+        return launched();
     };
-    const int zero[] = {A_E1, A_E2, A_E3, A_B, A_D3, A_D2};
-    for (int q = 0; q < 6 && r == CFD_OK; ++q) r = fill0(u->gacc[zero[q]].p, numel(u, zero[q], B));
-    if (r == CFD_OK) r = seq(13, 12, A_D1, A_D1A, A_C1);
-    if (r == CFD_OK) r = split(A_C1, A_D2, A_E1);
-    if (r == CFD_OK) r = to_bf(u, A_D2, numel(u, A_D2, B));
-    if (r == CFD_OK) r = seq(11, 10, A_D2, A_D2A, A_C2);
-    if (r == CFD_OK) r = split(A_C2, A_D3, A_E2);
-    if (r == CFD_OK) r = to_bf(u, A_D3, numel(u, A_D3, B));
-    if (r == CFD_OK) r = seq(9, 8, A_D3, A_D3A, A_C3);
-    if (r == CFD_OK) r = split(A_C3, A_B, A_E3);
-    if (r == CFD_OK) r = to_bf(u, A_B, numel(u, A_B, B));
-    if (r == CFD_OK) r = seq(7, 6, A_B, A_BA, A_P3);
-    if (r == CFD_OK) r = pool_bwd(A_E3, A_P3);
-    if (r == CFD_OK) r = to_bf(u, A_E3, numel(u, A_E3, B));
-    if (r == CFD_OK) r = seq(5, 4, A_E3, A_E3A, A_P2);
-    if (r == CFD_OK) r = pool_bwd(A_E2, A_P2);
-    if (r == CFD_OK) r = to_bf(u, A_E2, numel(u, A_E2, B));
-    if (r == CFD_OK) r = seq(3, 2, A_E2, A_E2A, A_P1);
-    if (r == CFD_OK) r = pool_bwd(A_E1, A_P1);
-    if (r == CFD_OK) r = to_bf(u, A_E1, numel(u, A_E1, B));
-    if (r == CFD_OK) r = seq(1, 0, A_E1, A_E1A, A_IN);
+    if (r == CFD_OK) r = seq(13, 12, A_D1, A_D1A, A_C1, false);
+    if (r == CFD_OK) r = up_bwd(A_C1, A_D2);
+    if (r == CFD_OK) r = seq(11, 10, A_D2, A_D2A, A_C2, false);
+    if (r == CFD_OK) r = up_bwd(A_C2, A_D3);
+    if (r == CFD_OK) r = seq(9, 8, A_D3, A_D3A, A_C3, false);
+    if (r == CFD_OK) r = up_bwd(A_C3, A_B);
+    if (r == CFD_OK) r = seq(7, 6, A_B, A_BA, A_P3, false);
+    if (r == CFD_OK) r = pool_skip_bwd(A_E3, A_P3, A_C3, A_B);
+    if (r == CFD_OK) r = seq(5, 4, A_E3, A_E3A, A_P2, false);
+    if (r == CFD_OK) r = pool_skip_bwd(A_E2, A_P2, A_C2, A_D3);
+    if (r == CFD_OK) r = seq(3, 2, A_E2, A_E2A, A_P1, false);
+    if (r == CFD_OK) r = pool_skip_bwd(A_E1, A_P1, A_C1, A_D2);
+    if (r == CFD_OK) r = seq(1, 0, A_E1, A_E1A, A_IN, false);
     return r;
 }
 
-int refresh_weights(cfd_unet *u)
+// BF16 copies of the FP32 masters (unless Adam already wrote them) and the flipped weights
+int refresh_weights(cfd_unet *u, bool bf16_copy = true)
 {
-    int r = f32_to_bf16(u->w32.p, (long)u->nparam, u->wb.p);
-    for (int q = 0; q < 15 && r == CFD_OK; ++q) {
+    int r = bf16_copy ? f32_to_bf16(u->w32.p, (long)u->nparam, u->wb.p) : CFD_OK;
+    FlipTable t;
+    for (int q = 0; q < 15; ++q) {
         const Conv &c = u->convs[q];
-        r = cfd_nn_flip_weights(u->w32.p + c.woff, c.cin, c.cout, u->wf.p + u->wf_off[q]);
+        t.foff[q] = (long)u->wf_off[q]; t.woff[q] = (long)c.woff; t.cin[q] = c.cin; t.cout[q] = c.cout;
+    }
+    t.foff[15] = t.foff[14] + (long)u->convs[14].cin * 9 * u->convs[14].cout;
+    if (r == CFD_OK) {
+        hipLaunchKernelGGL(k_flip_all, dim3(eb(t.foff[15])), dim3(256), 0, 0, (const float *)u->w32.p, t, u->wf.p);
+        r = launched();
     }
     return r;
 }
@@ -1054,7 +1140,6 @@ int cfd_unet_create(int32_t height, int32_t width, int32_t batch, uint64_t seed,
         const long n = numel(u, a, batch);
         r = u->act[a].alloc(n);
         if (r == CFD_OK) r = u->gbuf[a].alloc(n);
-        if (r == CFD_OK) r = u->gacc[a].alloc(n);
     }
     const long no = (long)batch * height * width * 4;
     if (r == CFD_OK) r = u->out32.alloc(no);
@@ -1085,7 +1170,7 @@ int cfd_unet_destroy(cfd_unet *u)
     if (!u) return CFD_OK;
     (void)hipDeviceSynchronize();
     u->w32.release(); u->g32.release(); u->m32.release(); u->v32.release(); u->wb.release(); u->wf.release();
-    for (int a = 0; a < 40; ++a) { u->act[a].release(); u->gacc[a].release(); u->gbuf[a].release(); }
+    for (int a = 0; a < 40; ++a) { u->act[a].release(); u->gbuf[a].release(); }
     u->out32.release(); u->target.release(); u->gout.release(); u->loss.release(); u->xin32.release(); u->xin.release();
     for (int k = 0; k < 4; ++k)
         if (u->ev[k]) (void)hipEventDestroy(u->ev[k]);
@@ -1171,9 +1256,9 @@ int cfd_unet_train_step(cfd_unet *u, const float *x, const float *t, int32_t B, 
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - powf(b1, (float)u->step), c2 = 1.f - powf(b2, (float)u->step);
     hipLaunchKernelGGL(k_adam, dim3(eb((long)u->nparam)), dim3(256), 0, 0, u->w32.p, u->g32.p, u->m32.p, u->v32.p,
-                       (long)u->nparam, lr, b1, b2, 1e-8f, c1, c2);
+                       (long)u->nparam, lr, b1, b2, 1e-8f, c1, c2, u->wb.p);
     r = launched();
-    if (r == CFD_OK) r = refresh_weights(u);
+    if (r == CFD_OK) r = refresh_weights(u, false);
     if (r == CFD_OK && u->ev[3]) (void)hipEventRecord(u->ev[3], 0);
     return r == CFD_OK ? CFD_OK : cfd__fail(r, "U-Net update failed");
 }
