@@ -219,14 +219,24 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_t(const bf16 *__restrict__ x
 // operands transposed in LDS: dy of a 16 x 16 pixel tile as dyT[n][pixel], and the 18 x 18
 // halo tile of x as xs[ci][halo pixel]. A tap's operand is then 8 consecutive halo pixels
 // from column 8h + dx of a halo row: a 16 B LDS read at a 2 B-aligned address, which gfx950
-// serves in one ds_read_b128 (unaligned LDS access). Block (tile group, ci chunk,
-// n block): wave w owns taps w, w+4 (and wave 0 tap 8) over all pixels of its group's tiles,
-// the next tile's loads in flight during the current tile's MFMAs; the waves' outputs are
-// disjoint, so each writes its sums straight to the block's slice of a partial buffer, which
-// k_wsum reduces over the groups (no atomics, in LDS or on the gradient).
+// serves in one ds_read_b128 (unaligned LDS access).
+// Block (tile group, ci chunk, n block) of three waves: wave w owns the three taps of column
+// shift dx = w. It walks the 18 halo rows of x once: halo row rr meets dy rows rr, rr-1 and
+// rr-2 (taps dy = 0, 1, 2), so each x operand read serves three MFMAs and each dy operand
+// (kept in a three-row register window) three more: 34 LDS reads per 48 MFMAs per tile,
+// where one tap per read would take 96. The next tile's loads are in flight during the
+// current tile's MFMAs; the waves' outputs are disjoint, so each writes its sums straight to
+// the block's slice of a partial buffer, which k_wsum reduces over the groups (no atomics).
 // ---------------------------------------------------------------------------------------
 constexpr int PPD = TT * TT + 8;            // dyT pitch (bf16): 528 B, conflict-free 16 B reads
 constexpr int PPX = TH2 * TH2 + 4;          // xs pitch per channel (bf16): 656 B = 164 dwords
+constexpr int WG_THREADS = 192;             // three waves
+#ifndef WG_WAVES_PER_SIMD
+#define WG_WAVES_PER_SIMD 2                 // 202 registers; at 3 the operand prefetch spills
+#endif
+#ifndef WG_TARGET_BLOCKS
+#define WG_TARGET_BLOCKS 1024               // weight-gradient blocks per launch (tile groups x channel blocks)
+#endif
 
 struct __align__(16) WgradSmem {
     struct {
@@ -244,72 +254,74 @@ __device__ __forceinline__ bf16x8 lds_read8_unaligned(const bf16 *p)
 }
 
 template <bool RAGGED>
-__global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x, const bf16 *__restrict__ dy,
-                                                            int B, int H, int W, int Cin, int Cout,
-                                                            float *__restrict__ part, long pstride,
-                                                            float *__restrict__ part_b, long bstride)
+__global__ __launch_bounds__(WG_THREADS, WG_WAVES_PER_SIMD) void k_conv3x3_wgrad_t(const bf16 *__restrict__ x,
+                                                                   const bf16 *__restrict__ dy, int B, int H, int W,
+                                                                   int Cin, int Cout, float *__restrict__ part,
+                                                                   long pstride, float *__restrict__ part_b,
+                                                                   long bstride)
 {
     // with part_b, the blocks of the first channel chunk also sum dy over their pixels (the
-    // bias gradient): thread t adds pixels 32 (t / 32) .. + 31 of cout t % 32 of each tile
+    // bias gradient): thread t < 128 adds pixels 64 (t / 32) .. + 63 of cout t % 32 of each tile
     const bool do_bias = part_b != nullptr && blockIdx.y == 0;
     float bsum = 0.f;
-    constexpr int NXP = TH2 * TH2, RX = (NXP + 63) / 64;
+    constexpr int NXP = TH2 * TH2;
+    // operand moves: a unit is one 8-channel group of 64 consecutive pixels (one 16 B load
+    // per lane); dy has 4 groups x 4 pixel steps, x 4 groups x 6 pixel steps (324 pixels),
+    // dealt round-robin to the three waves
+    constexpr int NUY = 16, NUX = 24, RY = (NUY + 2) / 3, RX = (NUX + 2) / 3;
     __shared__ WgradSmem sm;
     const int tw = (W + TT - 1) / TT, th = (H + TT - 1) / TT, ntiles = B * th * tw;
     const int ci0 = blockIdx.y * CK, n0 = blockIdx.z * 32;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int half = lane >> 5, l32 = lane & 31;
-    // wave w owns taps w, w + 4 and (wave 0 only) 8, over all 16 pixel rows of each tile:
-    // the waves' outputs are disjoint, so nothing is reduced across waves
-    const int ntap = wv == 0 ? 3 : 2;
-    f32x16 acc[3];
+    f32x16 acc[3];                               // taps (0, wv), (1, wv), (2, wv)
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[q][e] = 0.f;
 
-    // Each wave moves one group of 8 channels (qd = wave) for 64 consecutive pixels per
-    // step, so the transposed 2 B stores of a wave land in one LDS row (few bank conflicts).
-    const int qd = wv;
-    uint4 ry[4], rx[RX];
+    uint4 ry[RY], rx[RX];
     auto fetch = [&](int tile) {
         const int tx = tile % tw, ty = (tile / tw) % th, b = tile / (tw * th);
         const int h0 = ty * TT, w0 = tx * TT;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {            // dy: 256 pixels x 32 couts
-            const int pix = 64 * i + lane;
+        for (int k = 0; k < RY; ++k) {           // dy: 256 pixels x 32 couts
+            const int u = wv + 3 * k, qd = u & 3, pix = 64 * (u >> 2) + lane;
             const int hh = h0 + (pix >> 4), ww = w0 + (pix & 15);
             uint4 val = make_uint4(0, 0, 0, 0);
-            if (hh < H && ww < W) val = load8<RAGGED>(dy + ((size_t)(b * H + hh) * W + ww) * Cout, n0 + 8 * qd, Cout);
-            ry[i] = val;
+            if (u < NUY && hh < H && ww < W)
+                val = load8<RAGGED>(dy + ((size_t)(b * H + hh) * W + ww) * Cout, n0 + 8 * qd, Cout);
+            ry[k] = val;
         }
 #pragma unroll
-        for (int i = 0; i < RX; ++i) {           // x: 18 x 18 halo pixels x 32 channels
-            const int pix = 64 * i + lane;
+        for (int k = 0; k < RX; ++k) {           // x: 18 x 18 halo pixels x 32 channels
+            const int u = wv + 3 * k, qd = u & 3, pix = 64 * (u >> 2) + lane;
             const int hh = h0 - 1 + pix / TH2, ww = w0 - 1 + pix % TH2;
             uint4 val = make_uint4(0, 0, 0, 0);
-            if (pix < NXP && hh >= 0 && hh < H && ww >= 0 && ww < W)
+            if (u < NUX && pix < NXP && hh >= 0 && hh < H && ww >= 0 && ww < W)
                 val = load8<RAGGED>(x + ((size_t)(b * H + hh) * W + ww) * Cin, ci0 + 8 * qd, Cin);
-            rx[i] = val;
+            rx[k] = val;
         }
     };
+    // transposed stores: the 64 lanes of a unit write 64 consecutive pixels of one LDS row
     auto stash = [&]() {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int pix = 64 * i + lane;
-            union { uint4 v; bf16 e[8]; } u;
-            u.v = ry[i];
+        for (int k = 0; k < RY; ++k) {
+            const int u = wv + 3 * k, qd = u & 3, pix = 64 * (u >> 2) + lane;
+            if (u >= NUY) continue;
+            union { uint4 v; bf16 e[8]; } w;
+            w.v = ry[k];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) sm.t.dyT[8 * qd + q][pix] = u.e[q];
+            for (int q = 0; q < 8; ++q) sm.t.dyT[8 * qd + q][pix] = w.e[q];
         }
 #pragma unroll
-        for (int i = 0; i < RX; ++i) {
-            const int pix = 64 * i + lane;
-            if (pix >= NXP) continue;
-            union { uint4 v; bf16 e[8]; } u;
-            u.v = rx[i];
+        for (int k = 0; k < RX; ++k) {
+            const int u = wv + 3 * k, qd = u & 3, pix = 64 * (u >> 2) + lane;
+            if (u >= NUX || pix >= NXP) continue;
+            union { uint4 v; bf16 e[8]; } w;
+            w.v = rx[k];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) sm.t.xs[8 * qd + q][pix] = u.e[q];
+            for (int q = 0; q < 8; ++q) sm.t.xs[8 * qd + q][pix] = w.e[q];
         }
     };
 
@@ -319,38 +331,38 @@ __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restri
         stash();
         __syncthreads();
         if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-        if (do_bias) {
-            const uint4 *row = (const uint4 *)&sm.t.dyT[t & 31][32 * (t >> 5)];
+        if (do_bias && t < 128) {
+            const uint4 *row = (const uint4 *)&sm.t.dyT[t & 31][64 * (t >> 5)];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < 8; ++v) {
                 const uint4 q = row[v];
                 const bf16 *e = (const bf16 *)&q;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) bsum += bf2f(e[k]);
             }
         }
-#pragma unroll 4
-        for (int r = 0; r < TT; ++r) {      // tile row r: pixels 16 r .. 16 r + 15 are this k step
-            const bf16x8 a = *(const bf16x8 *)&sm.t.dyT[l32][16 * r + 8 * half];
+        // halo row rr pairs with dy rows rr - dy for dy = 0, 1, 2 (those inside 0..15); the
+        // window holds dy rows rr, rr - 1, rr - 2
+        bf16x8 a0, a1, a2;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (q < ntap) {
-                    const int tap = wv + 4 * q, dyy = tap / 3, dxx = tap - 3 * (tap / 3);
-                    const bf16x8 bb = lds_read8_unaligned(&sm.t.xs[l32][(r + dyy) * TH2 + 8 * half + dxx]);
-                    acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[q], 0, 0, 0);
-                }
-            }
+        for (int rr = 0; rr < TH2; ++rr) {
+            const bf16x8 bb = lds_read8_unaligned(&sm.t.xs[l32][rr * TH2 + 8 * half + wv]);
+            a2 = a1; a1 = a0;
+            if (rr < TT) a0 = *(const bf16x8 *)&sm.t.dyT[l32][16 * rr + 8 * half];
+            if (rr < TT) acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bb, acc[0], 0, 0, 0);
+            if (rr >= 1 && rr - 1 < TT) acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bb, acc[1], 0, 0, 0);
+            if (rr >= 2) acc[2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, bb, acc[2], 0, 0, 0);
         }
     }
-    if (do_bias) {                          // the eight pixel groups of each cout meet in LDS
+    if (do_bias) {                          // the four pixel groups of each cout meet in LDS
         __syncthreads();
         float *red = (float *)&sm.t.dyT[0][0];
-        red[t] = bsum;
+        if (t < 128) red[t] = bsum;
         __syncthreads();
         if (t < 32 && (!RAGGED || n0 + t < Cout)) {
             float sb = 0.f;
 #pragma unroll
-            for (int g = 0; g < 8; ++g) sb += red[32 * g + t];
+            for (int g = 0; g < 4; ++g) sb += red[32 * g + t];
             part_b[(size_t)blockIdx.x * bstride + n0 + t] = sb;
         }
     }
@@ -362,8 +374,7 @@ __global__ __launch_bounds__(256, 3) void k_conv3x3_wgrad_t(const bf16 *__restri
     const int ci = ci0 + l32;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        if (q >= ntap) continue;
-        const int tap = wv + 4 * q;
+        const int tap = 3 * q + wv;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int n = n0 + 8 * (e >> 2) + 4 * half + (e & 3);
@@ -751,10 +762,10 @@ int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W
     if (B < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return cfd__fail(CFD_ERR_ARG, "cfd_nn_conv3x3_wgrad: bad shape");
     const long total = 9L * Cin * Cout;
     const bool ragged = (Cin % CK) != 0 || (Cout % 32) != 0;
-    // tile groups so that the grid holds about 1536 blocks (two rounds of three per CU)
+    // tile groups so that the grid holds about WG_TARGET_BLOCKS blocks
     const int ntiles = B * cdiv(H, TT) * cdiv(W, TT);
     const int per = cdiv(Cin, CK) * cdiv(Cout, 32);
-    int G = cdiv(1536, per);
+    int G = cdiv(WG_TARGET_BLOCKS, per);
     if (G > ntiles) G = ntiles;
     if (G < 1) G = 1;
     float *part = workspace((size_t)G * (total + Cout));
@@ -765,11 +776,11 @@ int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W
     const long bstride = joint ? pstride : Cout;
     const dim3 grid(G, cdiv(Cin, CK), cdiv(Cout, 32));
     if (ragged)
-        hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, pstride,
-                           part_b, bstride);
+        hipLaunchKernelGGL(k_conv3x3_wgrad_t<true>, grid, dim3(WG_THREADS), 0, 0, x, dy, B, H, W, Cin, Cout, part,
+                           pstride, part_b, bstride);
     else
-        hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(256), 0, 0, x, dy, B, H, W, Cin, Cout, part, pstride,
-                           part_b, bstride);
+        hipLaunchKernelGGL(k_conv3x3_wgrad_t<false>, grid, dim3(WG_THREADS), 0, 0, x, dy, B, H, W, Cin, Cout, part,
+                           pstride, part_b, bstride);
     int r = launched();
     if (r != CFD_OK) return r;
     if (db && !joint) {
