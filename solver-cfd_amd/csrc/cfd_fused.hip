@@ -99,11 +99,16 @@ struct BatchTile {
 // ---------------------------------------------------------------------------------------
 // K1: u, v (three RK stages each) and p.
 // ---------------------------------------------------------------------------------------
+// A workgroup (or, in the split-chain kernels, each chain) spans NT columns, one per lane:
+// NT1 = 256 for a single grid; NTB = 64 (one wave) for batch cases, whose grids are narrow:
+// a 64..256-wide case fills 24..50 % of the lanes of 256-column strips (a strip carries 13
+// halo columns, so 254 interior columns already need two), and 48..79 % of 64-column ones.
 constexpr int NT1 = 256;
+constexpr int NTB = 64;
 constexpr int HL1 = 6, HR1 = 7;                 // column halo of the U/V/P chain
-constexpr int W1 = NT1 - HL1 - HR1;             // output columns per workgroup
 constexpr int PAD = 3;                          // LDS padding for j-3 .. j+2 reads
-constexpr int LROW1 = NT1 + 2 * PAD;
+__host__ __device__ constexpr int w1(int nt) { return nt - HL1 - HR1; }   // output columns per workgroup
+__host__ __device__ constexpr int lrow(int nt) { return nt + 2 * PAD; }
 
 // Scalars are precomputed on the host so that few values occupy scalar registers.
 template <typename R>
@@ -120,11 +125,11 @@ struct K1Args {
     double mu[5];
 };
 
-template <typename R>
+template <typename R, int NT>
 struct Smem1 {
-    R sUT[4][LROW1], sVT[4][LROW1], sT[2][LROW1];
-    R sU1[3][LROW1], sU2[3][LROW1], sV1[3][LROW1], sV2[3][LROW1], sV3[2][LROW1];
-    double sP[2][LROW1];
+    R sUT[4][lrow(NT)], sVT[4][lrow(NT)], sT[2][lrow(NT)];
+    R sU1[3][lrow(NT)], sU2[3][lrow(NT)], sV1[3][lrow(NT)], sV2[3][lrow(NT)], sV3[2][lrow(NT)];
+    double sP[2][lrow(NT)];
 };
 
 // v of the centre row at column j+dj with the BC formulas (boundary.f90:9-12,19-22 /
@@ -162,12 +167,12 @@ __device__ __forceinline__ void fix_v(R *row, int x, int j, int jm)
     }
 }
 
-template <typename R, bool STEADY, bool MUPOLY>
-__device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Smem1<R> &sm)
+template <typename R, bool STEADY, bool MUPOLY, int NT>
+__device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Smem1<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);        // output rows [ilo, ihi)
-    const int jlo = 2 + bj * W1, jhi = min(jlo + W1, jm);            // output cols [jlo, jhi)
+    const int jlo = 2 + bj * w1(NT), jhi = min(jlo + w1(NT), jm);            // output cols [jlo, jhi)
     const int x = threadIdx.x;
     const int j = jlo - HL1 + x;
     const bool chan = a.chan != 0, upwind_all = a.upwind_all != 0, bouss = a.bouss != 0;
@@ -177,7 +182,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     auto sUT = sm.sUT; auto sVT = sm.sVT; auto sT = sm.sT; auto sP = sm.sP;
     auto sU1 = sm.sU1; auto sU2 = sm.sU2; auto sV1 = sm.sV1; auto sV2 = sm.sV2; auto sV3 = sm.sV3;
 
-    for (int k = x; k < LROW1; k += NT1) {
+    for (int k = x; k < lrow(NT); k += NT) {
         for (int q = 0; q < 4; ++q) { sUT[q][k] = R(0); sVT[q][k] = R(0); }
         for (int q = 0; q < 3; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
         for (int q = 0; q < 2; ++q) { sP[q][k] = 0.0; sT[q][k] = R(1); sV3[q][k] = R(0); }
@@ -440,27 +445,27 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
 template <typename R, bool STEADY, bool MUPOLY, int OCC>
 __global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args<R> a)
 {
-    __shared__ Smem1<R> sm;
+    __shared__ Smem1<R, NT1> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
     if (L >= G) return;
-    uvp_tile<R, STEADY, MUPOLY>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile<R, STEADY, MUPOLY, NT1>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 // Persistent batch: `cases` holds two argument sets per case (even and odd tau parity);
 // ctl[c] bit 0 = active, bit 1 = parity. Every block walks the tile list with a stride of
 // the grid size; inactive cases are skipped.
 template <typename R, bool STEADY, bool MUPOLY, int OCC>
-__global__ __launch_bounds__(NT1, OCC) void k_uvp_batch(const K1Args<R> *cases, const BatchTile *tiles,
+__global__ __launch_bounds__(NTB, OCC) void k_uvp_batch(const K1Args<R> *cases, const BatchTile *tiles,
                                                         int ntiles, const int *ctl)
 {
-    __shared__ Smem1<R> sm;
+    __shared__ Smem1<R, NTB> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile<R, STEADY, MUPOLY>(a, tl.bi, tl.bj, sm);
+        uvp_tile<R, STEADY, MUPOLY, NTB>(a, tl.bi, tl.bj, sm);
         __syncthreads();   // the next tile re-initialises the LDS rings
     }
 }
@@ -476,13 +481,12 @@ __global__ __launch_bounds__(NT1, OCC) void k_uvp_batch(const K1Args<R> *cases, 
 // chain with v3 from the v chain's ring. Values and operation order are those of
 // uvp_tile, so both forms give identical results.
 // ---------------------------------------------------------------------------------------
-constexpr int NT1S = 2 * NT1;
 
-template <typename R>
+template <typename R, int NT>
 struct Smem1S {
-    R sUT[8][LROW1], sVT[16][LROW1], sT[2][LROW1];
-    R sU1[3][LROW1], sU2[3][LROW1], sV1[3][LROW1], sV2[3][LROW1], sV3[2][LROW1];
-    double sP[2][LROW1];
+    R sUT[8][lrow(NT)], sVT[16][lrow(NT)], sT[2][lrow(NT)];
+    R sU1[3][lrow(NT)], sU2[3][lrow(NT)], sV1[3][lrow(NT)], sV2[3][lrow(NT)], sV3[2][lrow(NT)];
+    double sP[2][lrow(NT)];
 };
 
 // The v ring is read down to row t-7 by the u stages while a v wave may already write row
@@ -490,14 +494,14 @@ struct Smem1S {
 __device__ __forceinline__ int s8(int r) { return (r + 24) & 7; }
 __device__ __forceinline__ int s16(int r) { return (r + 32) & 15; }
 
-template <typename R, bool STEADY, bool MUPOLY>
-__device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int bj, Smem1S<R> &sm)
+template <typename R, bool STEADY, bool MUPOLY, int NT>
+__device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int bj, Smem1S<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);
-    const int jlo = 2 + bj * W1, jhi = min(jlo + W1, jm);
-    const int x = threadIdx.x & (NT1 - 1);
-    const bool uchain = threadIdx.x < NT1;
+    const int jlo = 2 + bj * w1(NT), jhi = min(jlo + w1(NT), jm);
+    const int x = threadIdx.x & (NT - 1);
+    const bool uchain = threadIdx.x < NT;
     const int j = jlo - HL1 + x;
     const bool chan = a.chan != 0, upwind_all = a.upwind_all != 0, bouss = a.bouss != 0;
     const R dt = R(a.dt), dtau = R(a.dtau), vi = R(a.v_i), inv_re = R(a.inv_re), gg = R(a.g);
@@ -506,7 +510,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
     auto sUT = sm.sUT; auto sVT = sm.sVT; auto sT = sm.sT; auto sP = sm.sP;
     auto sU1 = sm.sU1; auto sU2 = sm.sU2; auto sV1 = sm.sV1; auto sV2 = sm.sV2; auto sV3 = sm.sV3;
 
-    for (int k = threadIdx.x; k < LROW1; k += NT1S) {
+    for (int k = threadIdx.x; k < lrow(NT); k += 2 * NT) {
         for (int q = 0; q < 8; ++q) sUT[q][k] = R(0);
         for (int q = 0; q < 16; ++q) sVT[q][k] = R(0);
         for (int q = 0; q < 3; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
@@ -761,26 +765,26 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
 }
 
 template <typename R, bool STEADY, bool MUPOLY>
-__global__ __launch_bounds__(NT1S, 1) void k_uvp_split(const K1Args<R> a)
+__global__ __launch_bounds__(2 * NT1, 1) void k_uvp_split(const K1Args<R> a)
 {
-    __shared__ Smem1S<R> sm;
+    __shared__ Smem1S<R, NT1> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    uvp_tile_split<R, STEADY, MUPOLY>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile_split<R, STEADY, MUPOLY, NT1>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, bool MUPOLY>
-__global__ __launch_bounds__(NT1S, 1) void k_uvp_split_batch(const K1Args<R> *cases, const BatchTile *tiles,
-                                                             int ntiles, const int *ctl)
+__global__ __launch_bounds__(2 * NTB, 1) void k_uvp_split_batch(const K1Args<R> *cases, const BatchTile *tiles,
+                                                                int ntiles, const int *ctl)
 {
-    __shared__ Smem1S<R> sm;
+    __shared__ Smem1S<R, NTB> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile_split<R, STEADY, MUPOLY>(a, tl.bi, tl.bj, sm);
+        uvp_tile_split<R, STEADY, MUPOLY, NTB>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -790,8 +794,7 @@ __global__ __launch_bounds__(NT1S, 1) void k_uvp_split_batch(const K1Args<R> *ca
 // ---------------------------------------------------------------------------------------
 constexpr int NT2 = 256;
 constexpr int HL2 = 3, HR2 = 3;
-constexpr int W2 = NT2 - HL2 - HR2;
-constexpr int LROW2 = NT2 + 2 * PAD;
+__host__ __device__ constexpr int w2(int nt) { return nt - HL2 - HR2; }
 
 template <typename R>
 struct K2Args {
@@ -803,10 +806,10 @@ struct K2Args {
     double dt, dtau, kt, kc, tinf, t_bottom, t_top, v_i, u_in, slot_lo, slot_hi, slot_value;
 };
 
-template <typename R>
+template <typename R, int NT>
 struct Smem2 {
-    R sTT[3][LROW2], sCT[3][LROW2], sV[2][LROW2];
-    R sZ1[2][LROW2], sZ2[2][LROW2], sC1[2][LROW2], sC2[2][LROW2];
+    R sTT[3][lrow(NT)], sCT[3][lrow(NT)], sV[2][lrow(NT)];
+    R sZ1[2][lrow(NT)], sZ2[2][lrow(NT)], sC1[2][lrow(NT)], sC2[2][lrow(NT)];
 };
 
 // T of the centre row at column j+dj (boundary.f90:85-88 / channel wall temperatures).
@@ -837,12 +840,12 @@ __device__ __forceinline__ R lds_c(const R *row, int x, int j, int dj, int jm, b
     return row[PAD + x + dj];
 }
 
-template <typename R, bool STEADY>
-__device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Smem2<R> &sm)
+template <typename R, bool STEADY, int NT>
+__device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Smem2<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);
-    const int jlo = 2 + bj * W2, jhi = min(jlo + W2, jm);
+    const int jlo = 2 + bj * w2(NT), jhi = min(jlo + w2(NT), jm);
     const int x = threadIdx.x;
     const int j = jlo - HL2 + x;
     const bool chan = a.chan != 0;
@@ -853,7 +856,7 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
     auto sTT = sm.sTT; auto sCT = sm.sCT; auto sV = sm.sV;
     auto sZ1 = sm.sZ1; auto sZ2 = sm.sZ2; auto sC1 = sm.sC1; auto sC2 = sm.sC2;
 
-    for (int k = x; k < LROW2; k += NT2) {
+    for (int k = x; k < lrow(NT); k += NT) {
         for (int q = 0; q < 3; ++q) { sTT[q][k] = R(0); sCT[q][k] = R(0); }
         for (int q = 0; q < 2; ++q) { sV[q][k] = R(0); sZ1[q][k] = R(0); sZ2[q][k] = R(0); sC1[q][k] = R(0); sC2[q][k] = R(0); }
     }
@@ -979,16 +982,15 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
 // step). Both chains read u, v, eps and L; the T chain publishes the v row for the j+1
 // neighbour. Same expressions as scal_tile.
 // ---------------------------------------------------------------------------------------
-constexpr int NT2S = 2 * NT2;
 
-template <typename R, bool STEADY>
-__device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int bj, Smem2<R> &sm)
+template <typename R, bool STEADY, int NT>
+__device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int bj, Smem2<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);
-    const int jlo = 2 + bj * W2, jhi = min(jlo + W2, jm);
-    const int x = threadIdx.x & (NT2 - 1);
-    const bool isC = threadIdx.x >= NT2;
+    const int jlo = 2 + bj * w2(NT), jhi = min(jlo + w2(NT), jm);
+    const int x = threadIdx.x & (NT - 1);
+    const bool isC = threadIdx.x >= NT;
     const int j = jlo - HL2 + x;
     const bool chan = a.chan != 0;
     const R dt = R(a.dt), dtau = R(a.dtau), kt = R(a.kt), kc = R(a.kc);
@@ -998,7 +1000,7 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
     auto sTT = sm.sTT; auto sCT = sm.sCT; auto sV = sm.sV;
     auto sZ1 = sm.sZ1; auto sZ2 = sm.sZ2; auto sC1 = sm.sC1; auto sC2 = sm.sC2;
 
-    for (int k = threadIdx.x; k < LROW2; k += NT2S) {
+    for (int k = threadIdx.x; k < lrow(NT); k += 2 * NT) {
         for (int q = 0; q < 3; ++q) { sTT[q][k] = R(0); sCT[q][k] = R(0); }
         for (int q = 0; q < 2; ++q) { sV[q][k] = R(0); sZ1[q][k] = R(0); sZ2[q][k] = R(0); sC1[q][k] = R(0); sC2[q][k] = R(0); }
     }
@@ -1111,26 +1113,26 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
 }
 
 template <typename R, bool STEADY>
-__global__ __launch_bounds__(NT2S, 4) void k_scal_split(const K2Args<R> a)
+__global__ __launch_bounds__(2 * NT2, 4) void k_scal_split(const K2Args<R> a)
 {
-    __shared__ Smem2<R> sm;
+    __shared__ Smem2<R, NT2> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    scal_tile_split<R, STEADY>(a, L / a.nbj, L % a.nbj, sm);
+    scal_tile_split<R, STEADY, NT2>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY>
-__global__ __launch_bounds__(NT2S, 4) void k_scal_split_batch(const K2Args<R> *cases, const BatchTile *tiles,
-                                                              int ntiles, const int *ctl)
+__global__ __launch_bounds__(2 * NTB, 4) void k_scal_split_batch(const K2Args<R> *cases, const BatchTile *tiles,
+                                                                 int ntiles, const int *ctl)
 {
-    __shared__ Smem2<R> sm;
+    __shared__ Smem2<R, NTB> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        scal_tile_split<R, STEADY>(a, tl.bi, tl.bj, sm);
+        scal_tile_split<R, STEADY, NTB>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1138,24 +1140,24 @@ __global__ __launch_bounds__(NT2S, 4) void k_scal_split_batch(const K2Args<R> *c
 template <typename R, bool STEADY, int OCC>
 __global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args<R> a)
 {
-    __shared__ Smem2<R> sm;
+    __shared__ Smem2<R, NT2> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    scal_tile<R, STEADY>(a, L / a.nbj, L % a.nbj, sm);
+    scal_tile<R, STEADY, NT2>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, int OCC>
-__global__ __launch_bounds__(NT2, OCC) void k_scal_batch(const K2Args<R> *cases, const BatchTile *tiles,
+__global__ __launch_bounds__(NTB, OCC) void k_scal_batch(const K2Args<R> *cases, const BatchTile *tiles,
                                                          int ntiles, const int *ctl)
 {
-    __shared__ Smem2<R> sm;
+    __shared__ Smem2<R, NTB> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        scal_tile<R, STEADY>(a, tl.bi, tl.bj, sm);
+        scal_tile<R, STEADY, NTB>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1254,7 +1256,7 @@ int pick_rows(int rows, int nbj, int min_rows, int want, int *hb)
 }
 
 template <typename R>
-void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params *P, const cfd_fused_args *a)
+void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params *P, const cfd_fused_args *a, int nt)
 {
     k1.um = (const R *)a->um; k1.vm = (const R *)a->vm; k1.ut = (const R *)a->um_tau; k1.vt = (const R *)a->vm_tau;
     k1.p = a->p; k1.T = (const R *)a->T;
@@ -1268,13 +1270,13 @@ void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params
     k1.inv_reda = 1.0 / (P->re * P->darcy); k1.cfsd = P->cf / sqrt(P->darcy); k1.g = P->g;
     k1.invfr2 = P->invfr2; k1.ri = P->ri; k1.t_ref = P->t_ref; k1.v_i = P->v_i; k1.u_in = P->u_in;
     for (int q = 0; q < 5; ++q) k1.mu[q] = P->mu_coef[q];
-    k1.nbj = cdiv(jm - 2, W1);
+    k1.nbj = cdiv(jm - 2, w1(nt));
     k1.nbi = pick_rows(im - 2, k1.nbj, a->lone ? 8 : 24, a->hb1, &k1.hb);
 }
 
 template <typename R>
 void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack, const cfd_params *P,
-             const cfd_fused_args *a)
+             const cfd_fused_args *a, int nt)
 {
     k2.un = (const R *)a->um_n; k2.vn = (const R *)a->vm_n; k2.T = (const R *)a->T; k2.Tt = (const R *)a->T_tau;
     k2.C = (const R *)a->C; k2.Ct = (const R *)a->C_tau;
@@ -1285,7 +1287,7 @@ void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack,
     k2.dt = P->dt; k2.dtau = P->dtau; k2.kt = 1.0 / P->pe; k2.kc = 1.0 / P->re / P->sc;
     k2.tinf = P->tinf; k2.t_bottom = P->t_bottom; k2.t_top = P->t_top; k2.v_i = P->v_i; k2.u_in = P->u_in;
     k2.slot_lo = P->slot_lo; k2.slot_hi = P->slot_hi; k2.slot_value = P->slot_value;
-    k2.nbj = cdiv(jm - 2, W2);
+    k2.nbj = cdiv(jm - 2, w2(nt));
     k2.nbi = pick_rows(im - 2, k2.nbj, a->lone ? 8 : 16, a->hb2, &k2.hb);
 }
 
@@ -1294,7 +1296,7 @@ template <typename R, bool STEADY, bool MUPOLY>
 void launch_k1_v(const K1Args<R> &k1, int occ, int grid1, hipStream_t s)
 {
     const dim3 g1(grid1);
-    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY>), g1, dim3(NT1S), 0, s, k1);
+    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY>), g1, dim3(2 * NT1), 0, s, k1);
     else if (occ == 2) hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 2>), g1, dim3(NT1), 0, s, k1);
     else hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 1>), g1, dim3(NT1), 0, s, k1);
 }
@@ -1319,8 +1321,8 @@ void launch_k2(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
     const dim3 g2(grid2), b2(NT2);
     if (occ == 2) {           // split chains: T and C in separate waves
-        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true>), g2, dim3(NT2S), 0, s, k2);
-        else hipLaunchKernelGGL((k_scal_split<R, false>), g2, dim3(NT2S), 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true>), g2, dim3(2 * NT2), 0, s, k2);
+        else hipLaunchKernelGGL((k_scal_split<R, false>), g2, dim3(2 * NT2), 0, s, k2);
     } else if (occ == 4) {
         if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4>), g2, b2, 0, s, k2);
         else hipLaunchKernelGGL((k_scal<R, false, 4>), g2, b2, 0, s, k2);
@@ -1342,7 +1344,7 @@ int fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P
     const int im = g->imax, jm = g->jmax;
     const int n = materialize_count(im, jm), ntc = materialize_tc_start(im, jm);
     K1Args<R> k1;
-    fill_k1<R>(k1, im, jm, f->pack, P, a);
+    fill_k1<R>(k1, im, jm, f->pack, P, a, NT1);
     if (two && a->ev_k2_wait) (void)hipStreamWaitEvent(s, a->ev_k2_wait, 0);
     if (timed) (void)hipEventRecord(a->ev[0], s);
     launch_k1<R>(k1, P, a->occ1, s);
@@ -1359,7 +1361,7 @@ int fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P
         (void)hipStreamWaitEvent(s2, a->ev_k1, 0);
     }
     K2Args<R> k2;
-    fill_k2<R>(k2, im, jm, g->y, f->pack, P, a);
+    fill_k2<R>(k2, im, jm, g->y, f->pack, P, a, NT2);
     launch_k2<R>(k2, P, a->occ2, s2);
     r = launched();
     if (r != CFD_OK) return r;
@@ -1555,11 +1557,11 @@ __global__ void k_batch_count(const int *ctl, int n, int *out)
 template <typename R, bool STEADY, bool MUPOLY>
 void batch_launch_k1(const cfd_batch_launch *L, hipStream_t s)
 {
-    const dim3 g1(L->grid1), b1(NT1);
+    const dim3 g1(L->grid1), b1(NTB);
     const K1Args<R> *a = (const K1Args<R> *)L->k1;
     const BatchTile *t = (const BatchTile *)L->tiles1;
     if (L->occ1 == 3)
-        hipLaunchKernelGGL((k_uvp_split_batch<R, STEADY, MUPOLY>), g1, dim3(NT1S), 0, s, a, t, L->ntiles1, L->ctl);
+        hipLaunchKernelGGL((k_uvp_split_batch<R, STEADY, MUPOLY>), g1, dim3(2 * NTB), 0, s, a, t, L->ntiles1, L->ctl);
     else if (L->occ1 == 2)
         hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 2>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
     else hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 1>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
@@ -1568,11 +1570,11 @@ void batch_launch_k1(const cfd_batch_launch *L, hipStream_t s)
 template <typename R, bool STEADY>
 void batch_launch_k2(const cfd_batch_launch *L, hipStream_t s)
 {
-    const dim3 g2(L->grid2), b2(NT2);
+    const dim3 g2(L->grid2), b2(NTB);
     const K2Args<R> *a = (const K2Args<R> *)L->k2;
     const BatchTile *t = (const BatchTile *)L->tiles2;
     if (L->occ2 == 2)
-        hipLaunchKernelGGL((k_scal_split_batch<R, STEADY>), g2, dim3(NT2S), 0, s, a, t, L->ntiles2, L->ctl);
+        hipLaunchKernelGGL((k_scal_split_batch<R, STEADY>), g2, dim3(2 * NTB), 0, s, a, t, L->ntiles2, L->ctl);
     else if (L->occ2 == 4) hipLaunchKernelGGL((k_scal_batch<R, STEADY, 4>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
     else hipLaunchKernelGGL((k_scal_batch<R, STEADY, 1>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
 }
@@ -1626,8 +1628,8 @@ void build_args(int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1,
             cfd_fused_args a = b.a[q];
             a.hb1 = hb1 ? hb1[c] : 0;
             a.hb2 = hb2 ? hb2[c] : 0;
-            fill_k1<R>(k1[2 * c + q], im, jm, b.pack, &b.P, &a);
-            fill_k2<R>(k2[2 * c + q], im, jm, b.y, b.pack, &b.P, &a);
+            fill_k1<R>(k1[2 * c + q], im, jm, b.pack, &b.P, &a, NTB);
+            fill_k2<R>(k2[2 * c + q], im, jm, b.y, b.pack, &b.P, &a, NTB);
             MatArgs<R> &m = mat[2 * c + q];
             m.g = Geo{};
             m.g.y = b.y; m.g.imax = im; m.g.jmax = jm;
@@ -1662,7 +1664,7 @@ int cfdk_batch_build_args(int32_t prec, int32_t ncases, const cfd_batch_case *ca
 
 int cfdk_batch_case_tiles(int kernel, int32_t imax, int32_t jmax, int32_t hb, int32_t c, int32_t *out)
 {
-    const int W = kernel == 1 ? W1 : W2;
+    const int W = kernel == 1 ? w1(NTB) : w2(NTB);
     const int nbj = cdiv(jmax - 2, W);
     int h = 0;
     const int nbi = pick_rows(imax - 2, nbj, kernel == 1 ? 24 : 16, hb, &h);
@@ -1682,13 +1684,13 @@ int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t o
     int b1 = 0, b2 = 0;
     hipError_t e1, e2;
 #define OCC1(R, S, M)                                                                                      \
-    (occ1 == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_split_batch<R, S, M>, NT1S, 0)   \
-     : occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NT1, 0)     \
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NT1, 0))
+    (occ1 == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_split_batch<R, S, M>, 2 * NTB, 0) \
+     : occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NTB, 0)     \
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NTB, 0))
 #define OCC2(R, S)                                                                                         \
-    (occ2 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_split_batch<R, S>, NT2S, 0)     \
-     : occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NT2, 0)       \
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NT2, 0))
+    (occ2 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_split_batch<R, S>, 2 * NTB, 0)  \
+     : occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NTB, 0)       \
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NTB, 0))
     if (prec == CFD_PREC_FP32) {
         if (steady) { e1 = mupoly ? OCC1(float, true, true) : OCC1(float, true, false); e2 = OCC2(float, true); }
         else { e1 = mupoly ? OCC1(float, false, true) : OCC1(float, false, false); e2 = OCC2(float, false); }
