@@ -442,14 +442,14 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
 
 // OCC: waves per SIMD the register budget is sized for (1: no spills for FP64; 2: 256
 // registers). The autotuner picks the faster one per grid.
-template <typename R, bool STEADY, bool MUPOLY, int OCC>
-__global__ __launch_bounds__(NT1, OCC) void k_uvp(const K1Args<R> a)
+template <typename R, bool STEADY, bool MUPOLY, int OCC, int NT>
+__global__ __launch_bounds__(NT, OCC) void k_uvp(const K1Args<R> a)
 {
-    __shared__ Smem1<R, NT1> sm;
+    __shared__ Smem1<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
     if (L >= G) return;
-    uvp_tile<R, STEADY, MUPOLY, NT1>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile<R, STEADY, MUPOLY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 // Persistent batch: `cases` holds two argument sets per case (even and odd tau parity);
@@ -764,14 +764,14 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
     fold_max(a.res + 2, mp_);
 }
 
-template <typename R, bool STEADY, bool MUPOLY>
-__global__ __launch_bounds__(2 * NT1, 1) void k_uvp_split(const K1Args<R> a)
+template <typename R, bool STEADY, bool MUPOLY, int NT>
+__global__ __launch_bounds__(2 * NT, 1) void k_uvp_split(const K1Args<R> a)
 {
-    __shared__ Smem1S<R, NT1> sm;
+    __shared__ Smem1S<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    uvp_tile_split<R, STEADY, MUPOLY, NT1>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile_split<R, STEADY, MUPOLY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, bool MUPOLY>
@@ -1112,14 +1112,14 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
     }
 }
 
-template <typename R, bool STEADY>
-__global__ __launch_bounds__(2 * NT2, 4) void k_scal_split(const K2Args<R> a)
+template <typename R, bool STEADY, int NT>
+__global__ __launch_bounds__(2 * NT, 4) void k_scal_split(const K2Args<R> a)
 {
-    __shared__ Smem2<R, NT2> sm;
+    __shared__ Smem2<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    scal_tile_split<R, STEADY, NT2>(a, L / a.nbj, L % a.nbj, sm);
+    scal_tile_split<R, STEADY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY>
@@ -1137,14 +1137,14 @@ __global__ __launch_bounds__(2 * NTB, 4) void k_scal_split_batch(const K2Args<R>
     }
 }
 
-template <typename R, bool STEADY, int OCC>
-__global__ __launch_bounds__(NT2, OCC) void k_scal(const K2Args<R> a)
+template <typename R, bool STEADY, int OCC, int NT>
+__global__ __launch_bounds__(NT, OCC) void k_scal(const K2Args<R> a)
 {
-    __shared__ Smem2<R, NT2> sm;
+    __shared__ Smem2<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    scal_tile<R, STEADY, NT2>(a, L / a.nbj, L % a.nbj, sm);
+    scal_tile<R, STEADY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, int OCC>
@@ -1291,45 +1291,71 @@ void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack,
     k2.nbi = pick_rows(im - 2, k2.nbj, a->lone ? 8 : 16, a->hb2, &k2.hb);
 }
 
+// Column span of a single grid's workgroups: 64 when 64-column strips need no more lanes
+// than 256-column ones (jmax 64, 128, 204, 256, 512 among others; not 1024 or 2048), else
+// 256. The per-lane cost of the two is about the same and one-wave workgroups have no
+// cross-wave barrier. Measured per pseudo-step, 256 -> 64 columns: 64x64 0.073 -> 0.064 ms,
+// 204x204 0.076 -> 0.064 ms, 512x256 0.078 -> 0.071 ms, 1024x512 0.118 -> 0.113 ms; at
+// 2048x2048 (2624 against 2304 lanes) 0.47 against 0.37 ms for K1.
+int lone_span(int jm)
+{
+    const int lanes_b = cdiv(jm - 2, w1(NTB)) * NTB, lanes_1 = cdiv(jm - 2, w1(NT1)) * NT1;
+    return lanes_b <= lanes_1 ? NTB : NT1;
+}
+
 // occ: 1 or 2 select k_uvp's register budget, 3 the split-chain kernel.
-template <typename R, bool STEADY, bool MUPOLY>
+template <typename R, bool STEADY, bool MUPOLY, int NT>
 void launch_k1_v(const K1Args<R> &k1, int occ, int grid1, hipStream_t s)
 {
     const dim3 g1(grid1);
-    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY>), g1, dim3(2 * NT1), 0, s, k1);
-    else if (occ == 2) hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 2>), g1, dim3(NT1), 0, s, k1);
-    else hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 1>), g1, dim3(NT1), 0, s, k1);
+    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY, NT>), g1, dim3(2 * NT), 0, s, k1);
+    else if (occ == 2) hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 2, NT>), g1, dim3(NT), 0, s, k1);
+    else hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 1, NT>), g1, dim3(NT), 0, s, k1);
 }
 
-template <typename R>
-void launch_k1(const K1Args<R> &k1, const cfd_params *P, int occ, hipStream_t s)
+template <typename R, int NT>
+void launch_k1_nt(const K1Args<R> &k1, const cfd_params *P, int occ, hipStream_t s)
 {
     const int G1 = k1.nbi * k1.nbj, grid1 = ((G1 + 7) / 8) * 8;
     const bool mp = P->mu_model == CFD_MU_POLY4;
     if (P->steady) {
-        if (mp) launch_k1_v<R, true, true>(k1, occ, grid1, s);
-        else launch_k1_v<R, true, false>(k1, occ, grid1, s);
+        if (mp) launch_k1_v<R, true, true, NT>(k1, occ, grid1, s);
+        else launch_k1_v<R, true, false, NT>(k1, occ, grid1, s);
     } else {
-        if (mp) launch_k1_v<R, false, true>(k1, occ, grid1, s);
-        else launch_k1_v<R, false, false>(k1, occ, grid1, s);
+        if (mp) launch_k1_v<R, false, true, NT>(k1, occ, grid1, s);
+        else launch_k1_v<R, false, false, NT>(k1, occ, grid1, s);
     }
 }
 
 template <typename R>
-void launch_k2(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
+void launch_k1(const K1Args<R> &k1, const cfd_params *P, int occ, hipStream_t s, int nt)
+{
+    if (nt == NTB) launch_k1_nt<R, NTB>(k1, P, occ, s);
+    else launch_k1_nt<R, NT1>(k1, P, occ, s);
+}
+
+template <typename R, int NT>
+void launch_k2_nt(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
 {
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
-    const dim3 g2(grid2), b2(NT2);
+    const dim3 g2(grid2), b2(NT);
     if (occ == 2) {           // split chains: T and C in separate waves
-        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true>), g2, dim3(2 * NT2), 0, s, k2);
-        else hipLaunchKernelGGL((k_scal_split<R, false>), g2, dim3(2 * NT2), 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true, NT>), g2, dim3(2 * NT), 0, s, k2);
+        else hipLaunchKernelGGL((k_scal_split<R, false, NT>), g2, dim3(2 * NT), 0, s, k2);
     } else if (occ == 4) {
-        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4>), g2, b2, 0, s, k2);
-        else hipLaunchKernelGGL((k_scal<R, false, 4>), g2, b2, 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4, NT>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<R, false, 4, NT>), g2, b2, 0, s, k2);
     } else {
-        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 1>), g2, b2, 0, s, k2);
-        else hipLaunchKernelGGL((k_scal<R, false, 1>), g2, b2, 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 1, NT>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<R, false, 1, NT>), g2, b2, 0, s, k2);
     }
+}
+
+template <typename R>
+void launch_k2(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s, int nt)
+{
+    if (nt == NTB) launch_k2_nt<R, NTB>(k2, P, occ, s);
+    else launch_k2_nt<R, NT2>(k2, P, occ, s);
 }
 
 template <typename R>
@@ -1344,10 +1370,11 @@ int fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P
     const int im = g->imax, jm = g->jmax;
     const int n = materialize_count(im, jm), ntc = materialize_tc_start(im, jm);
     K1Args<R> k1;
-    fill_k1<R>(k1, im, jm, f->pack, P, a, NT1);
+    const int nt = lone_span(jm);
+    fill_k1<R>(k1, im, jm, f->pack, P, a, nt);
     if (two && a->ev_k2_wait) (void)hipStreamWaitEvent(s, a->ev_k2_wait, 0);
     if (timed) (void)hipEventRecord(a->ev[0], s);
-    launch_k1<R>(k1, P, a->occ1, s);
+    launch_k1<R>(k1, P, a->occ1, s, nt);
     int r = launched();
     if (r != CFD_OK) return r;
     if (timed) (void)hipEventRecord(a->ev[1], s);
@@ -1361,8 +1388,8 @@ int fused_step(const cfd_devgeo *g, const cfd_devfactors *f, const cfd_params *P
         (void)hipStreamWaitEvent(s2, a->ev_k1, 0);
     }
     K2Args<R> k2;
-    fill_k2<R>(k2, im, jm, g->y, f->pack, P, a, NT2);
-    launch_k2<R>(k2, P, a->occ2, s2);
+    fill_k2<R>(k2, im, jm, g->y, f->pack, P, a, nt);
+    launch_k2<R>(k2, P, a->occ2, s2, nt);
     r = launched();
     if (r != CFD_OK) return r;
     if (timed) (void)hipEventRecord(a->ev[2], s);
