@@ -11,8 +11,11 @@ Prints one JSON line. `roofline` prices the dominant kernel (the fused u/v/p ker
 HBM (this path uses no MFMA): achieved = its algorithmic bytes per launch (88 B per cell:
 every field it reads once, every output once, times imax*jmax cells) / its mean launch
 duration from HIP events recorded around it on the context's stream. `step_roofline` does
-the same for the whole pseudo-step (128 B per cell). `cpu_baseline` times the CPU oracle (the C
-restatement, OpenMP) on a bounded sample of the same workload on this host.
+the same for the whole pseudo-step (128 B per cell). `cpu_baseline` times the reference's own
+Fortran hot path (OpenMp_Multicore's solve_U..solve_C, built from its sources with the
+harness in oracle/ref into oracle/_ref/ref_b2048) on a bounded sample of the same grid on
+this host; `cpu_port` times the C restatement (the oracle, OpenMP) on the same workload. Where
+the reference build is absent, the C restatement is the baseline.
 """
 import argparse
 import json
@@ -45,6 +48,28 @@ BYTES_PER_CELL = {
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "ref_b2048")
+
+
+def cpu_reference(cfg, steps):
+    """Time the reference's own Fortran hot path (OpenMp_Multicore solve_U, solve_V, solve_P,
+    solve_Z, solve_C per pseudo-step, through the harness's mode-2 loop) on this host, on the
+    same 2048 x 2048 mesh, porosity and initial state. Its physics is the reference's own
+    (compile-time constants Re 200, Pe 140, Sc 0.7, Da 1e-3, Cf 0.55; its BC set and buoyancy
+    form; constant viscosity), which is what the drop-in replaces. Seconds per step come from
+    two runs (1 and 1 + steps pseudo-steps) whose difference cancels start-up and file I/O."""
+    from oracle import refharness as H
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    p = cfg["params"]
+    im, jm = p.imax, p.jmax
+    pay = H.geometry_payload(cfg["x"], cfg["y"], cfg["eps"], cfg["lp"]) + H.state_payload(cfg["state"], im, jm)
+    per, times = H.time_pseudo_steps(REF_BENCH, [p.dtau, p.dt, p.beta, 0.0, p.tinf, 0.0], pay, 1, 1 + steps, threads)
+    return dict(value=im * jm / per, unit="cell-updates/s", cores=threads, kind="reference",
+                sample="%d pseudo-steps of the Fortran reference's own hot path (OpenMP, flang -O2) on the "
+                       "same %dx%d mesh and state with its own physics: %.1f s against %.1f s for 1 step"
+                       % (steps, im, jm, times[1], times[0]))
 
 
 def cpu_baseline(cfg, seconds):
@@ -360,6 +385,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: eager)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ref-steps", type=int, default=60,
+                    help="pseudo-steps of the reference's Fortran hot path timed for cpu_baseline")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_full_physics_2048x2048_fp64.json"))
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in tile shapes")
     ap.add_argument("--tiles", default="",
@@ -495,7 +522,13 @@ def main():
     ctx.close()
     if not args.no_cpu:
         log("cpu baseline (%.0f s sample)" % args.cpu_seconds)
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+        port = cpu_baseline(cfg, args.cpu_seconds)
+        if os.path.exists(REF_BENCH) and (prm.imax, prm.jmax) == (2048, 2048):
+            log("reference CPU path (%d steps)" % args.ref_steps)
+            out["cpu_baseline"] = cpu_reference(cfg, args.ref_steps)
+            out["cpu_port"] = port
+        else:
+            out["cpu_baseline"] = port
     print(json.dumps(out), flush=True)
 
 

@@ -16,7 +16,9 @@
 #   * compile-time PARAMETER values (comum.f90:29-115) overridden from the command line,
 #     because the reference fixes grid size and physics constants at compile time.
 #
-# usage: build_ref.sh TAG KEY=VALUE ...
+# usage: [REF_FFLAGS=...] build_ref.sh TAG KEY=VALUE ...
+#   REF_FFLAGS adds compiler flags that do not change the arithmetic (e.g. -mcmodel=medium for
+#   grids whose static arrays exceed 2 GB).
 #   keys: imax jmax Re Pe Sc Darcy_number Cf g Fr v_i Lhori y_up y_down Px_grid Py_grid Q_grid
 #   values are Fortran literals (e.g. 1.0d-3).
 set -euo pipefail
@@ -54,7 +56,7 @@ sed -i -E '/^subroutine RESC/,/^end subroutine RESC/ s/\bRC\b/RC_d/g' "$W/equati
 sed -i -E '/^SUBROUTINE bcC/,/^END SUBROUTINE bcC/ s/\bCn\b/Cn_d/g' "$W/boundary.f90"
 
 cp "$HERE/harness.f90" "$W/harness.f90"
-( cd "$W" && "$FC" -O2 -ffp-contract=off -fopenmp -o "$OUT/ref_$TAG" \
+( cd "$W" && "$FC" -O2 -ffp-contract=off -fopenmp ${REF_FFLAGS:-} -o "$OUT/ref_$TAG" \
       comum.f90 boundary.f90 equations.f90 nonsymetric_mesh.f90 harness.f90 )
 grep -E "PARAMETER :: (imax|jmax|Re|Pe|Sc|Darcy_number|Cf|g|Fr|v_i|Px_grid|Lhori|y_up|y_down) " "$W/comum.f90" > "$OUT/ref_$TAG.params"
 echo "built $OUT/ref_$TAG"

@@ -13,7 +13,6 @@ import json
 import os
 import subprocess
 import sys
-import tempfile
 
 import numpy as np
 
@@ -21,6 +20,8 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
 sys.path.insert(0, os.path.join(ROOT, "solver-cfd_amd"))
 from cfd_amd import cases as C  # noqa: E402
 from cfd_amd.params import make_params, params_to_dict  # noqa: E402
+sys.path.insert(0, ROOT)
+from oracle import refharness as H  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(ROOT, "oracle", "ref", "build_ref.sh")
@@ -46,49 +47,12 @@ def build(tag, **ov):
 
 
 def run(binary, header_i, header_d, payload):
-    with tempfile.TemporaryDirectory() as d:
-        os.makedirs(os.path.join(d, "data"), exist_ok=True)
-        with open(os.path.join(d, "harness_in.bin"), "wb") as f:
-            np.asarray(header_i, dtype=np.int32).tofile(f)
-            np.asarray(header_d, dtype=np.float64).tofile(f)
-            for a in payload:
-                np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
-        env = dict(os.environ, OMP_NUM_THREADS="1")
-        subprocess.run([binary], cwd=d, check=True, stdout=subprocess.DEVNULL, env=env)
-        return open(os.path.join(d, "harness_out.bin"), "rb").read()
+    return H.run(binary, header_i, header_d, payload, threads=1)
 
 
-class Reader:
-    def __init__(self, buf):
-        self.buf, self.off = buf, 0
-
-    def f64(self, shape):
-        n = int(np.prod(shape))
-        a = np.frombuffer(self.buf, dtype=np.float64, count=n, offset=self.off)
-        self.off += 8 * n
-        return a.reshape(shape, order="F")
-
-    def i32(self, n):
-        a = np.frombuffer(self.buf, dtype=np.int32, count=n, offset=self.off)
-        self.off += 4 * n
-        return a
-
-    def field(self, kind, im, jm):
-        return C.from_fortran(self.f64(C.fortran_shape(kind, im, jm)), kind, im, jm)
-
-
-def geometry_payload(x, y, eps, lp):
-    im, jm = len(x) - 1, len(y) - 1
-    g = C.derive_geometry(x, y)
-    return [x[1:], y[1:], g["xm"][1:], g["ym"][1:], g["aun"][2:im + 1], g["aue"][1:jm + 1],
-            g["avn"][1:im + 1], g["ave"][2:jm + 1],
-            C.to_fortran(eps, "p", im, jm), C.to_fortran(lp, "p", im, jm)]
-
-
-def state_payload(st, im, jm):
-    return [C.to_fortran(st["um"], "u", im, jm), C.to_fortran(st["vm"], "v", im, jm),
-            C.to_fortran(st["p"], "p", im, jm), C.to_fortran(st["T"], "p", im, jm),
-            C.to_fortran(st["C"], "p", im, jm)]
+Reader = H.Reader
+geometry_payload = H.geometry_payload
+state_payload = H.state_payload
 
 
 def save(name, prm, x, y, eps, lp, st, extra, out):
