@@ -1,0 +1,87 @@
+"""GPU: the learned warm start end to end, on a problem small enough for a test.
+
+Steady states of seeded 32 x 32 cases come from the batched solver; the U-Net is trained on
+them with the library's own kernels; held-out cases are then solved from the network's
+prediction and from the zero initial state. The trained network must save pseudo-steps,
+both starts must reach the same steady state, and a start the solver cannot use (one that
+diverges) must fall back to the zero initial state and still converge.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cfd_amd import lib as L
+from cfd_amd import warmstart as WS
+
+pytestmark = pytest.mark.gpu
+
+H = 32
+N_TRAIN, N_TEST = 256, 32
+TOL_DATA, TOL_SOLVE, ITC = 1e-5, 1e-6, 100000
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    say("case: warm start data, %d cases on %dx%d" % (N_TRAIN + N_TEST, H, H))
+    cs = WS.steady_cases(N_TRAIN + N_TEST, seed=9, size=H, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
+    sol = WS.solve(cs, TOL_DATA, ITC)
+    conv = sol["status"] == 1
+    X = np.stack([WS.features(c) for c in cs])
+    Y = np.stack([WS.targets(sol["batch"], i) for i in range(len(cs))])
+    sol["batch"].close()
+    tr = [i for i in range(N_TRAIN) if conv[i]]
+    te = [i for i in range(N_TRAIN, N_TRAIN + N_TEST) if conv[i]]
+    say("converged: train %d/%d, held out %d/%d" % (len(tr), N_TRAIN, len(te), N_TEST))
+    assert len(tr) >= 0.5 * N_TRAIN and len(te) >= 12
+    mu = Y[tr].mean(axis=(0, 1, 2)); sd = Y[tr].std(axis=(0, 1, 2)) + 1e-6
+    net = L.UNet(H, H, 16, seed=5)
+    hist = WS.train(net, X[tr], ((Y[tr] - mu) / sd).astype(np.float32), 20, 16, 1e-3)
+    say("training loss", hist[0], "->", hist[-1])
+    pred = np.concatenate([net.predict(X[te[k:k + 16]]) for k in range(0, len(te), 16)]) * sd + mu
+    net.close()
+    yield cs, te, pred, Y, hist
+
+
+def test_trained_network_saves_pseudo_steps(trained):
+    cs, te, pred, Y, hist = trained
+    assert hist[-1] < 0.2 * hist[0]
+    err = float(np.sqrt(np.mean((pred - Y[te]) ** 2)) / np.sqrt(np.mean(Y[te] ** 2)))
+    cases = [cs[i] for i in te]
+    states = [WS.state_from_prediction(pred[k], H, H) for k in range(len(te))]
+    solw = WS.solve(cases, TOL_SOLVE, ITC, states=states, fallback_factor=10.0)
+    solz = WS.solve(cases, TOL_SOLVE, ITC)
+    itw = solw["iters32"] + solw["iters64"]
+    itz = solz["iters32"] + solz["iters64"]
+    say("prediction error %.3f; pseudo-steps from prediction %d (median %d), from zero %d (median %d); "
+        "fallbacks %d" % (err, itw.sum(), np.median(itw), itz.sum(), np.median(itz), solw["fallback"].sum()))
+    assert (solw["status"] == 1).all() and (solz["status"] == 1).all()
+    assert err < 0.5
+    assert itw.sum() < 0.9 * itz.sum()
+    # both starts end at the same steady state, to the spread a 1e-6 stop leaves
+    worst = 0.0
+    for k in range(len(te)):
+        for f in ("um_tau", "vm_tau", "T_tau"):
+            a, b = solw["batch"].download(k, f), solz["batch"].download(k, f)
+            worst = max(worst, float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)))
+    say("steady fields, prediction start vs zero start: max rel diff %.2e" % worst)
+    assert worst < 5e-2
+    solw["batch"].close(); solz["batch"].close()
+
+
+def test_unusable_start_falls_back_to_zero_init(trained):
+    cs, te, pred, Y, hist = trained
+    cases = [cs[i] for i in te[:8]]
+    # velocities 1000x too large: the explicit pseudo-step diverges from such a start
+    states = [WS.state_from_prediction(pred[k] * np.array([1e3, 1e3, 1.0, 1.0], np.float32), H, H)
+              for k in range(len(cases))]
+    sol = WS.solve(cases, TOL_SOLVE, ITC, states=states, fallback_factor=10.0)
+    say("fallbacks %d/%d, status %s" % (sol["fallback"].sum(), len(cases), sol["status"]))
+    assert sol["fallback"].all()
+    assert (sol["status"] == 1).all()
+    sol["batch"].close()
