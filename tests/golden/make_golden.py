@@ -204,17 +204,22 @@ def case_steady51_converged(tol=1e-8, nmax=400000, every=1000):
     save("steady51_converged", prm, x, y, eps, lp, st, dict(nsteps=taken, tol=tol, every=every), out)
 
 
-def case_quick_porous48():
+def case_quick_porous48(n=48, name="quick_porous48", tag="c48"):
     """QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2), Da=1e-3, Cf=0.55,
     buoyancy on, at 48x48 with the reference BC set and buoyancy form."""
-    n = 48
     ov = dict(imax=n, jmax=n, Re=200.0, Pe=200.0, Sc=1.0, Darcy_number=1.0e-3, Cf=0.55,
               g=0.0, Fr=1.0, v_i=0.5)
     x = C.uniform_nodes(n); y = C.uniform_nodes(n)
     eps, lp = C.porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
     st = C.perturbed_state(n, n, seed=2, amp=1e-3, T0=1.5, C0=0.0)
     hd = dict(dtau=0.05, dt=0.005, beta=1.0, b_art=0.0, tinf=1.0)
-    case_steps("quick_porous48", "c48", ov, x, y, eps, lp, st, 2000, hd)
+    case_steps(name, tag, ov, x, y, eps, lp, st, 2000, hd)
+
+
+def case_quick_porous204():
+    """The quick_porous48 problem at the 204 x 204 size of the porous-block benchmark
+    configuration (2000 pseudo-steps)."""
+    case_quick_porous48(204, "quick_porous204", "c204")
 
 
 def case_mesh51():
@@ -276,7 +281,8 @@ def case_refvtk51x84():
 
 
 CASES = dict(refout51x84=case_refout51x84, refvtk51x84=case_refvtk51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
-             phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, steady51_converged=case_steady51_converged, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
+             phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, steady51_converged=case_steady51_converged,
+             quick_porous204=case_quick_porous204, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)

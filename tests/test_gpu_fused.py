@@ -33,7 +33,7 @@ def _setup():
     L.load()
 
 
-@pytest.mark.parametrize("name", ["uniform51_pe100", "quick_porous48", "steady24"])
+@pytest.mark.parametrize("name", ["uniform51_pe100", "quick_porous48", "steady24", "quick_porous204"])
 def test_fused_trajectory_matches_reference(name):
     say("case: fused trajectory vs Fortran golden", name)
     prm, x, y, eps, lp, st, out, meta = load_golden(name)

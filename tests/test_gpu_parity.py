@@ -117,7 +117,7 @@ def _ctx_from_golden(name, kernel=0):
     return ctx, prm, out, meta
 
 
-@pytest.mark.parametrize("name", ["steps24", "steady24", "uniform51_pe100", "quick_porous48"])
+@pytest.mark.parametrize("name", ["steps24", "steady24", "uniform51_pe100", "quick_porous48", "quick_porous204"])
 def test_exact_path_trajectory_matches_reference(name):
     say("case: exact-path trajectory", name)
     ctx, prm, out, meta = _ctx_from_golden(name, kernel=0)

@@ -64,7 +64,7 @@ def test_solves_match_reference_exactly():
     assert np.array_equal(s["C_n_tau"], out["sc_C_n_tau"])
 
 
-@pytest.mark.parametrize("name", ["steps24", "steady24", "uniform51_pe100", "quick_porous48"])
+@pytest.mark.parametrize("name", ["steps24", "steady24", "uniform51_pe100", "quick_porous48", "quick_porous204"])
 def test_trajectories_match_reference_exactly(name):
     o, prm, st, out, meta = _oracle(name)
     s, cs = o.make_state(st)
