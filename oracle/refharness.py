@@ -43,21 +43,22 @@ def write_input(path, header_i, header_d, payload):
             np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
 
 
-def run(binary, header_i, header_d, payload, threads=1):
-    """Run the harness once; returns the bytes of harness_out.bin."""
-    with tempfile.TemporaryDirectory() as d:
-        os.makedirs(os.path.join(d, "data"), exist_ok=True)
-        write_input(os.path.join(d, "harness_in.bin"), header_i, header_d, payload)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        subprocess.run([binary], cwd=d, check=True, stdout=subprocess.DEVNULL, env=env)
-        return open(os.path.join(d, "harness_out.bin"), "rb").read()
-
-
 def _big_stack():
     """The reference's routines keep grid-sized automatic arrays on the stack: lift the
     soft stack limit to the hard one for the child (the OpenMP workers get OMP_STACKSIZE)."""
     soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
     resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def run(binary, header_i, header_d, payload, threads=1):
+    """Run the harness once; returns the bytes of harness_out.bin."""
+    with tempfile.TemporaryDirectory() as d:
+        os.makedirs(os.path.join(d, "data"), exist_ok=True)
+        write_input(os.path.join(d, "harness_in.bin"), header_i, header_d, payload)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="1G")
+        subprocess.run([binary], cwd=d, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       env=env, preexec_fn=_big_stack)
+        return open(os.path.join(d, "harness_out.bin"), "rb").read()
 
 
 def time_pseudo_steps(binary, header_d, payload, n_short, n_long, threads):

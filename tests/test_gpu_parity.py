@@ -6,6 +6,7 @@ rounding per operation in the reference's order, so it is held to 1e-13 relative
 difference per field after 2000 pseudo-steps.
 """
 import ctypes
+import os
 import sys
 
 import numpy as np
@@ -271,3 +272,41 @@ def test_run_to_tol_matches_reference_exit_step():
     code, it, res = ctx.run_to_tol(tol, 10 ** 6, check_every=1)
     assert it == n_ref
     ctx.close()
+
+
+# ---- full size against the reference's own build ----------------------------------------------
+REF2048 = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_b2048")
+
+
+@pytest.mark.skipif(not os.path.exists(REF2048), reason="reference Fortran build at 2048x2048 (oracle/_ref) absent")
+def test_full_size_matches_reference_build():
+    """2048 x 2048: the reference's own Fortran hot path (built from its sources with the
+    benchmark's constants: Re 200, Pe 140, Sc 0.7, Da 1e-3, Cf 0.55, its BC set and buoyancy
+    form) and the library (first step on the exact path, then the fused kernels) run 20
+    pseudo-steps from the benchmark's tanh mesh, porous block and seeded state; every field
+    within 1e-10 (max relative difference, normwise), the residual history within 1e-10."""
+    from oracle import refharness as H
+    say("case: 2048x2048, 20 pseudo-steps against the reference build")
+    cfg = C.case_full_physics(2048, 2048)
+    n, im, jm = 20, 2048, 2048
+    hd = dict(dtau=0.05, dt=cfg["params"].dt, beta=1.0, b_art=0.0, tinf=1.0)
+    prm = make_params(im, jm, re=200.0, pe=140.0, sc=0.7, darcy=1e-3, cf=0.55, g=9.80665, invfr2=1.0, v_i=0.5,
+                      **hd)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    buf = H.run(REF2048, [2, n, 0, 0, 0], [hd["dtau"], hd["dt"], hd["beta"], hd["b_art"], hd["tinf"], 0.0],
+                H.geometry_payload(cfg["x"], cfg["y"], cfg["eps"], cfg["lp"]) + H.state_payload(cfg["state"], im, jm),
+                threads=threads)
+    r = H.Reader(buf)
+    hist = np.ascontiguousarray(r.f64((3, n)).T)
+    ref = {nm: r.field(kind, im, jm) for nm, kind in (("um_tau", "u"), ("vm_tau", "v"), ("p", "p"),
+                                                       ("T_tau", "p"), ("C_tau", "p"))}
+    ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"])
+    ctx.set_state(cfg["state"])
+    res, h = ctx.pseudo_steps(n, history=True)
+    nf, ne = ctx.step_counts()
+    errs = {k: max_rel(ctx.download(k), ref[k]) for k in FIELDS_OUT}
+    h_err = max_rel(h, hist)
+    say("fused steps", nf, "exact steps", ne, "history", h_err, "fields", errs)
+    ctx.close()
+    assert nf == n - 1 and ne == 1
+    assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
