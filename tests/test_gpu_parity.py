@@ -278,15 +278,16 @@ def test_run_to_tol_matches_reference_exit_step():
 REF2048 = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_b2048")
 
 
-@pytest.mark.skipif(not os.path.exists(REF2048), reason="reference Fortran build at 2048x2048 (oracle/_ref) absent")
-def test_full_size_matches_reference_build():
-    """2048 x 2048: the reference's own Fortran hot path (built from its sources with the
+@pytest.fixture(scope="module")
+def ref2048():
+    """The reference's own Fortran hot path at 2048 x 2048 (built from its sources with the
     benchmark's constants: Re 200, Pe 140, Sc 0.7, Da 1e-3, Cf 0.55, its BC set and buoyancy
-    form) and the library (first step on the exact path, then the fused kernels) run 20
-    pseudo-steps from the benchmark's tanh mesh, porous block and seeded state; every field
-    within 1e-10 (max relative difference, normwise), the residual history within 1e-10."""
+    form): 20 pseudo-steps from the benchmark's tanh mesh, porous block and seeded state, run
+    on this host through the harness."""
+    if not os.path.exists(REF2048):
+        pytest.skip("reference Fortran build at 2048x2048 (oracle/_ref) absent")
     from oracle import refharness as H
-    say("case: 2048x2048, 20 pseudo-steps against the reference build")
+    say("case: 2048x2048 reference build, 20 pseudo-steps")
     cfg = C.case_full_physics(2048, 2048)
     n, im, jm = 20, 2048, 2048
     hd = dict(dtau=0.05, dt=cfg["params"].dt, beta=1.0, b_art=0.0, tinf=1.0)
@@ -300,13 +301,26 @@ def test_full_size_matches_reference_build():
     hist = np.ascontiguousarray(r.f64((3, n)).T)
     ref = {nm: r.field(kind, im, jm) for nm, kind in (("um_tau", "u"), ("vm_tau", "v"), ("p", "p"),
                                                        ("T_tau", "p"), ("C_tau", "p"))}
-    ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"])
+    return cfg, prm, n, hist, ref
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_full_size_matches_reference_build(ref2048, prec):
+    """The library on the same 2048 x 2048 problem (first step on the exact path, then the
+    fused kernels): FP64 every field and the residual history within 1e-10 (max relative
+    difference, normwise); FP32 storage within 1e-5 (its error against FP64 is reported)."""
+    from cfd_amd.params import PREC_FP32, PREC_FP64
+    cfg, prm, n, hist, ref = ref2048
+    ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], precision=PREC_FP32 if prec == "fp32" else PREC_FP64)
     ctx.set_state(cfg["state"])
     res, h = ctx.pseudo_steps(n, history=True)
     nf, ne = ctx.step_counts()
     errs = {k: max_rel(ctx.download(k), ref[k]) for k in FIELDS_OUT}
     h_err = max_rel(h, hist)
-    say("fused steps", nf, "exact steps", ne, "history", h_err, "fields", errs)
+    say(prec, "fused steps", nf, "exact steps", ne, "history", h_err, "fields", errs)
     ctx.close()
     assert nf == n - 1 and ne == 1
-    assert h_err <= 1e-10 and max(errs.values()) <= 1e-10
+    bar = 1e-10 if prec == "fp64" else 1e-5
+    assert max(errs.values()) <= bar
+    if prec == "fp64":
+        assert h_err <= 1e-10
