@@ -275,26 +275,29 @@ def test_run_to_tol_matches_reference_exit_step():
 
 
 # ---- full size against the reference's own build ----------------------------------------------
-REF2048 = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_b2048")
+REF_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
+REF_FULL = {(2048, 2048): "ref_b2048", (1024, 512): "ref_b1024x512"}
 
 
-@pytest.fixture(scope="module")
-def ref2048():
-    """The reference's own Fortran hot path at 2048 x 2048 (built from its sources with the
-    benchmark's constants: Re 200, Pe 140, Sc 0.7, Da 1e-3, Cf 0.55, its BC set and buoyancy
-    form): 20 pseudo-steps from the benchmark's tanh mesh, porous block and seeded state, run
-    on this host through the harness."""
-    if not os.path.exists(REF2048):
-        pytest.skip("reference Fortran build at 2048x2048 (oracle/_ref) absent")
+@pytest.fixture(scope="module", params=sorted(REF_FULL), ids=lambda s: "%dx%d" % s)
+def ref_full(request):
+    """The reference's own Fortran hot path at a full-physics benchmark size (built from its
+    sources with the benchmark's constants: Re 200, Pe 140, Sc 0.7, Da 1e-3, Cf 0.55, its BC
+    set and buoyancy form): 20 pseudo-steps from the benchmark's tanh mesh, porous block and
+    seeded state, run on this host through the harness."""
+    im, jm = request.param
+    binary = os.path.join(REF_DIR, REF_FULL[(im, jm)])
+    if not os.path.exists(binary):
+        pytest.skip("reference Fortran build at %dx%d (oracle/_ref) absent" % (im, jm))
     from oracle import refharness as H
-    say("case: 2048x2048 reference build, 20 pseudo-steps")
-    cfg = C.case_full_physics(2048, 2048)
-    n, im, jm = 20, 2048, 2048
+    say("case: %dx%d reference build, 20 pseudo-steps" % (im, jm))
+    cfg = C.case_full_physics(im, jm)
+    n = 20
     hd = dict(dtau=0.05, dt=cfg["params"].dt, beta=1.0, b_art=0.0, tinf=1.0)
     prm = make_params(im, jm, re=200.0, pe=140.0, sc=0.7, darcy=1e-3, cf=0.55, g=9.80665, invfr2=1.0, v_i=0.5,
                       **hd)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    buf = H.run(REF2048, [2, n, 0, 0, 0], [hd["dtau"], hd["dt"], hd["beta"], hd["b_art"], hd["tinf"], 0.0],
+    buf = H.run(binary, [2, n, 0, 0, 0], [hd["dtau"], hd["dt"], hd["beta"], hd["b_art"], hd["tinf"], 0.0],
                 H.geometry_payload(cfg["x"], cfg["y"], cfg["eps"], cfg["lp"]) + H.state_payload(cfg["state"], im, jm),
                 threads=threads)
     r = H.Reader(buf)
@@ -305,12 +308,12 @@ def ref2048():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_full_size_matches_reference_build(ref2048, prec):
-    """The library on the same 2048 x 2048 problem (first step on the exact path, then the
+def test_full_size_matches_reference_build(ref_full, prec):
+    """The library on the same full-size problem (first step on the exact path, then the
     fused kernels): FP64 every field and the residual history within 1e-10 (max relative
     difference, normwise); FP32 storage within 1e-5 (its error against FP64 is reported)."""
     from cfd_amd.params import PREC_FP32, PREC_FP64
-    cfg, prm, n, hist, ref = ref2048
+    cfg, prm, n, hist, ref = ref_full
     ctx = L.Context(prm, cfg["x"], cfg["y"], cfg["eps"], cfg["lp"], precision=PREC_FP32 if prec == "fp32" else PREC_FP64)
     ctx.set_state(cfg["state"])
     res, h = ctx.pseudo_steps(n, history=True)
