@@ -183,7 +183,8 @@ def bench_batch(args):
 def bench_warmstart(args):
     """Learned warm start on held-out cases: solver-generated steady states (one grid size)
     train the U-Net with the library's own kernels; the held-out cases are then solved from
-    the network's prediction (with fallback to zero fields) and from zero fields."""
+    the network's prediction (with fallback to zero fields) and from zero fields, and the
+    plain FP64 path (one context per case from zero fields) is priced for comparison."""
     from cfd_amd import warmstart as WS
     n = args.ws_train + args.ws_test
     H = args.ws_size
@@ -214,13 +215,33 @@ def bench_warmstart(args):
     solw = WS.solve(test_cases, tol, args.ws_itc, states=states, fallback_factor=10.0, log=log)
     solz = WS.solve(test_cases, tol, args.ws_itc, log=log)
     itw = solw["iters32"] + solw["iters64"]; itz = solz["iters32"] + solz["iters64"]
-    both = (solw["status"] == 1) & (solz["status"] == 1)
-    errs = {k: 0.0 for k in ("um_tau", "vm_tau", "p", "T_tau")}
-    for k in np.nonzero(both)[0][:32]:
-        for f in errs:
-            a, b = solw["batch"].download(int(k), f), solz["batch"].download(int(k), f)
-            errs[f] = max(errs[f], float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)))
-    solw["batch"].close(); solz["batch"].close()
+    # the plain FP64 path on the held-out cases: one context per case from zero fields. Its
+    # arithmetic is the FP64 batch's (tested bit for bit), so the FP64 batch from zero fields
+    # gives its per-case step counts, priced at one context's measured time per step
+    b64 = WS.build_batch(test_cases, PREC_FP64)
+    (_, it64, _, st64), ms64 = b64.run_sliced(tol, args.ws_itc, log=log)
+    c0 = test_cases[0]
+    ctx = L.Context(c0["params"], c0["x"], c0["y"], c0["eps"], c0["lp"])
+    ctx.set_state(c0["state"])
+    ctx.autotune(steps=10)
+    ctx.run_to_tol(0.0, 50, check_every=50, raise_on_fail=False)
+    ctx.run_to_tol(0.0, 400, check_every=100, raise_on_fail=False)
+    t_ctx = ctx.last_ms_per_step()
+    ctx.close()
+    t_plain = float(it64.sum()) * t_ctx * 1e-3
+    log("plain FP64 path: %d steps x %.4f ms = %.1f s; warm-started batch %.1f s"
+        % (int(it64.sum()), t_ctx, t_plain, solw["seconds"]))
+
+    def field_diffs(sol_a, b_ref, ok):
+        e = {k: 0.0 for k in ("um_tau", "vm_tau", "p", "T_tau")}
+        for k in np.nonzero(ok)[0][:32]:
+            for f in e:
+                a, b = sol_a.download(int(k), f), b_ref.download(int(k), f)
+                e[f] = max(e[f], float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)))
+        return e
+    errs = field_diffs(solw["batch"], solz["batch"], (solw["status"] == 1) & (solz["status"] == 1))
+    errs_plain = field_diffs(solw["batch"], b64, (solw["status"] == 1) & (st64 == 1))
+    solw["batch"].close(); solz["batch"].close(); b64.close()
     out = {
         "metric": "warm-start speedup (held-out batch time to residual, zero init / U-Net init)",
         "value": solz["seconds"] / solw["seconds"], "unit": "x", "n_gpus": 1, "higher_is_better": True,
@@ -236,6 +257,10 @@ def bench_warmstart(args):
         "converged_zero": int((solz["status"] == 1).sum()), "converged_warm": int((solw["status"] == 1).sum()),
         "solve_s_zero": solz["seconds"], "solve_s_warm": solw["seconds"],
         "steady_field_max_rel_diff_warm_vs_zero": errs,
+        "plain_fp64": {"seconds": t_plain, "ms_per_step": t_ctx, "iters_total": int(it64.sum()),
+                       "converged": int((st64 == 1).sum()), "fp64_batch_s": ms64 * 1e-3,
+                       "speedup_warm_batched": t_plain / solw["seconds"],
+                       "steady_field_max_rel_diff_warm_batched_vs_plain": errs_plain},
     }
     net.close()
     print(json.dumps(out), flush=True)
