@@ -474,12 +474,13 @@ __global__ __launch_bounds__(NTB, OCC) void k_uvp_batch(const K1Args<R> *cases, 
 // K1, split chains: the u chain and the v chain of a tile in separate waves.
 //
 // The three u stages read v only at the tau level (and v likewise u), so the two chains are
-// independent until the pressure update. A 512-thread workgroup runs the u chain in waves
-// 0-3 and the v chain in waves 4-7 over the same 256 columns; each chain holds only its own
-// register windows (no spills, two waves per SIMD), loads its own inputs, and reads the
-// other field's tau rows from deeper LDS rings (u: 8 rows, v: 16 rows). The pressure update runs in the u
-// chain with v3 from the v chain's ring. Values and operation order are those of
-// uvp_tile, so both forms give identical results.
+// independent until the pressure update. A 2 NT-thread workgroup runs the u chain in the
+// first NT lanes and the v chain in the second NT over the same NT columns; each chain holds
+// only its own register windows (no spills, two waves per SIMD), loads its own inputs, and
+// reads the other field's tau rows from deeper LDS rings (u: 8 rows, v: 16 rows). The
+// pressure update runs in the u chain with v3 from the v chain's ring. The expressions are
+// those of uvp_tile; with FMA contraction the compiler pairs some products differently, so
+// the two forms agree to a few ulps.
 // ---------------------------------------------------------------------------------------
 
 template <typename R, int NT>
@@ -976,8 +977,8 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
 }
 
 // ---------------------------------------------------------------------------------------
-// K2, split chains: T in waves 0-3, C in waves 4-7 of a 512-lane workgroup over the same 256
-// columns. T and C do not interact, so each chain keeps only its own windows (about half
+// K2, split chains: T in the first NT lanes, C in the second NT of a 2 NT-lane workgroup
+// over the same NT columns. T and C do not interact, so each chain keeps only its own windows (about half
 // the registers of scal_tile: more waves per SIMD, and room beside K1 in the two-stream
 // step). Both chains read u, v, eps and L; the T chain publishes the v row for the j+1
 // neighbour. Same expressions as scal_tile.
