@@ -356,7 +356,7 @@ def bench_unet(args):
     against MI355X's dense BF16 MFMA peak on the device time of the step (HIP events around
     the forward, backward and update; the host-to-device copies of the batch are reported
     separately as the wall time)."""
-    H = W = args.ws_size
+    H = W = args.unet_size
     B = 16
     net = L.UNet(H, W, B, seed=1)
     rng = np.random.default_rng(0)
@@ -432,7 +432,8 @@ def main():
     ap.add_argument("--acc-itc", type=int, default=300000)
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
-    ap.add_argument("--ws-size", type=int, default=64)
+    ap.add_argument("--ws-size", type=int, default=64, help="grid of the warm-start and FP32-accuracy workloads")
+    ap.add_argument("--unet-size", type=int, default=256, help="input grid of the U-Net training-step workload")
     ap.add_argument("--ws-epochs", type=int, default=30)
     ap.add_argument("--ws-itc", type=int, default=60000)
     ap.add_argument("--ws-train-tol", type=float, default=1e-5)
