@@ -52,6 +52,7 @@ struct cfd_batch {
     uint64_t *res;
     void *d_k1, *d_k2, *d_mat, *d_t1, *d_t2;
     int32_t ntiles1, ntiles2, max_mat, cus;
+    int32_t span;                      /* columns per workgroup of the batch kernels */
     cfd_tiles tiles;
     int args_dirty;
     hipEvent_t ev0, ev1;
@@ -352,6 +353,13 @@ static int build_tables(cfd_batch *b)
     if (!cases || !hb1 || !hb2 || !h1 || !h2 || !hm) r = cfd__fail(CFD_ERR_ARG, "out of host memory");
     int64_t nt1 = 0, nt2 = 0;
     int maxm = 0;
+    if (r == CFD_OK) {
+        int32_t *jm = (int32_t *)malloc((size_t)b->n * sizeof(int32_t));
+        if (!jm) r = cfd__fail(CFD_ERR_ARG, "out of host memory");
+        for (int32_t c = 0; c < b->n && r == CFD_OK; ++c) jm[c] = b->cs[c].P.jmax;
+        if (r == CFD_OK) b->span = cfdk_batch_span(b->n, jm);
+        free(jm);
+    }
     for (int32_t c = 0; c < b->n && r == CFD_OK; ++c) {
         bcase *k = &b->cs[c];
         cfd_batch_case *bc = &cases[c];
@@ -376,8 +384,8 @@ static int build_tables(cfd_batch *b)
             a->res_max = b->res + 3 * c;
             a->prec = b->prec;
         }
-        nt1 += cfdk_batch_case_tiles(1, k->P.imax, k->P.jmax, hb1[c], c, NULL);
-        nt2 += cfdk_batch_case_tiles(2, k->P.imax, k->P.jmax, hb2[c], c, NULL);
+        nt1 += cfdk_batch_case_tiles(1, b->span, k->P.imax, k->P.jmax, hb1[c], c, NULL);
+        nt2 += cfdk_batch_case_tiles(2, b->span, k->P.imax, k->P.jmax, hb2[c], c, NULL);
         const int im = k->P.imax, jm = k->P.jmax;
         const int m = 2 * (im + 1) + 4 * (jm - 2) + 4 * im + 2 * (jm - 3) + 3 * (2 * im + 2 * (jm - 2));
         if (m > maxm) maxm = m;
@@ -403,11 +411,11 @@ static int build_tables(cfd_batch *b)
         int64_t o1 = 0, o2 = 0;
         for (int32_t q = 0; q < b->n && r == CFD_OK; ++q) {
             const int32_t c = order[q];
-            o1 += cfdk_batch_case_tiles(1, b->cs[c].P.imax, b->cs[c].P.jmax, hb1[c], c, t1 + 4 * o1);
-            o2 += cfdk_batch_case_tiles(2, b->cs[c].P.imax, b->cs[c].P.jmax, hb2[c], c, t2 + 4 * o2);
+            o1 += cfdk_batch_case_tiles(1, b->span, b->cs[c].P.imax, b->cs[c].P.jmax, hb1[c], c, t1 + 4 * o1);
+            o2 += cfdk_batch_case_tiles(2, b->span, b->cs[c].P.imax, b->cs[c].P.jmax, hb2[c], c, t2 + 4 * o2);
         }
         free(order);
-        if (r == CFD_OK) r = cfdk_batch_build_args(b->prec, b->n, cases, hb1, hb2, h1, h2, hm);
+        if (r == CFD_OK) r = cfdk_batch_build_args(b->prec, b->n, cases, hb1, hb2, h1, h2, hm, b->span);
     }
     if (r == CFD_OK) {
         void *old[] = {b->d_k1, b->d_k2, b->d_mat, b->d_t1, b->d_t2};
@@ -438,11 +446,11 @@ static int make_launch(cfd_batch *b, cfd_batch_launch *L, double tol, int64_t it
     L->prec = b->prec; L->steady = b->steady; L->mupoly = b->mupoly;
     L->occ1 = b->tiles.waves_uvp >= 2 ? b->tiles.waves_uvp : 1;
     L->occ2 = (b->tiles.waves_tc == 2 || b->tiles.waves_tc == 4) ? b->tiles.waves_tc : 1;
-    L->ncases = b->n; L->max_mat = b->max_mat;
+    L->ncases = b->n; L->max_mat = b->max_mat; L->span = b->span;
     L->k1 = b->d_k1; L->k2 = b->d_k2; L->mat = b->d_mat; L->tiles1 = b->d_t1; L->tiles2 = b->d_t2;
     L->ntiles1 = b->ntiles1; L->ntiles2 = b->ntiles2;
     int o1 = 1, o2 = 1;
-    BCHK(cfdk_batch_occupancy(b->prec, b->steady, b->mupoly, L->occ1, L->occ2, &o1, &o2));
+    BCHK(cfdk_batch_occupancy(b->prec, b->steady, b->mupoly, L->occ1, L->occ2, L->span, &o1, &o2));
     const int64_t g1 = (int64_t)b->cus * o1, g2 = (int64_t)b->cus * o2;
     L->grid1 = (int32_t)(g1 < b->ntiles1 ? g1 : b->ntiles1);
     L->grid2 = (int32_t)(g2 < b->ntiles2 ? g2 : b->ntiles2);

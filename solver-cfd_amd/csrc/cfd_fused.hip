@@ -455,17 +455,17 @@ __global__ __launch_bounds__(NT, OCC) void k_uvp(const K1Args<R> a)
 // Persistent batch: `cases` holds two argument sets per case (even and odd tau parity);
 // ctl[c] bit 0 = active, bit 1 = parity. Every block walks the tile list with a stride of
 // the grid size; inactive cases are skipped.
-template <typename R, bool STEADY, bool MUPOLY, int OCC>
-__global__ __launch_bounds__(NTB, OCC) void k_uvp_batch(const K1Args<R> *cases, const BatchTile *tiles,
-                                                        int ntiles, const int *ctl)
+template <typename R, bool STEADY, bool MUPOLY, int OCC, int NT>
+__global__ __launch_bounds__(NT, OCC) void k_uvp_batch(const K1Args<R> *cases, const BatchTile *tiles,
+                                                       int ntiles, const int *ctl)
 {
-    __shared__ Smem1<R, NTB> sm;
+    __shared__ Smem1<R, NT> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile<R, STEADY, MUPOLY, NTB>(a, tl.bi, tl.bj, sm);
+        uvp_tile<R, STEADY, MUPOLY, NT>(a, tl.bi, tl.bj, sm);
         __syncthreads();   // the next tile re-initialises the LDS rings
     }
 }
@@ -775,17 +775,17 @@ __global__ __launch_bounds__(2 * NT, 1) void k_uvp_split(const K1Args<R> a)
     uvp_tile_split<R, STEADY, MUPOLY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
-template <typename R, bool STEADY, bool MUPOLY>
-__global__ __launch_bounds__(2 * NTB, 1) void k_uvp_split_batch(const K1Args<R> *cases, const BatchTile *tiles,
-                                                                int ntiles, const int *ctl)
+template <typename R, bool STEADY, bool MUPOLY, int NT>
+__global__ __launch_bounds__(2 * NT, 1) void k_uvp_split_batch(const K1Args<R> *cases, const BatchTile *tiles,
+                                                               int ntiles, const int *ctl)
 {
-    __shared__ Smem1S<R, NTB> sm;
+    __shared__ Smem1S<R, NT> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile_split<R, STEADY, MUPOLY, NTB>(a, tl.bi, tl.bj, sm);
+        uvp_tile_split<R, STEADY, MUPOLY, NT>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1123,17 +1123,17 @@ __global__ __launch_bounds__(2 * NT, 4) void k_scal_split(const K2Args<R> a)
     scal_tile_split<R, STEADY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
-template <typename R, bool STEADY>
-__global__ __launch_bounds__(2 * NTB, 4) void k_scal_split_batch(const K2Args<R> *cases, const BatchTile *tiles,
-                                                                 int ntiles, const int *ctl)
+template <typename R, bool STEADY, int NT>
+__global__ __launch_bounds__(2 * NT, 4) void k_scal_split_batch(const K2Args<R> *cases, const BatchTile *tiles,
+                                                                int ntiles, const int *ctl)
 {
-    __shared__ Smem2<R, NTB> sm;
+    __shared__ Smem2<R, NT> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        scal_tile_split<R, STEADY, NTB>(a, tl.bi, tl.bj, sm);
+        scal_tile_split<R, STEADY, NT>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1148,17 +1148,17 @@ __global__ __launch_bounds__(NT, OCC) void k_scal(const K2Args<R> a)
     scal_tile<R, STEADY, NT>(a, L / a.nbj, L % a.nbj, sm);
 }
 
-template <typename R, bool STEADY, int OCC>
-__global__ __launch_bounds__(NTB, OCC) void k_scal_batch(const K2Args<R> *cases, const BatchTile *tiles,
-                                                         int ntiles, const int *ctl)
+template <typename R, bool STEADY, int OCC, int NT>
+__global__ __launch_bounds__(NT, OCC) void k_scal_batch(const K2Args<R> *cases, const BatchTile *tiles,
+                                                        int ntiles, const int *ctl)
 {
-    __shared__ Smem2<R, NTB> sm;
+    __shared__ Smem2<R, NT> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        scal_tile<R, STEADY, NTB>(a, tl.bi, tl.bj, sm);
+        scal_tile<R, STEADY, NT>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1582,29 +1582,43 @@ __global__ void k_batch_count(const int *ctl, int n, int *out)
     if (threadIdx.x == 0) *out = part[0];
 }
 
-template <typename R, bool STEADY, bool MUPOLY>
-void batch_launch_k1(const cfd_batch_launch *L, hipStream_t s)
+template <typename R, bool STEADY, bool MUPOLY, int NT>
+void batch_launch_k1_nt(const cfd_batch_launch *L, hipStream_t s)
 {
-    const dim3 g1(L->grid1), b1(NTB);
+    const dim3 g1(L->grid1), b1(NT);
     const K1Args<R> *a = (const K1Args<R> *)L->k1;
     const BatchTile *t = (const BatchTile *)L->tiles1;
     if (L->occ1 == 3)
-        hipLaunchKernelGGL((k_uvp_split_batch<R, STEADY, MUPOLY>), g1, dim3(2 * NTB), 0, s, a, t, L->ntiles1, L->ctl);
+        hipLaunchKernelGGL((k_uvp_split_batch<R, STEADY, MUPOLY, NT>), g1, dim3(2 * NT), 0, s, a, t, L->ntiles1, L->ctl);
     else if (L->occ1 == 2)
-        hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 2>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
-    else hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 1>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
+        hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 2, NT>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
+    else hipLaunchKernelGGL((k_uvp_batch<R, STEADY, MUPOLY, 1, NT>), g1, b1, 0, s, a, t, L->ntiles1, L->ctl);
+}
+
+template <typename R, bool STEADY, bool MUPOLY>
+void batch_launch_k1(const cfd_batch_launch *L, hipStream_t s)
+{
+    if (L->span == NT1) batch_launch_k1_nt<R, STEADY, MUPOLY, NT1>(L, s);
+    else batch_launch_k1_nt<R, STEADY, MUPOLY, NTB>(L, s);
+}
+
+template <typename R, bool STEADY, int NT>
+void batch_launch_k2_nt(const cfd_batch_launch *L, hipStream_t s)
+{
+    const dim3 g2(L->grid2), b2(NT);
+    const K2Args<R> *a = (const K2Args<R> *)L->k2;
+    const BatchTile *t = (const BatchTile *)L->tiles2;
+    if (L->occ2 == 2)
+        hipLaunchKernelGGL((k_scal_split_batch<R, STEADY, NT>), g2, dim3(2 * NT), 0, s, a, t, L->ntiles2, L->ctl);
+    else if (L->occ2 == 4) hipLaunchKernelGGL((k_scal_batch<R, STEADY, 4, NT>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
+    else hipLaunchKernelGGL((k_scal_batch<R, STEADY, 1, NT>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
 }
 
 template <typename R, bool STEADY>
 void batch_launch_k2(const cfd_batch_launch *L, hipStream_t s)
 {
-    const dim3 g2(L->grid2), b2(NTB);
-    const K2Args<R> *a = (const K2Args<R> *)L->k2;
-    const BatchTile *t = (const BatchTile *)L->tiles2;
-    if (L->occ2 == 2)
-        hipLaunchKernelGGL((k_scal_split_batch<R, STEADY>), g2, dim3(2 * NTB), 0, s, a, t, L->ntiles2, L->ctl);
-    else if (L->occ2 == 4) hipLaunchKernelGGL((k_scal_batch<R, STEADY, 4>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
-    else hipLaunchKernelGGL((k_scal_batch<R, STEADY, 1>), g2, b2, 0, s, a, t, L->ntiles2, L->ctl);
+    if (L->span == NT1) batch_launch_k2_nt<R, STEADY, NT1>(L, s);
+    else batch_launch_k2_nt<R, STEADY, NTB>(L, s);
 }
 
 template <typename R>
@@ -1644,7 +1658,7 @@ __global__ void k_apply_bc_batch(const MatArgs<F> *mats, const int *ctl)
 
 template <typename R>
 void build_args(int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1, const int32_t *hb2, void *k1v,
-                void *k2v, void *matv)
+                void *k2v, void *matv, int span)
 {
     K1Args<R> *k1 = (K1Args<R> *)k1v;
     K2Args<R> *k2 = (K2Args<R> *)k2v;
@@ -1656,8 +1670,8 @@ void build_args(int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1,
             cfd_fused_args a = b.a[q];
             a.hb1 = hb1 ? hb1[c] : 0;
             a.hb2 = hb2 ? hb2[c] : 0;
-            fill_k1<R>(k1[2 * c + q], im, jm, b.pack, &b.P, &a, NTB);
-            fill_k2<R>(k2[2 * c + q], im, jm, b.y, b.pack, &b.P, &a, NTB);
+            fill_k1<R>(k1[2 * c + q], im, jm, b.pack, &b.P, &a, span);
+            fill_k2<R>(k2[2 * c + q], im, jm, b.y, b.pack, &b.P, &a, span);
             MatArgs<R> &m = mat[2 * c + q];
             m.g = Geo{};
             m.g.y = b.y; m.g.imax = im; m.g.jmax = jm;
@@ -1683,16 +1697,30 @@ int cfdk_batch_arg_sizes(int32_t prec, size_t *k1, size_t *k2, size_t *mat)
 }
 
 int cfdk_batch_build_args(int32_t prec, int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1,
-                          const int32_t *hb2, void *k1, void *k2, void *mat)
+                          const int32_t *hb2, void *k1, void *k2, void *mat, int32_t span)
 {
-    if (prec == CFD_PREC_FP32) build_args<float>(ncases, cases, hb1, hb2, k1, k2, mat);
-    else build_args<double>(ncases, cases, hb1, hb2, k1, k2, mat);
+    if (span != NTB && span != NT1) return CFD_ERR_ARG;
+    if (prec == CFD_PREC_FP32) build_args<float>(ncases, cases, hb1, hb2, k1, k2, mat, span);
+    else build_args<double>(ncases, cases, hb1, hb2, k1, k2, mat, span);
     return CFD_OK;
 }
 
-int cfdk_batch_case_tiles(int kernel, int32_t imax, int32_t jmax, int32_t hb, int32_t c, int32_t *out)
+// Column span of the batch's workgroups: 64 unless 256-column strips need fewer lanes over
+// all cases (wide grids: a 2048-column case takes 2304 lanes in 256-column strips, 2624 in
+// 64-column ones), as for a single grid.
+int32_t cfdk_batch_span(int32_t ncases, const int32_t *jmax)
 {
-    const int W = kernel == 1 ? w1(NTB) : w2(NTB);
+    long lanes_b = 0, lanes_1 = 0;
+    for (int32_t c = 0; c < ncases; ++c) {
+        lanes_b += (long)cdiv(jmax[c] - 2, w1(NTB)) * NTB;
+        lanes_1 += (long)cdiv(jmax[c] - 2, w1(NT1)) * NT1;
+    }
+    return lanes_b <= lanes_1 ? NTB : NT1;
+}
+
+int cfdk_batch_case_tiles(int kernel, int32_t span, int32_t imax, int32_t jmax, int32_t hb, int32_t c, int32_t *out)
+{
+    const int W = kernel == 1 ? w1(span) : w2(span);
     const int nbj = cdiv(jmax - 2, W);
     int h = 0;
     const int nbi = pick_rows(imax - 2, nbj, kernel == 1 ? 24 : 16, hb, &h);
@@ -1706,19 +1734,22 @@ int cfdk_batch_case_tiles(int kernel, int32_t imax, int32_t jmax, int32_t hb, in
     return nbi * nbj;
 }
 
-int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2,
-                         int *k1_blocks_per_cu, int *k2_blocks_per_cu)
+}  // extern "C"
+
+namespace {
+
+template <int NT>
+hipError_t batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2, int *b1, int *b2)
 {
-    int b1 = 0, b2 = 0;
     hipError_t e1, e2;
 #define OCC1(R, S, M)                                                                                      \
-    (occ1 == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_split_batch<R, S, M>, 2 * NTB, 0) \
-     : occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 2>, NTB, 0)     \
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_uvp_batch<R, S, M, 1>, NTB, 0))
+    (occ1 == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b1, k_uvp_split_batch<R, S, M, NT>, 2 * NT, 0) \
+     : occ1 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b1, k_uvp_batch<R, S, M, 2, NT>, NT, 0)    \
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(b1, k_uvp_batch<R, S, M, 1, NT>, NT, 0))
 #define OCC2(R, S)                                                                                         \
-    (occ2 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_split_batch<R, S>, 2 * NTB, 0)  \
-     : occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 4>, NTB, 0)       \
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_scal_batch<R, S, 1>, NTB, 0))
+    (occ2 == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b2, k_scal_split_batch<R, S, NT>, 2 * NT, 0) \
+     : occ2 == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(b2, k_scal_batch<R, S, 4, NT>, NT, 0)      \
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(b2, k_scal_batch<R, S, 1, NT>, NT, 0))
     if (prec == CFD_PREC_FP32) {
         if (steady) { e1 = mupoly ? OCC1(float, true, true) : OCC1(float, true, false); e2 = OCC2(float, true); }
         else { e1 = mupoly ? OCC1(float, false, true) : OCC1(float, false, false); e2 = OCC2(float, false); }
@@ -1728,7 +1759,20 @@ int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t o
     }
 #undef OCC1
 #undef OCC2
-    if (e1 != hipSuccess || e2 != hipSuccess) return CFD_ERR_HIP;
+    return e1 != hipSuccess ? e1 : e2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2, int32_t span,
+                         int *k1_blocks_per_cu, int *k2_blocks_per_cu)
+{
+    int b1 = 0, b2 = 0;
+    const hipError_t e = span == NT1 ? batch_occupancy<NT1>(prec, steady, mupoly, occ1, occ2, &b1, &b2)
+                                     : batch_occupancy<NTB>(prec, steady, mupoly, occ1, occ2, &b1, &b2);
+    if (e != hipSuccess) return CFD_ERR_HIP;
     *k1_blocks_per_cu = b1 > 0 ? b1 : 1;
     *k2_blocks_per_cu = b2 > 0 ? b2 : 1;
     return CFD_OK;

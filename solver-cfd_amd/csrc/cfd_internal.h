@@ -121,12 +121,16 @@ int cfdk_batch_arg_sizes(int32_t prec, size_t *k1, size_t *k2, size_t *mat);
 /* Fill the three host tables (2 entries per case, parity-major within a case) and the tile
  * geometry: rows per tile hb1/hb2 per case (0: built-in rule). */
 int cfdk_batch_build_args(int32_t prec, int32_t ncases, const cfd_batch_case *cases, const int32_t *hb1,
-                          const int32_t *hb2, void *k1, void *k2, void *mat);
+                          const int32_t *hb2, void *k1, void *k2, void *mat, int32_t span);
+/* Column span (lanes) of the batch's workgroups for these case widths: 64 or 256. */
+int32_t cfdk_batch_span(int32_t ncases, const int32_t *jmax);
 /* Tiles of one case: count and fill (c, bi, bj) triples into out (4 int32 per tile). */
-int cfdk_batch_case_tiles(int kernel, int32_t imax, int32_t jmax, int32_t hb, int32_t c, int32_t *out);
+int cfdk_batch_case_tiles(int kernel, int32_t span, int32_t imax, int32_t jmax, int32_t hb, int32_t c,
+                          int32_t *out);
 
 typedef struct cfd_batch_launch {
     int32_t prec, steady, mupoly, occ1, occ2, ncases, max_mat;
+    int32_t span;                    /* columns per workgroup of the batch kernels (64 or 256) */
     const void *k1, *k2, *mat;       /* device argument tables */
     const void *tiles1, *tiles2;     /* device tile lists (4 int32 per tile) */
     int32_t ntiles1, ntiles2, grid1, grid2;
@@ -142,7 +146,7 @@ typedef struct cfd_batch_launch {
 } cfd_batch_launch;
 
 /* Blocks per CU the batch kernels can keep resident (for the persistent grid size). */
-int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2,
+int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2, int32_t span,
                          int *k1_blocks_per_cu, int *k2_blocks_per_cu);
 /* One batched pseudo-step: residual slots, K1, K2, boundary cells, convergence control. */
 int cfdk_batch_step(const cfd_batch_launch *L, hipStream_t s);

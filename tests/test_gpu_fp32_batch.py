@@ -116,10 +116,10 @@ def test_fp32_physical_steps_close_to_fp64():
 BATCH_SHAPES = [(40, 70, 31, 80.0), (96, 33, 32, 300.0), (150, 260, 33, 25.0), (24, 20, 34, 150.0)]
 
 
-def _bc_consistent(prec, steady):
+def _bc_consistent(prec, steady, shapes=BATCH_SHAPES):
     """Cases stepped once through a context (exact first step): BC-consistent iterates."""
     cases = []
-    for (im, jm, seed, re) in BATCH_SHAPES:
+    for (im, jm, seed, re) in shapes:
         prm, x, y, eps, lp, st = _case(im, jm, seed=seed, steady=steady, re=re, da=10 ** (-1 - seed % 3))
         ctx = L.Context(prm, x, y, eps, lp, precision=prec)
         ctx.set_state(st)
@@ -155,6 +155,27 @@ def test_batch_matches_single_contexts_bitwise(prec, steady, waves, waves_tc):
     for i, (prm, x, y, eps, lp, st1, ctx) in enumerate(cases):
         ctx.set_tiles(24, 12, waves, waves_tc)
         r = ctx.pseudo_steps(25)
+        for k in FIELDS:
+            a, c = b.download(i, k), ctx.download(k)
+            assert np.array_equal(a, c), (i, k, max_rel(a, c))
+        assert np.array_equal(res[i], r), (i, res[i], r)
+        ctx.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_wide_batch_matches_single_contexts_bitwise(prec):
+    """Cases 700 and 1030 columns wide: the batch takes 256-column workgroups (fewer lanes
+    than 64-column ones for these widths) and still reproduces the contexts bit for bit."""
+    say("case: wide batch vs contexts", "fp32" if prec == PREC_FP32 else "fp64")
+    cases = _bc_consistent(prec, 1, shapes=[(40, 1030, 41, 80.0), (30, 700, 42, 120.0)])
+    b = _make_batch(cases, prec)
+    b.set_tiles(16, 8, 1, 1)
+    code, iters, res, status = b.run(0.0, 20, steps_per_graph=10)
+    assert code == 0 and list(iters) == [20] * len(cases)
+    for i, (prm, x, y, eps, lp, st1, ctx) in enumerate(cases):
+        ctx.set_tiles(16, 8, 1, 1)
+        r = ctx.pseudo_steps(20)
         for k in FIELDS:
             a, c = b.download(i, k), ctx.download(k)
             assert np.array_equal(a, c), (i, k, max_rel(a, c))
