@@ -6,17 +6,18 @@
 // index grid, from which the solver's staggered initial fields are taken.
 //
 // Numerics: activations and their gradients are stored in BF16 (NHWC), every convolution
-// is an implicit GEMM on the CDNA matrix cores (BF16 operands, FP32 accumulation): layers
-// whose channel counts are multiples of 32 run spatially tiled kernels on the gfx950
-// v_mfma_f32_32x32x16_bf16, the first layer and the head a generic im2col kernel on
-// v_mfma_f32_32x32x8bf16_1k. Master weights, weight gradients, biases and the Adam state
-// are FP32.
+// is an implicit GEMM on the CDNA matrix cores (BF16 operands, FP32 accumulation) in
+// spatially tiled kernels on the gfx950 v_mfma_f32_32x32x16_bf16; layers whose channel
+// counts are off the 32 grid (the first layer's 5 inputs, the head's 4 outputs) use the same
+// kernels with element loads and zero fill. Master weights, weight gradients, biases and the
+// Adam state are FP32.
 //
 //   forward   y[m][n]  = sum_k A[m][k] W[n][k] + b[n]     m = pixel, n = cout,
 //                                                          k = (dy, dx, cin), A = im2col(x)
 //   input grad dx = the same kernel on dy with the flipped, transposed weights
 //             Wf[ci][(dy', dx', co)] = W[co][(2 - dy', 2 - dx')][ci]
 //   weight grad dW[n][k] = sum_m dy[m][n] A[m][k]           (MFMA over m, split over the grid)
+//   bias grad  db[n]    = sum_m dy[m][n]                      (in the weight-gradient pass)
 //
 // Network: enc1 (5->32->32) pool enc2 (->64->64) pool enc3 (->128->128) pool bottom
 // (->256->256); up + concat enc3 -> (384->128->128); up + concat enc2 -> (192->64->64);
