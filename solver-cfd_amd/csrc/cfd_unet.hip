@@ -911,8 +911,6 @@ struct cfd_unet {
 
 namespace {
 
-const int CH[4] = {32, 64, 128, 256};
-
 // conv indices: enc1a enc1b enc2a enc2b enc3a enc3b bota botb dec3a dec3b dec2a dec2b dec1a dec1b head
 void build_convs(cfd_unet *u)
 {
