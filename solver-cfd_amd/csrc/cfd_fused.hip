@@ -1272,7 +1272,7 @@ void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params
     k1.invfr2 = P->invfr2; k1.ri = P->ri; k1.t_ref = P->t_ref; k1.v_i = P->v_i; k1.u_in = P->u_in;
     for (int q = 0; q < 5; ++q) k1.mu[q] = P->mu_coef[q];
     k1.nbj = cdiv(jm - 2, w1(nt));
-    k1.nbi = pick_rows(im - 2, k1.nbj, a->lone ? 8 : 24, a->hb1, &k1.hb);
+    k1.nbi = pick_rows(im - 2, k1.nbj, a->lone ? 4 : 24, a->hb1, &k1.hb);
 }
 
 template <typename R>
@@ -1289,7 +1289,7 @@ void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack,
     k2.tinf = P->tinf; k2.t_bottom = P->t_bottom; k2.t_top = P->t_top; k2.v_i = P->v_i; k2.u_in = P->u_in;
     k2.slot_lo = P->slot_lo; k2.slot_hi = P->slot_hi; k2.slot_value = P->slot_value;
     k2.nbj = cdiv(jm - 2, w2(nt));
-    k2.nbi = pick_rows(im - 2, k2.nbj, a->lone ? 8 : 16, a->hb2, &k2.hb);
+    k2.nbi = pick_rows(im - 2, k2.nbj, a->lone ? 4 : 16, a->hb2, &k2.hb);
 }
 
 // Column span of a single grid's workgroups: 64 when 64-column strips need no more lanes

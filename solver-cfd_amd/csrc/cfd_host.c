@@ -1065,13 +1065,18 @@ static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_
  * 512-lane workgroup, each with half the register windows. The T/C kernel runs 3 waves per
  * SIMD as compiled, 4 with spills, or (variant 2) the T and C chains in separate waves of a
  * 512-lane workgroup. Shapes are timed with the pipeline as configured. */
+/* small row blocks (4-12 rows) serve grids too small to fill the CUs, whose step time is
+ * the row count of one block plus the stage lags */
 static const cfd_tiles CAND_UVP[] = {
-    {0, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0}, {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
+    {0, 0, 1, 0}, {4, 0, 1, 0}, {8, 0, 1, 0}, {12, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0},
+    {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
     {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
-    {0, 0, 3, 0}, {16, 0, 3, 0}, {24, 0, 3, 0}, {32, 0, 3, 0}, {48, 0, 3, 0}, {64, 0, 3, 0}, {96, 0, 3, 0},
+    {0, 0, 3, 0}, {4, 0, 3, 0}, {8, 0, 3, 0}, {12, 0, 3, 0}, {16, 0, 3, 0}, {24, 0, 3, 0}, {32, 0, 3, 0},
+    {48, 0, 3, 0}, {64, 0, 3, 0}, {96, 0, 3, 0},
 };
 static const cfd_tiles CAND_TC[] = {
-    {0, 0, 0, 1}, {0, 8, 0, 1}, {0, 12, 0, 1}, {0, 16, 0, 1}, {0, 24, 0, 1}, {0, 32, 0, 1}, {0, 64, 0, 1},
+    {0, 0, 0, 1}, {0, 4, 0, 1}, {0, 8, 0, 1}, {0, 12, 0, 1}, {0, 16, 0, 1}, {0, 24, 0, 1}, {0, 32, 0, 1},
+    {0, 64, 0, 1}, {0, 4, 0, 2},
     {0, 8, 0, 4}, {0, 12, 0, 4}, {0, 16, 0, 4}, {0, 24, 0, 4}, {0, 32, 0, 4},
     {0, 0, 0, 2}, {0, 8, 0, 2}, {0, 16, 0, 2}, {0, 32, 0, 2}, {0, 64, 0, 2},
 };

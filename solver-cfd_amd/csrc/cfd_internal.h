@@ -71,7 +71,7 @@ typedef struct cfd_fused_args {
      * chosen by the autotuner */
     int32_t hb1, hb2, occ1, occ2;
     /* built-in heuristic for a lone grid (1) or a batch case (0): a lone grid that fills
-     * fewer blocks than there are CUs takes row blocks down to 8 rows, since its step time
+     * fewer blocks than there are CUs takes row blocks down to 4 rows, since its step time
      * is the row count of one block (plus the stage lags) times the per-row latency; batch
      * tiles keep 24 (K1) and 16 (K2) rows, where halo rows cost throughput */
     int32_t lone;

@@ -156,11 +156,12 @@ def test_graph_replays_match_eager():
 
 
 @pytest.mark.parametrize("tiles", [(16, 8, 1, 1), (37, 11, 2, 4), (64, 64, 2, 1), (24, 5, 1, 4), (29, 8, 3, 1),
-                                   (0, 0, 3, 4), (24, 13, 1, 2), (0, 0, 3, 2)])
+                                   (0, 0, 3, 4), (24, 13, 1, 2), (0, 0, 3, 2), (4, 4, 1, 1), (4, 4, 3, 2),
+                                   (3, 2, 1, 1)])
 def test_fused_explicit_tiles_match_oracle(tiles):
     """Row-block heights other than the built-in choice (including ones that do not divide
-    the row count) and both register-budget variants of each kernel give the same
-    trajectory."""
+    the row count and ones shorter than the stage lags) and the register-budget variants of
+    each kernel give the same trajectory."""
     say("case: explicit tiles", tiles)
     prm, x, y, eps, lp, st = _multiblock_case(180, 300, bc=0, invfr2=0.7, g=1.5)
     o = O.Oracle(prm, x, y, eps, lp)
@@ -249,7 +250,7 @@ def test_autotune_restores_state_and_keeps_parity():
     tiles, ms = ctx.autotune(steps=3)
     say("tuned", tiles, ms)
     rows_uvp, rows_tc, waves_uvp, waves_tc = tiles
-    assert rows_uvp in (0, 16, 24, 32, 40, 48, 64, 96, 128) and rows_tc in (0, 8, 12, 16, 24, 32, 64)
+    assert rows_uvp in (0, 4, 8, 12, 16, 24, 32, 40, 48, 64, 96, 128) and rows_tc in (0, 4, 8, 12, 16, 24, 32, 64)
     assert waves_uvp in (1, 2, 3) and waves_tc in (1, 2, 4)
     assert ms > 0 and ctx.get_tiles() == tiles
     assert ctx.step_counts() == (0, 0)
