@@ -236,10 +236,18 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             if (!STEADY) q_vm = a.vm[(size_t)rp * sv + j];
         }
     };
+    // Rows run two at a time, so that the window shifts of the first row of a pair are
+    // register renames rather than moves. Pairs start at even rows: a row is always computed
+    // by the same half of the pair, whatever the block, so blocks of different heights (a
+    // batch case against a single context) give bit-identical results. The loop may thus
+    // start one row before ilo - 6 and end one row after ihi + 6; in those rows every stage,
+    // the P update and every global store are outside their row ranges (only ring slots that
+    // no computed row reads are written).
+    const int t0 = (ilo - 6) - ((ilo - 6) & 1);
     __syncthreads();
-    fetch(ilo - 6);
+    fetch(t0);
 
-    for (int t = ilo - 6; t <= ihi + 6; ++t) {
+    auto row = [&](const int t) __attribute__((always_inline)) {
         // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
         const R ut_new = q_ut, vt_new = q_vt, T_rp = q_T, e = q_e, Lp = q_L;
         const double p_rp = q_p;
@@ -434,6 +442,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 const unsigned long long bb = dbits(dd); mp_ = bb > mp_ ? bb : mp_;
             }
         }
+    };
+    for (int t = t0; t <= ihi + 6; t += 2) {
+        row(t);
+        row(t + 1);
     }
     fold_max(a.res + 0, mu_);
     fold_max(a.res + 1, mv_);
