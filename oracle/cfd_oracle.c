@@ -489,6 +489,7 @@ double orc_solve_u(orc_ctx *o, const double *um, double *um_tau, double *vm_tau,
             um_n_tau[k] = 1.0 / 3.0 * um_tau[k] + 2.0 / 3.0 * (ui[k] + res[k]);
         }
     orc_bcuv(o, um_n_tau, vm_tau);
+    /* c_cuda/solve_U.cu:58: max_reduce(dev_res_u, imax-1, jmax-2) */
     if (o->P.compat == CFD_COMPAT_CUDA)
         return maxabs_linear_(res, (long)(im - 1) * (jm - 2));
     return orc_maxabs(res, jm + 1, 3, im - 1, 2, jm - 1);
@@ -525,8 +526,9 @@ double orc_solve_v(orc_ctx *o, const double *vm, double *um_tau, double *vm_tau,
             vm_n_tau[k] = 1.0 / 3.0 * vm_tau[k] + 2.0 / 3.0 * (vi[k] + res[k]);
         }
     orc_bcuv(o, um_tau, vm_n_tau);
+    /* c_cuda/solve_V.cu:52: max_reduce(dev_res_v, imax-1, jmax) */
     if (o->P.compat == CFD_COMPAT_CUDA)
-        return maxabs_linear_(res, (long)im * (jm + 1));
+        return maxabs_linear_(res, (long)(im - 1) * jm);
     return orc_maxabs(res, jm + 2, 2, im - 1, 3, jm - 1);
 }
 
@@ -560,6 +562,7 @@ double orc_solve_p(orc_ctx *o, const double *p, const double *um_n, const double
             pn[k] = 1.0 / 3.0 * p[k] + 2.0 / 3.0 * (pi[k] + res[k]);
         }
     orc_bcp(o, pn);
+    /* c_cuda/solve_P.cu:50: max_reduce(dev_res_p, imax, jmax) */
     if (o->P.compat == CFD_COMPAT_CUDA)
         return maxabs_linear_(res, (long)im * jm);
     return orc_maxabs(res, jm + 1, 2, im - 1, 2, jm - 1);

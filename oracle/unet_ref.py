@@ -181,3 +181,15 @@ def forward_backward(theta, x, t, bf=True):
 
 def loss_only(theta, x, t):
     return forward_backward(theta, x, t, bf=False)[0]
+
+
+def adam(theta, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) in float64: moments m, v updated from
+    the gradient g, bias-corrected by 1 - b1^step and 1 - b2^step, and the step
+    lr * m_hat / (sqrt(v_hat) + eps) taken. Returns (theta', m', v')."""
+    g = np.asarray(g, dtype=np.float64)
+    m = b1 * np.asarray(m, dtype=np.float64) + (1.0 - b1) * g
+    v = b2 * np.asarray(v, dtype=np.float64) + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** step)
+    vh = v / (1.0 - b2 ** step)
+    return np.asarray(theta, dtype=np.float64) - lr * mh / (np.sqrt(vh) + eps), m, v

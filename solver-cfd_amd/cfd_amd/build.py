@@ -55,13 +55,13 @@ def build(verbose=False):
         objs.append(obj)
     for src in C_SOURCES:
         obj = os.path.join(BUILD, src.replace(".c", ".o"))
-        _run(["gcc", "-std=c11", "-O2", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+        _run(["gcc", "-std=c11", "-O2", "-fPIC", "-pthread", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
               "-I" + os.path.join(ROCM, "include"), "-I" + os.path.join(REPO, "include"),
               "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-lm", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-lm", "-lpthread", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     # the Fortran binding module is compile-checked when a Fortran compiler is present
     fc = os.path.join(ROCM, "lib", "llvm", "bin", "flang")

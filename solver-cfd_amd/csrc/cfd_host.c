@@ -16,6 +16,7 @@
 #include "../../include/cfd_legacy.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -521,8 +522,9 @@ static int x_residual(const xenv *e, int region, const double *res, uint64_t *sl
     const int im = e->g->imax, jm = e->g->jmax;
     if (e->P->compat == CFD_COMPAT_CUDA) {
         /* max_reduce(dev_res, n, m) over the first n*m linear entries (max_reduce.cu:14-41),
-         * called with (imax-1, jmax-2) for u, (imax, jmax+1)... as solve_*.cu pass them. */
-        long n = region == 0 ? (long)(im - 1) * (jm - 2) : (region == 1 ? (long)im * (jm + 1) : (long)im * jm);
+         * called with (imax-1, jmax-2) for u (solve_U.cu:58), (imax-1, jmax) for v
+         * (solve_V.cu:52) and (imax, jmax) for p (solve_P.cu:50). */
+        long n = region == 0 ? (long)(im - 1) * (jm - 2) : (region == 1 ? (long)(im - 1) * jm : (long)im * jm);
         return cfdk_maxabs_linear(res, n, slot, e->s);
     }
     if (region == 0) return cfdk_maxabs(res, jm + 1, 3, im - 1, 2, jm - 1, slot, e->s);
@@ -929,31 +931,55 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
     if (steps_per_graph < 2 || (steps_per_graph & 1)) return fail(CFD_ERR_ARG, "steps_per_graph must be even and >= 2");
     if (!c->owns_stream) return fail(CFD_ERR_STATE, "graph capture needs a context stream (not the legacy default stream)");
     if (replays < 1) return fail(CFD_ERR_ARG, "replays must be >= 1");
-    /* the first step after new inputs goes through the exact path, outside the graph */
-    if (!fused_ok(c) && !c->bc_clean) {
-        CHK(ensure_hist(c, 1));
+    CHK(ensure_hist(c, 1));
+    /* A replay repeats the captured launches with the pointers fixed at capture, so every
+     * captured step must be one that the next step would repeat as it is. Steps that are
+     * not run here, outside the graph: the first step after new inputs (exact path, with
+     * the reference's side effects and the vm_n corner seeding), and a step that reads pn
+     * after a physical step stopped without the tau update. One exact step leaves the
+     * iterates boundary-consistent, reading p, with the first-step seeding done. */
+    if (c->first_step || c->uv_reads_pn || !c->bc_clean) {
         CHK(one_step(c, c->d_hist));
         tau_swap(c);
     }
+    if (c->first_step || c->uv_reads_pn || !c->bc_clean)
+        return fail(CFD_ERR_STATE, "the iterates are not in a state that a graph replay can repeat");
+    const int fused = fused_ok(c);
+    if (c->kernel == 1 && c->have_fused && c->P.b_art == 0.0 && c->P.compat == CFD_COMPAT_FORTRAN && !fused)
+        return fail(CFD_ERR_STATE, "fused steps are not available for the graph");
     /* precision conversions happen outside the graph: it replays fused steps only */
-    if (fused_ok(c) && c->prec == CFD_PREC_FP32) CHK(to32(c));
-    CHK(ensure_hist(c, 1));
+    if (fused && c->prec == CFD_PREC_FP32) CHK(to32(c));
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
     CHK(pipe_join(c));            /* no wait inside the graph on work from before it */
+    /* buffer roles and step counts before the capture: a failed capture executes nothing */
+    double *st0[S_NSTATE];
+    float *s320[S_NSTATE];
+    memcpy(st0, c->st, sizeof st0);
+    memcpy(s320, c->s32, sizeof s320);
+    const int64_t nf0 = c->n_fused, ne0 = c->n_exact, pipe0 = c->pipe_n;
+    const int pend0 = c->pending_in_tau;
     HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     int r = CFD_OK;
     for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) {
         r = one_step(c, c->d_hist);
-        tau_swap(c);
+        if (r == CFD_OK) tau_swap(c);
     }
     if (r == CFD_OK) r = pipe_join(c);    /* the second stream rejoins before the capture ends */
     c->ek2_live[0] = c->ek2_live[1] = 0;
     hipError_t e = hipStreamEndCapture(c->s, &graph);
-    if (r != CFD_OK) { if (graph) hipGraphDestroy(graph); return r; }
-    if (e != hipSuccess) return fail(CFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&exec, graph, NULL, NULL, 0);
-    if (e != hipSuccess) { hipGraphDestroy(graph); return fail(CFD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+    if (r == CFD_OK && e == hipSuccess) e = hipGraphInstantiate(&exec, graph, NULL, NULL, 0);
+    if (r != CFD_OK || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        memcpy(c->st, st0, sizeof st0);
+        memcpy(c->s32, s320, sizeof s320);
+        c->n_fused = nf0; c->n_exact = ne0; c->pipe_n = pipe0; c->pending_in_tau = pend0;
+        if (r != CFD_OK) return r;
+        return fail(CFD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    /* each replay runs the captured steps once more */
+    c->n_fused = nf0 + (c->n_fused - nf0) * replays;
+    c->n_exact = ne0 + (c->n_exact - ne0) * replays;
     HIPCHK(hipEventRecord(c->ev0, c->s));
     for (int32_t q = 0; q < replays && e == hipSuccess; ++q) e = hipGraphLaunch(exec, c->s);
     hipEventRecord(c->ev1, c->s);
@@ -983,6 +1009,8 @@ typedef struct tile_entry {
 enum { TILE_CACHE = 32 };
 static tile_entry g_tiles[TILE_CACHE];
 static int g_ntiles;
+/* contexts on different host threads share the cache */
+static pthread_mutex_t g_tiles_lock = PTHREAD_MUTEX_INITIALIZER;
 
 static int tile_key_eq(const tile_entry *e, const cfd_ctx *c)
 {
@@ -993,12 +1021,15 @@ static int tile_key_eq(const tile_entry *e, const cfd_ctx *c)
 
 static void tiles_from_cache(cfd_ctx *c)
 {
+    pthread_mutex_lock(&g_tiles_lock);
     for (int k = 0; k < g_ntiles; ++k)
-        if (tile_key_eq(&g_tiles[k], c)) { c->tiles = g_tiles[k].t; c->tiles_set = 1; return; }
+        if (tile_key_eq(&g_tiles[k], c)) { c->tiles = g_tiles[k].t; c->tiles_set = 1; break; }
+    pthread_mutex_unlock(&g_tiles_lock);
 }
 
 static void tiles_to_cache(const cfd_ctx *c)
 {
+    pthread_mutex_lock(&g_tiles_lock);
     int k = 0;
     while (k < g_ntiles && !tile_key_eq(&g_tiles[k], c)) ++k;
     if (k == g_ntiles) {
@@ -1008,6 +1039,7 @@ static void tiles_to_cache(const cfd_ctx *c)
     const cfd_params *P = &c->P;
     tile_entry e = {P->imax, P->jmax, c->prec, P->steady, P->mu_model, P->scheme, P->bc, P->buoyancy, c->tiles};
     g_tiles[k] = e;
+    pthread_mutex_unlock(&g_tiles_lock);
 }
 
 static int tiles_valid(const cfd_ctx *c, const cfd_tiles *t)

@@ -19,6 +19,7 @@ from helpers import load_golden, max_rel, field_rel
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10   # per field: max |a - b| / max |b| (the FP64 drop-in bar)
+PER_ENTRY = 1e-8   # per entry, |a - b| / max(|b|, 1e-6 max |b|)
 FIELDS = ("um_tau", "vm_tau", "p", "T_tau", "C_tau")
 
 
@@ -41,11 +42,17 @@ def test_fused_trajectory_matches_reference(name):
     ctx.set_state(st)
     res, hist = ctx.pseudo_steps(meta["nsteps"], history=True)
     nf, ne = ctx.step_counts()
-    errs = {k: max_rel(ctx.download(k), out[k]) for k in FIELDS}
+    got = {k: ctx.download(k) for k in FIELDS}
+    errs = {k: max_rel(got[k], out[k]) for k in FIELDS}
+    # per entry: |a - b| / max(|b|, floor * max|b|), at two floors
+    ent = {k: field_rel(got[k], out[k]) for k in FIELDS}
+    ent6 = {k: field_rel(got[k], out[k], floor=1e-6) for k in FIELDS}
     h = max_rel(hist, out["hist"])
-    say(name, "fused steps", nf, "exact steps", ne, "history", h, "fields", errs)
+    say(name, "fused steps", nf, "exact steps", ne, "history", h, "fields (normwise)", errs,
+        "per entry (floor 1e-12)", ent, "per entry (floor 1e-6)", ent6)
     assert nf == meta["nsteps"] - 1 and ne == 1
     assert h <= TOL and max(errs.values()) <= TOL
+    assert max(ent6.values()) <= PER_ENTRY
     ctx.close()
 
 
@@ -153,6 +160,34 @@ def test_graph_replays_match_eager():
         assert np.array_equal(a, b), k
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_graph_after_converged_physical_step_matches_eager(prec):
+    """After a physical step that stops on its tolerance (no tau update: the next step's
+    U/V read pn through the exact path), a graph runs that step outside the capture and
+    replays fused steps only; the state and the step counts match eager steps bit for bit."""
+    say("case: graph after a converged physical step, precision", prec)
+    prm, x, y, eps, lp, st, out, meta = load_golden("phys24")
+    outs = []
+    for mode in ("eager", "graph"):
+        ctx = L.Context(prm, x, y, eps, lp, precision=prec)
+        ctx.set_state(st)
+        for t in range(2):
+            ctx.physical_step(meta["eps_conv"], meta["itc_max"], t == 0)
+        nf0, ne0 = ctx.step_counts()
+        if mode == "eager":
+            ctx.pseudo_steps(25)
+        else:
+            ctx.pseudo_steps_graph(8, 3)       # 1 step outside the graph + 8 x 3 replayed
+        nf, ne = ctx.step_counts()
+        outs.append(({k: ctx.download(k) for k in FIELDS}, (nf - nf0, ne - ne0)))
+        ctx.close()
+    (fe, ce), (fg, cg) = outs
+    say("step counts eager", ce, "graph", cg)
+    assert ce == cg == (24, 1)
+    for k in FIELDS:
+        assert np.array_equal(fe[k], fg[k]), (k, max_rel(fg[k], fe[k]))
 
 
 @pytest.mark.parametrize("tiles", [(16, 8, 1, 1), (37, 11, 2, 4), (64, 64, 2, 1), (24, 5, 1, 4), (29, 8, 3, 1),

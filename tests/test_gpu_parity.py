@@ -68,6 +68,21 @@ def test_legacy_entry_points_match_reference():
         strip[2, 2:jm] = strip[im, 2:jm] = True
         assert np.all(g[~strip] == 0.0)
         assert max_rel(g[strip], out["ru"][strip]) <= TIGHT
+        # upwind_V_pair (upwind_V_pair.cu:103-130) with the Fortran strip placement
+        # (equations.f90:413-414, 496): rows j=2 and j=jmax over i=2..imax-1, columns i=2 and
+        # i=imax-1 over j=2..jmax; against the reference's RESV there and its upwind row
+        # operator (upwind_Vi) on every strip row
+        rv2 = DeviceArray(np.zeros_like(st["vm"]))
+        lib.upwind_V_pair(d["um"].ptr, d["vm"].ptr, d["p"].ptr, d["T"].ptr, rv2.ptr)
+        g = rv2.get()
+        strip = np.zeros_like(g, dtype=bool)
+        strip[2:im, 2] = strip[2:im, jm] = True
+        strip[2, 2:jm + 1] = strip[im - 1, 2:jm + 1] = True
+        assert np.all(g[~strip] == 0.0)
+        assert max_rel(g[strip], out["rv"][strip]) <= TIGHT
+        for j in (2, jm):
+            assert max_rel(g[2:im, j], out["rv_up"][2:im, j]) <= TIGHT, j
+        say("upwind_V_pair strips bit-exact", np.array_equal(g[strip], out["rv"][strip]))
         # boundary conditions
         u, v = DeviceArray(st["um"]), DeviceArray(st["vm"])
         lib.bcUV(u.ptr, v.ptr)
@@ -105,6 +120,15 @@ def test_legacy_entry_points_match_reference():
         da = DeviceArray(a)
         assert lib.max_reduce(da.ptr, 5, 10) == 3.0
         assert lib.max_reduce(da.ptr, 2, 5) == 3.0
+        # the maximum in the tail past n*m entries is not seen; one in the leading entry is
+        b = np.linspace(0.0, 1.0, 60)
+        b[45] = -7.5
+        db_ = DeviceArray(b)
+        assert lib.max_reduce(db_.ptr, 5, 9) == b[44]
+        assert lib.max_reduce(db_.ptr, 6, 8) == 7.5
+        b[0] = 9.0
+        db_ = DeviceArray(b)
+        assert lib.max_reduce(db_.ptr, 1, 1) == 9.0
     finally:
         lib.desalocar_globais()
 

@@ -153,3 +153,63 @@ def test_unet_training_fits_a_batch():
     p = net.predict(x)
     assert np.all(np.isfinite(p)) and np.mean((p - t) ** 2) <= 0.2 * losses[0]
     net.close()
+
+
+def test_adam_steps_match_oracle():
+    """Two training steps: the FP32 master weights after each equal the oracle's Adam
+    (float64, Kingma & Ba's algorithm with bias correction) applied to the gradient the GPU
+    computed at the same weights. Bar: the FP32 rounding of the weights and of the
+    constants 0.9f, 0.999f (1e-7 |w| + 1e-5 lr per entry)."""
+    say("case: Adam, two steps, 16x16 batch 2")
+    net = L.UNet(16, 16, 2, seed=21)
+    rng = np.random.default_rng(22)
+    x = rng.standard_normal((2, 16, 16, 5)).astype(np.float32)
+    t = rng.standard_normal((2, 16, 16, 4)).astype(np.float32)
+    lr = 1e-3
+    th = net.get_params().astype(np.float64)
+    m = np.zeros_like(th); v = np.zeros_like(th)
+    for step in (1, 2):
+        _, g = net.loss_grad(x, t)
+        _, g2 = net.loss_grad(x, t)
+        assert np.array_equal(g, g2)             # the gradient is deterministic
+        net.train_step(x, t, lr)
+        got = net.get_params().astype(np.float64)
+        exp, m, v = U.adam(th, g.astype(np.float64), m, v, step, lr)
+        err = np.abs(got - exp) / (1e-7 * np.abs(exp) + 1e-5 * lr)
+        moved = np.abs(got - th)
+        say("step", step, "worst error / bar", float(err.max()), "median move / lr", float(np.median(moved) / lr))
+        assert err.max() <= 1.0
+        th = got
+    net.close()
+
+
+def test_unet_full_size_loss_and_gradients_match_oracle():
+    """The warm-start network at its production size: 256 x 256 inputs (mask, porosity,
+    log Re, log Da, Ri), batch 1, loss and every parameter tensor's gradient against the
+    oracle with the same BF16 rounding points (bars as at 16 x 16)."""
+    say("case: U-Net gradients 256x256, batch 1")
+    H = W = 256
+    net = L.UNet(H, W, 1, seed=31)
+    th = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(32)
+    ii, jj = np.meshgrid(np.linspace(0, 1, H), np.linspace(0, 1, W), indexing="ij")
+    x = np.zeros((1, H, W, 5), np.float32)
+    blk = (ii > 0.3) & (ii < 0.62) & (jj > 0.25) & (jj < 0.7)
+    x[0, ..., 0] = blk
+    x[0, ..., 1] = np.where(blk, rng.uniform(0.3, 0.95, (H, W)), 1.0)
+    x[0, ..., 2] = np.log(120.0) / np.log(500.0)
+    x[0, ..., 3] = np.log10(3e-3) / 4 + 1
+    x[0, ..., 4] = 2.5 / 5
+    t = np.stack([np.sin(np.pi * jj) * (1 - 0.5 * blk), 0.1 * np.cos(np.pi * ii), 1 - ii, jj * blk], -1)
+    t = t[None].astype(np.float32)
+    loss, g = net.loss_grad(x, t)
+    loss_ref, g_ref, _ = U.forward_backward(th, x.astype(np.float64), t.astype(np.float64), bf=True)
+    say("loss", loss, loss_ref)
+    assert abs(loss - loss_ref) <= 1e-3 * abs(loss_ref)
+    lay, n = U.param_layout()
+    for q, (cin, cout, relu, wo, bo) in enumerate(lay):
+        ew = rel(g[wo:wo + cout * 9 * cin], g_ref[wo:wo + cout * 9 * cin])
+        eb = rel(g[bo:bo + cout], g_ref[bo:bo + cout])
+        say("conv", q, "dW", ew, "db", eb)
+        assert ew <= 3e-2 and eb <= 3e-2, q
+    net.close()

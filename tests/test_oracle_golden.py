@@ -252,6 +252,193 @@ def test_mu_poly_reduces_to_constant():
     assert np.allclose(o2.resu(um, vm, pp, T), o3.resu(um, vm, pp, T), rtol=1e-13, atol=1e-13)
 
 
+def _ops24_without(prm, *drop):
+    return {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax") + drop}
+
+
+@pytest.mark.parametrize("k,t", [(0, 2.0), (1, 2.0), (2, 3.0), (3, 2.0), (4, 2.0), (1, 0.5)])
+def test_mu_poly_each_power_scales_reynolds(k, t):
+    """mu(T) = T^k (one coefficient set) at a uniform temperature t is the constant-viscosity
+    operator at Re / t^k: a wrong Horner order, a dropped power or a swapped coefficient
+    gives another power of t and fails."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    base = _ops24_without(prm, "mu_coef", "mu_model")
+    lp0 = np.zeros_like(lp)                      # the Darcy drag also carries Re
+    um, vm, pp = (st[q] for q in ("um", "vm", "p"))
+    Tu = np.full_like(st["T"], t)
+    coef = [0.0] * 5
+    coef[k] = 1.0
+    om = O.Oracle(make_params(prm.imax, prm.jmax, mu_model=MU_POLY4, mu_coef=coef, **base), x, y, eps, lp0)
+    b2 = dict(base)
+    b2["re"] = base["re"] / t ** k
+    oc = O.Oracle(make_params(prm.imax, prm.jmax, **b2), x, y, eps, lp0)
+    for f in ("resu", "resv"):
+        a, b = getattr(om, f)(um, vm, pp, Tu), getattr(oc, f)(um, vm, pp, Tu)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.max(np.abs(b))), (f, k, t)
+        if k > 0 and t != 1.0:
+            # and it is not the unscaled operator
+            c = getattr(O.Oracle(make_params(prm.imax, prm.jmax, **base), x, y, eps, lp0), f)(um, vm, pp, Tu)
+            assert np.max(np.abs(a - c)) > 1e-6 * np.max(np.abs(c))
+
+
+def test_mu_poly_reads_the_temperature_of_its_own_cell():
+    """A temperature change in one cell (i0, j0) changes RU and RV at that cell only (the
+    viscous factor of the u face (i, j) and of the v face (i, j) is mu(T(i, j))); with the
+    buoyancy off nothing else reads T."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    base = _ops24_without(prm, "mu_coef", "mu_model", "invfr2")
+    o = O.Oracle(make_params(prm.imax, prm.jmax, mu_model=MU_POLY4, mu_coef=(0.6, 0.3, 0.2, 0.1, 0.05),
+                             invfr2=0.0, **base), x, y, eps, lp)
+    um, vm, pp, T = (st[q] for q in ("um", "vm", "p", "T"))
+    for (i0, j0) in ((7, 9), (12, 4), (20, 15)):
+        T2 = T.copy()
+        T2[i0, j0] *= 1.5
+        du = np.argwhere(o.resu(um, vm, pp, T2) != o.resu(um, vm, pp, T)).tolist()
+        dv = np.argwhere(o.resv(um, vm, pp, T2) != o.resv(um, vm, pp, T)).tolist()
+        assert du == [[i0, j0]], (i0, j0, du)
+        assert dv == [[i0, j0]], (i0, j0, dv)
+
+
+def test_mu_poly_reads_the_physical_level_temperature():
+    """In a dual-time pseudo-step mu(T) takes T of the physical level (the T that solve_V's
+    buoyancy reads, main.f90:124), not the T iterate: changing T_tau leaves the u, v and p
+    outputs unchanged, changing T does not."""
+    prm, x, y, eps, lp = small_case()
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "mu_model",
+                                                                        "invfr2", "b_art")}
+    o = O.Oracle(make_params(24, 20, mu_model=MU_POLY4, mu_coef=(0.6, 0.3, 0.2, 0.1, 0.05), invfr2=0.0,
+                             b_art=0.0, **d), x, y, eps, lp)
+    st = C.perturbed_state(24, 20, seed=31, amp=1e-2, p0=1.0, T0=1.0, C0=0.5, perturb_scalars=True)
+    outs = []
+    for which in (None, "T_tau", "T"):
+        st2 = {k: v.copy() for k, v in st.items()}
+        st2["T_tau"] = st["T"].copy()
+        if which:
+            st2[which] = st2[which] * 1.3
+        s, cs = o.make_state(st2)
+        o.pseudo_steps(s, cs, 1)
+        outs.append({k: s[k].copy() for k in ("um_tau", "vm_tau", "p")})
+    for k in ("um_tau", "vm_tau", "p"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    assert not np.array_equal(outs[0]["um_tau"], outs[2]["um_tau"])
+
+
+# ---- compat = CUDA: the c_cuda differences, pinned against the reference's own rows and
+# closed forms (the CUDA sources cannot be built here) ------------------------------------
+def test_cuda_compat_resv_rows_are_reference_rows():
+    """c_cuda's RESV (resv.cu QUICK interior i=3..imax-2, j=3..jmax-1, then the strips of
+    upwind_V_pair.cu:103-130: rows j=2 and j=jmax-1 over i=2..imax-1, columns i=2 and
+    imax-1 over j=2..jmax-1) assembled from the Fortran reference's own outputs: its QUICK
+    RESV on rows 3..jmax-2, its upwind row operator upwind_Vi (equations.f90:423-542) on
+    the strips, and nothing on row jmax."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    um, vm, p, T = (st[k] for k in ("um", "vm", "p", "T"))
+    oc = O.Oracle(make_params(prm.imax, prm.jmax, compat=1, **_ops24_without(prm, "mu_coef", "compat")),
+                  x, y, eps, lp)
+    rv = oc.resv(um, vm, p, T).reshape(prm.imax + 1, prm.jmax + 2)
+    quick = out["rv"].reshape(rv.shape)
+    up = out["rv_up"].reshape(rv.shape)
+    im, jm = prm.imax, prm.jmax
+    exp = np.zeros_like(rv)
+    exp[3:im - 1, 3:jm - 1] = quick[3:im - 1, 3:jm - 1]
+    exp[2:im, 2] = up[2:im, 2]
+    exp[2:im, jm - 1] = up[2:im, jm - 1]
+    exp[2, 2:jm] = up[2, 2:jm]
+    exp[im - 1, 2:jm] = up[im - 1, 2:jm]
+    assert np.array_equal(rv, exp)
+    # the Fortran operator differs on row jmax-1 (QUICK there) and row jmax (upwind strip)
+    assert not np.array_equal(rv[3:im - 1, jm - 1], quick[3:im - 1, jm - 1])
+    assert np.any(quick[2:im, jm] != 0) and np.all(rv[:, jm] == 0)
+
+
+def test_cuda_compat_solve_p_stage_two_skips_row_two():
+    """solve_P.cu:16-24 starts stage 2 at j=3, so row j=2 keeps the stage-1 value and the
+    update there is pn = p + (4/3) a with a = dtau beta RP, against pn = p + a on every
+    other row and in the Fortran stage order (equations.f90:607-665)."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    im, jm = prm.imax, prm.jmax
+    base = _ops24_without(prm, "mu_coef", "compat")
+    g = O.Oracle(prm, x, y, eps, lp).geometry()
+    um, vm, p = st["um"], st["vm"], st["p"]
+    U = um.reshape(im + 2, jm + 1); V = vm.reshape(im + 1, jm + 2)
+    a = np.zeros((im + 1, jm + 1))
+    for i in range(2, im):
+        for j in range(2, jm):
+            rp = -((U[i + 1, j] * g["aue"][j] - U[i, j] * g["aue"][j])
+                   + (V[i, j + 1] * g["avn"][i] - V[i, j] * g["avn"][i]))
+            a[i, j] = prm.dtau * prm.beta * rp
+    P = p.reshape(im + 1, jm + 1)
+    for compat in (0, 1):
+        o = O.Oracle(make_params(im, jm, compat=compat, **base), x, y, eps, lp)
+        pn = np.zeros_like(p)
+        r = o.solve_p(p.copy(), um, vm, pn)
+        pn = pn.reshape(P.shape)
+        c2 = 4.0 / 3.0 if compat else 1.0
+        exp = P + a
+        exp[2:im, 2] = P[2:im, 2] + c2 * a[2:im, 2]
+        scale = np.max(np.abs(P)) + np.max(np.abs(a))
+        assert np.max(np.abs(pn[2:im, 2:jm] - exp[2:im, 2:jm])) <= 1e-14 * scale, compat
+        if compat == 0:
+            assert abs(r - np.max(np.abs(a[2:im, 2:jm]))) <= 1e-15 * np.max(np.abs(a))
+
+
+def _res_region_case(kind, cell, compat):
+    """One solve with dt = 0, where every RK stage's residual is exactly (phi - phi_tau) dtau
+    (the operator is multiplied by dt), and phi differs from phi_tau by 1 in one cell."""
+    prm, x, y, eps, lp = small_case()
+    im, jm = prm.imax, prm.jmax
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "compat", "dt")}
+    o = O.Oracle(make_params(im, jm, compat=compat, dt=0.0, **d), x, y, eps, lp)
+    st = random_state(im, jm, seed=41)
+    um_tau, vm_tau = st["um"].copy(), st["vm"].copy()
+    if kind == "u":
+        um = um_tau.copy(); um[cell] += 1.0
+        return o.solve_u(um, um_tau, vm_tau, np.zeros_like(um), st["p"], st["T"]), prm
+    vm = vm_tau.copy(); vm[cell] += 1.0
+    return o.solve_v(vm, um_tau, vm_tau, np.zeros_like(vm), st["p"], st["T"]), prm
+
+
+@pytest.mark.parametrize("kind,n_i,n_j,ld,cells", [
+    # c_cuda/solve_U.cu:58 max_reduce(dev_res_u, imax-1, jmax-2), u layout i*(jmax+1)+j
+    ("u", lambda im: im - 1, lambda jm: jm - 2, lambda jm: jm + 1, [(3, 5), (19, 8), (20, 12), (23, 10)]),
+    # c_cuda/solve_V.cu:52 max_reduce(dev_res_v, imax-1, jmax), v layout i*(jmax+2)+j
+    ("v", lambda im: im - 1, lambda jm: jm, lambda jm: jm + 2, [(2, 3), (20, 19), (21, 3), (22, 10), (23, 4)]),
+])
+def test_cuda_compat_residual_scans_the_first_linear_entries(kind, n_i, n_j, ld, cells):
+    """c_cuda reduces max|res| over the first n_i*n_j linear entries of the residual array
+    (max_reduce.cu:14-41), not over the update region: a residual in a cell past that count
+    is not seen (compat = CUDA), while the Fortran region sees every updated cell."""
+    prm, _, _, _, _ = small_case()
+    im, jm = prm.imax, prm.jmax
+    n = n_i(im) * n_j(jm)
+    for cell in cells:
+        rf, _ = _res_region_case(kind, cell, 0)
+        rc, _ = _res_region_case(kind, cell, 1)
+        seen = cell[0] * ld(jm) + cell[1] < n
+        assert rf == pytest.approx(prm.dtau, rel=1e-15), cell
+        assert rc == (pytest.approx(prm.dtau, rel=1e-15) if seen else 0.0), (cell, rc, n)
+
+
+def test_cuda_compat_solve_p_residual_scans_the_first_linear_entries():
+    """solve_P.cu:50 max_reduce(dev_res_p, imax, jmax): a divergence only in row i=imax-1
+    (linear entries from (imax-1)(jmax+1) on, past imax*jmax) is invisible to c_cuda."""
+    prm, x, y, eps, lp = small_case()
+    im, jm = prm.imax, prm.jmax
+    assert (im - 1) * (jm + 1) + 2 >= im * jm
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "compat")}
+    for i0, seen in ((im - 1, False), (3, True)):
+        um = C.zeros("u", im, jm)
+        vm = C.zeros("v", im, jm)
+        um[i0 + 1, 7] = 1.0           # RP(i0, 7) = -aue(7), RP(i0 + 1, 7) = +aue(7) if in range
+        r = {}
+        for compat in (0, 1):
+            o = O.Oracle(make_params(im, jm, compat=compat, **d), x, y, eps, lp)
+            r[compat] = o.solve_p(C.zeros("p", im, jm), um, vm, C.zeros("p", im, jm))
+        a = prm.dtau * prm.beta * O.Oracle(prm, x, y, eps, lp).geometry()["aue"][7]
+        assert r[0] == pytest.approx(a, rel=1e-14)
+        assert r[1] == (pytest.approx(a, rel=1e-14) if seen else 0.0), (i0, r)
+
+
 def test_steady_mode_is_dual_time_with_tracking_level():
     """steady=1 equals the dual-time step with the physical level set to the iterate."""
     prm, x, y, eps, lp = small_case()

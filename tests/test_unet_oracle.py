@@ -70,3 +70,27 @@ def test_library_layer_table_matches_oracle():
     """The package's layer table (used for FLOP counts) is the oracle's network."""
     from cfd_amd import lib as L
     assert [tuple(s) for s in L.UNET_LAYERS] == [tuple(s) for s in U.SPEC]
+
+
+def test_adam_closed_forms():
+    """Adam's bias correction makes the first step lr * g / (|g| + eps), i.e. lr in the
+    direction of -g whatever |g| is; under a constant gradient every later step is the same
+    size; after g then -g the moments are m = -0.01 g, v = 0.001999 g^2, so the second step
+    is lr * (0.01 / 0.19) against the first direction. A missing or off-by-one bias correction fails these."""
+    rng = np.random.default_rng(3)
+    th = rng.standard_normal(257)
+    g = rng.standard_normal(257) * np.logspace(-6, 3, 257)
+    lr = 1e-3
+    z = np.zeros_like(th)
+    th1, m1, v1 = U.adam(th, g, z, z, 1, lr)
+    assert np.allclose(th1, th - lr * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-15)
+    # constant gradient: each step moves lr * g / (|g| + eps), steps 2..5
+    t, m, v = th1, m1, v1
+    for k in range(2, 6):
+        t2, m, v = U.adam(t, g, m, v, k, lr)
+        assert np.allclose(t2 - t, -lr * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-17), k
+        t = t2
+    # reversal: m2 = -0.01 g, v2 = 0.001999 g^2; corrected by 0.19 and 0.001999
+    th2, _, _ = U.adam(th1, -g, m1, v1, 2, lr)
+    exp = lr * (0.01 / 0.19) * g / (np.abs(g) + 1e-8)
+    assert np.allclose(th2 - th1, exp, rtol=1e-9, atol=1e-18)
