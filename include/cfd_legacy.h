@@ -73,6 +73,16 @@ void bcC(double *dev_cn);
  * a NaN entry makes the result NaN (the reference's fmax would drop it). */
 double max_reduce(double *dev_matriz_linearizada, int imax, int jmax);
 
+/* No counterpart in the reference: max_reduce for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). The same
+ * max |a| over the first n_i*n_j linear entries (NaN wins, 0 for no entries), written to the
+ * caller's device double dev_out on `stream` (a hipStream_t; NULL: the default stream),
+ * without synchronising and without any allocation: the partial maxima live in a static
+ * 8 KiB device array of the library (one per GPU, kept until the process ends). Needs no
+ * default context. n_i*n_j at most 2^36 entries. Calls that overlap on one GPU share that
+ * array and race. Returns CFD_OK or an error code (cfd_last_error()). */
+int max_reduce_noalloc(const double *dev_a, int n_i, int n_j, double *dev_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,7 @@ HIP_SOURCES = [
     ("cfd_ops.hip", ["-ffp-contract=off"]),
     ("cfd_fused.hip", ["-ffp-contract=fast"]),
     ("cfd_unet.hip", ["-ffp-contract=fast"]),
+    ("cfd_noalloc.hip", ["-ffp-contract=off"]),
 ]
 C_SOURCES = ["cfd_host.c", "cfd_batch.c", "cfd_mesh_io.c"]
 

@@ -39,6 +39,7 @@ EXPORTS_LEGACY = [
     "RESU", "RESV", "RESZ", "RESC", "upwind_U_pair", "upwind_V_pair", "solve_U", "solve_V", "solve_P",
     "solve_Z", "solve_C", "bcUV", "bcP", "bcZ", "bcC", "max_reduce", "alocar_globais", "desalocar_globais",
     "cfd_legacy_init", "cfd_legacy_set_params", "cfd_legacy_set_mesh", "cfd_legacy_context",
+    "max_reduce_noalloc",
 ]
 
 
@@ -156,6 +157,7 @@ def load():
         "solve_Z": (None, [_vp] * 6), "solve_C": (None, [_vp] * 6),
         "bcUV": (None, [_vp] * 2), "bcP": (None, [_vp]), "bcZ": (None, [_vp]), "bcC": (None, [_vp]),
         "max_reduce": (dbl, [_vp, ctypes.c_int, ctypes.c_int]),
+        "max_reduce_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
         "alocar_globais": (None, []), "desalocar_globais": (None, []),
     }
     for name, (res, args) in sig.items():
