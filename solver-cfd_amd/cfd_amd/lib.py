@@ -12,7 +12,8 @@ import numpy as np
 from .params import CfdParams, LAYOUT_C, PREC_FP64, FIELD_NAMES
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcfd_amd.so")
+# CFD_AMD_LIB names another build of the library (default: the in-tree one)
+LIB_PATH = os.environ.get("CFD_AMD_LIB") or os.path.join(HERE, "libcfd_amd.so")
 
 CFD_OK = 0
 ERRORS = {-1: "CFD_ERR_ARG", -2: "CFD_ERR_HIP", -3: "CFD_ERR_NONFINITE", -4: "CFD_ERR_NOCONV",

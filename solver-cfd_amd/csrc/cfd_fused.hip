@@ -128,9 +128,13 @@ struct K1Args {
 template <typename R, int NT>
 struct Smem1 {
     R sUT[4][lrow(NT)], sVT[4][lrow(NT)], sT[2][lrow(NT)];
-    R sU1[3][lrow(NT)], sU2[3][lrow(NT)], sV1[3][lrow(NT)], sV2[3][lrow(NT)], sV3[2][lrow(NT)];
+    R sU1[4][lrow(NT)], sU2[4][lrow(NT)], sV1[4][lrow(NT)], sV2[4][lrow(NT)], sV3[2][lrow(NT)];
     double sP[2][lrow(NT)];
+    // per-row constants, read from here once per row rather than held in scalar registers
+    // for the whole march
+    double cst[16];
 };
+enum { CST_INV_REDA, CST_CFSD, CST_MU0, CST_RI = CST_MU0 + 5, CST_T_REF, CST_INVFR2, CST_U_IN, CST_BETA, CST_N };
 
 // v of the centre row at column j+dj with the BC formulas (boundary.f90:9-12,19-22 /
 // channel walls): used where the ring's boundary columns are not yet fixed up.
@@ -182,9 +186,15 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     auto sUT = sm.sUT; auto sVT = sm.sVT; auto sT = sm.sT; auto sP = sm.sP;
     auto sU1 = sm.sU1; auto sU2 = sm.sU2; auto sV1 = sm.sV1; auto sV2 = sm.sV2; auto sV3 = sm.sV3;
 
+    if (x < CST_N) {
+        const double cv[CST_N] = {a.inv_reda, a.cfsd, a.mu[0], a.mu[1], a.mu[2], a.mu[3], a.mu[4], a.ri, a.t_ref,
+                                  a.invfr2, a.u_in, a.beta};
+        sm.cst[x] = cv[x];
+    }
+    const double *cst = sm.cst;
     for (int k = x; k < lrow(NT); k += NT) {
         for (int q = 0; q < 4; ++q) { sUT[q][k] = R(0); sVT[q][k] = R(0); }
-        for (int q = 0; q < 3; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
+        for (int q = 0; q < 4; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
         for (int q = 0; q < 2; ++q) { sP[q][k] = 0.0; sT[q][k] = R(1); sV3[q][k] = R(0); }
     }
 
@@ -220,21 +230,22 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     // next-row registers: the loads for row t+1 are issued before row t is computed
     R q_ut, q_vt, q_T, q_e, q_L, q_um, q_vm;
     double q_p;
+    // Loads at clamped (always valid) indices, one site per field, defaults selected for
+    // rows and columns outside the arrays: no divergent branches around the loads.
+    const int ju = min(max(j, 1), jm), jv = min(max(j, 1), jm + 1);
+    const bool lu = jvalid && j <= jm;
     auto fetch = [&](int t) {
-        q_ut = R(0); q_vt = R(0); q_p = 0.0; q_T = R(1); q_e = R(1); q_L = R(0); q_um = R(0); q_vm = R(0);
-        if (jvalid && t >= 1) {
-            if (t <= im + 1 && j <= jm) q_ut = a.ut[(size_t)t * su + j];
-            if (t <= im) q_vt = a.vt[(size_t)t * sv + j];
-        }
         const int rp = t - 2;
-        if (jvalid && rp >= 1 && rp <= im) {
-            if (j <= jm) {
-                const size_t k = (size_t)rp * su + j;
-                q_p = a.p[k]; q_T = a.T[k]; q_e = a.eps[k]; q_L = a.lp[k];
-                if (!STEADY) q_um = a.um[k];
-            }
-            if (!STEADY) q_vm = a.vm[(size_t)rp * sv + j];
-        }
+        const int tu = min(max(t, 1), im + 1), tv = min(max(t, 1), im), rc = min(max(rp, 1), im);
+        const bool in_u = t >= 1 && t <= im + 1, in_v = t >= 1 && t <= im, in_p = rp >= 1 && rp <= im;
+        const size_t k = (size_t)rc * su + ju;
+        const R ut_ = a.ut[(size_t)tu * su + ju], vt_ = a.vt[(size_t)tv * sv + jv];
+        const double p_ = a.p[k];
+        const R T_ = a.T[k], e_ = a.eps[k], L_ = a.lp[k];
+        q_ut = ut_; q_vt = vt_; q_p = p_; q_T = T_; q_e = e_; q_L = L_;
+        if (!STEADY) { q_um = a.um[k]; q_vm = a.vm[(size_t)rc * sv + jv]; }
+        else { q_um = R(0); q_vm = R(0); }
+        (void)in_u; (void)in_v; (void)in_p;
     };
     // Rows run two at a time, so that the window shifts of the first row of a pair are
     // register renames rather than moves. Pairs start at even rows: a row is always computed
@@ -249,9 +260,14 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
 
     auto row = [&](const int t) __attribute__((always_inline)) {
         // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
-        const R ut_new = q_ut, vt_new = q_vt, T_rp = q_T, e = q_e, Lp = q_L;
-        const double p_rp = q_p;
-        const R um_rp = q_um, vm_rp = q_vm;
+        // the defaults of rows and columns outside the arrays, applied where the values are
+        // consumed (a select at the load would wait for it)
+        const bool in_u = t >= 1 && t <= im + 1, in_v = t >= 1 && t <= im, in_p = t - 2 >= 1 && t - 2 <= im;
+        const bool pu = in_p && lu, pv = in_p && jvalid;
+        const R ut_new = (in_u && lu) ? q_ut : R(0), vt_new = (in_v && jvalid) ? q_vt : R(0);
+        const R T_rp = pu ? q_T : R(1), e = pu ? q_e : R(1), Lp = pu ? q_L : R(0);
+        const double p_rp = pu ? q_p : 0.0;
+        const R um_rp = pu ? q_um : R(0), vm_rp = pv ? q_vm : R(0);
         fetch(t + 1);
         const int rp = t - 2;
         ut.push(ut_new);
@@ -264,10 +280,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
 
         // ---------------- phase B
         if (!chan) {   // boundary columns of the ring rows produced in the previous iteration
-            fix_u(sU1[s3(t - 3)], x, j, jm);
-            fix_u(sU2[s3(t - 5)], x, j, jm);
-            fix_v(sV1[s3(t - 3)], x, j, jm);
-            fix_v(sV2[s3(t - 5)], x, j, jm);
+            fix_u(sU1[s4(t - 3)], x, j, jm);
+            fix_u(sU2[s4(t - 5)], x, j, jm);
+            fix_v(sV1[s4(t - 3)], x, j, jm);
+            fix_v(sV2[s4(t - 5)], x, j, jm);
         }
         Su.push(ut_new + sUT[s4(t)][PAD + x - 1]);
         Svn.push(sVT[s4(t - 1)][PAD + x + 1] + sVT[s4(t - 2)][PAD + x + 1]);
@@ -277,18 +293,20 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             if (rp >= ilo - 4 && rp <= ihi + 4) {
                 iev = e == R(1) ? R(1) : R(1) / e;
                 if (Lp != R(0)) {
-                    k1 = e * Lp * R(a.inv_reda);
-                    k2 = Lp * R(a.cfsd) * sqrt(e);
+                    k1 = e * Lp * R(cst[CST_INV_REDA]);
+                    k2 = Lp * R(cst[CST_CFSD]) * sqrt(e);
                 }
             }
             ie.push(iev); ew.push(e); k1w.push(k1); k2w.push(k2);
             if (MUPOLY) {
                 const R tt = T_rp;
-                nuw.push(e * inv_re * ((((R(a.mu[4]) * tt + R(a.mu[3])) * tt + R(a.mu[2])) * tt + R(a.mu[1])) * tt + R(a.mu[0])));
+                nuw.push(e * inv_re * ((((R(cst[CST_MU0 + 4]) * tt + R(cst[CST_MU0 + 3])) * tt + R(cst[CST_MU0 + 2])) * tt
+                                         + R(cst[CST_MU0 + 1])) * tt + R(cst[CST_MU0])));
             }
             dpj.push(R(p_rp - sP[s2(rp)][PAD + x - 1]));
             const R Tm = sT[s2(rp)][PAD + x - 1];
-            bw.push(bouss ? R(a.ri) * ((T_rp + Tm) * R(0.5) - R(a.t_ref)) : R(a.invfr2) * (R(1) - R(2) / (T_rp + Tm)));
+            bw.push(bouss ? R(cst[CST_RI]) * ((T_rp + Tm) * R(0.5) - R(cst[CST_T_REF]))
+                          : R(cst[CST_INVFR2]) * (R(1) - R(2) / (T_rp + Tm)));
             if (!STEADY) { du.push(um_rp - ut[2]); dv.push(vm_rp - vt[2]); }
         }
 
@@ -301,8 +319,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             bool interior = false;                                                                \
             if (r >= ilo - 2 * (3 - K) && r <= ihi + 2 * (3 - K)) {                               \
                 if (r >= 3 && r <= im - 1) {                                                      \
-                    if (ucol) {                                                                   \
-                        interior = true;                                                          \
+                    /* every lane computes (inputs outside the arrays hold finite defaults); */   \
+                    /* columns outside the u update keep 0 */                                     \
+                    {                                                                             \
+                        interior = ucol;                                                          \
                         const R uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0];     \
                         const R *c = &RING_C[PAD + x];                                            \
                         const R uS = c[-1], uN = c[1];                                            \
@@ -336,12 +356,14 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         }                                                                         \
                         rs = ((STEADY ? R(0) : du[bk]) + Rr * dt) * dtau;                         \
                         const R tau = ut[2 * K];                                                  \
-                        if (K == 1) val = tau + rs;                                               \
-                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (uP + rs);               \
-                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (uP + rs);                 \
+                        R cv;                                                                     \
+                        if (K == 1) cv = tau + rs;                                                \
+                        else if (K == 2) cv = R(0.75) * tau + R(0.25) * (uP + rs);                \
+                        else cv = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (uP + rs);                  \
+                        val = ucol ? cv : R(0);                                                   \
                     }                                                                             \
                 } else if (r <= 2) {                                                              \
-                    val = chan ? R(a.u_in) : R(0);                                                \
+                    val = chan ? R(cst[CST_U_IN]) : R(0);                                                \
                 } else {                                                                          \
                     val = (chan && ucol) ? WOUT[0] : R(0);                                        \
                 }                                                                                 \
@@ -362,11 +384,11 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             R val = R(0), rs = R(0);                                                              \
             bool interior = false;                                                                \
             if (r >= ilo - 2 * (3 - K) && r <= ihi - 1 + 2 * (3 - K)) {                           \
-                if (j <= 2) {                                                                     \
-                    val = chan ? R(0) : vi;                                                       \
-                } else if (r >= 2 && r <= im - 1) {                                               \
-                    if (vcol) {                                                                   \
-                        interior = true;                                                          \
+                if (r >= 2 && r <= im - 1) {                                                      \
+                    /* every lane computes; columns outside the v update keep 0, the inflow */    \
+                    /* columns j <= 2 take their boundary value below */                         \
+                    {                                                                             \
+                        interior = vcol;                                                          \
                         const R vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0];     \
                         const R *c = &RING_C[PAD + x];                                            \
                         const R vS = c[-1], vN = c[1];                                            \
@@ -399,14 +421,18 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         }                                                                         \
                         rs = ((STEADY ? R(0) : dv[bk]) + Rr * dt) * dtau;                         \
                         const R tau = vt[2 * K];                                                  \
-                        if (K == 1) val = tau + rs;                                               \
-                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (vP + rs);               \
-                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (vP + rs);                 \
-                        if (r == 2 && !chan) WOUT.set(0, val);   /* row 1 = row 2 (boundary.f90:34-36) */ \
+                        R cv;                                                                     \
+                        if (K == 1) cv = tau + rs;                                                \
+                        else if (K == 2) cv = R(0.75) * tau + R(0.25) * (vP + rs);                \
+                        else cv = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (vP + rs);                  \
+                        val = vcol ? cv : R(0);                                                   \
+                        /* row 1 = row 2 (boundary.f90:34-36), in the v update's columns */       \
+                        if (r == 2 && !chan) WOUT.set(0, vcol ? cv : WOUT[0]);                    \
                     }                                                                             \
                 } else if (r == im) {                                                             \
                     val = vcol ? WOUT[0] : R(0);                                                  \
                 }                                                                                 \
+                if (j <= 2) val = chan ? R(0) : vi;                                               \
                 RING_OUT                                                                          \
             }                                                                                     \
             WOUT.push(val);                                                                       \
@@ -416,12 +442,12 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             }                                                                                     \
         }
 
-        U_STAGE(1, ut, sUT[s4(t - 2)], w1, { sU1[s3(t - 2)][PAD + x] = val; })
-        V_STAGE(1, vt, sVT[s4(t - 2)], x1, { sV1[s3(t - 2)][PAD + x] = val; })
-        U_STAGE(2, w1, sU1[s3(t - 4)], w2, { sU2[s3(t - 4)][PAD + x] = val; })
-        V_STAGE(2, x1, sV1[s3(t - 4)], x2, { sV2[s3(t - 4)][PAD + x] = val; })
-        U_STAGE(3, w2, sU2[s3(t - 6)], w3, {})
-        V_STAGE(3, x2, sV2[s3(t - 6)], x3, { sV3[s2(t - 6)][PAD + x] = val; })
+        U_STAGE(1, ut, sUT[s4(t - 2)], w1, { sU1[s4(t - 2)][PAD + x] = val; })
+        V_STAGE(1, vt, sVT[s4(t - 2)], x1, { sV1[s4(t - 2)][PAD + x] = val; })
+        U_STAGE(2, w1, sU1[s4(t - 4)], w2, { sU2[s4(t - 4)][PAD + x] = val; })
+        V_STAGE(2, x1, sV1[s4(t - 4)], x2, { sV2[s4(t - 4)][PAD + x] = val; })
+        U_STAGE(3, w2, sU2[s4(t - 6)], w3, {})
+        V_STAGE(3, x2, sV2[s4(t - 6)], x3, { sV3[s2(t - 6)][PAD + x] = val; })
 #undef U_STAGE
 #undef V_STAGE
 
@@ -434,7 +460,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 const double v3n = (double)lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
                 const double avn_r = pr[RR + r];
                 const double rpv = -((u3e * aue_jd - u3p * aue_jd) + (v3n * avn_r - v3p * avn_r));
-                const double dd = a.dtau * rpv * a.beta;
+                const double dd = a.dtau * rpv * cst[CST_BETA];
                 const double pp = pw[5];
                 double pi = pp + dd;
                 pi = 0.75 * pp + 0.25 * (pi + dd);
@@ -821,7 +847,7 @@ struct K2Args {
 
 template <typename R, int NT>
 struct Smem2 {
-    R sTT[3][lrow(NT)], sCT[3][lrow(NT)], sV[2][lrow(NT)];
+    R sTT[4][lrow(NT)], sCT[4][lrow(NT)], sV[2][lrow(NT)];
     R sZ1[2][lrow(NT)], sZ2[2][lrow(NT)], sC1[2][lrow(NT)], sC2[2][lrow(NT)];
 };
 
@@ -853,6 +879,26 @@ __device__ __forceinline__ R lds_c(const R *row, int x, int j, int dj, int jm, b
     return row[PAD + x + dj];
 }
 
+// Neighbour values of the centre row with the boundary formulas as selects: zs/zn the ring
+// values at j-1/j+1, zp the lane's own value (the extrapolation a1 + (a1 - a2) at jj = jm
+// uses a1 = zp and a2 = the raw j-1 value).
+template <typename R>
+__device__ __forceinline__ void nb_t(R &zs, R &zn, R zp, int j, int jm, bool chan, R tinf, R tb, R ttop)
+{
+    const R ext = chan ? ttop : zp + R(1) * (zp - zs);
+    const R s1 = chan ? tb : tinf;
+    zn = (j + 1 == jm) ? ext : zn;
+    zs = (j - 1 == 1) ? s1 : zs;
+}
+template <typename R>
+__device__ __forceinline__ void nb_c(R &zs, R &zn, R zp, int j, int jm, bool chan)
+{
+    const R ext = chan ? zp : zp + R(1) * (zp - zs);
+    const R s1 = chan ? zp : R(0);
+    zn = (j + 1 == jm) ? ext : zn;
+    zs = (j - 1 == 1) ? s1 : zs;
+}
+
 template <typename R, bool STEADY, int NT>
 __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Smem2<R, NT> &sm)
 {
@@ -870,7 +916,7 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
     auto sZ1 = sm.sZ1; auto sZ2 = sm.sZ2; auto sC1 = sm.sC1; auto sC2 = sm.sC2;
 
     for (int k = x; k < lrow(NT); k += NT) {
-        for (int q = 0; q < 3; ++q) { sTT[q][k] = R(0); sCT[q][k] = R(0); }
+        for (int q = 0; q < 4; ++q) { sTT[q][k] = R(0); sCT[q][k] = R(0); }
         for (int q = 0; q < 2; ++q) { sV[q][k] = R(0); sZ1[q][k] = R(0); sZ2[q][k] = R(0); sC1[q][k] = R(0); sC2[q][k] = R(0); }
     }
     const bool jvalid = j >= 1 && j <= jm;
@@ -890,37 +936,56 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
     z1.fill(R(0)); z2.fill(R(0)); c1.fill(R(0)); c2.fill(R(0)); z3.fill(R(0)); c3.fill(R(0));
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
-    R q_t, q_c, q_u, q_v, q_T, q_C, q_e, q_L;
+    // two prefetch sets: the loads of row t go out two rows ahead (set t & 1)
+    R qa_t, qa_c, qa_u, qa_v, qa_T, qa_C, qa_e, qa_L;
+    R qb_t, qb_c, qb_u, qb_v, qb_T, qb_C, qb_e, qb_L;
     // next row: u3/v3 from K1's interior cells, boundary rows/columns from the BC formulas
-    auto fetch = [&](int t) {
-        q_t = R(0); q_c = R(0); q_u = R(0); q_v = R(0); q_T = R(0); q_C = R(0); q_e = R(1); q_L = R(0);
-        if (jvalid && t >= 1 && t <= im + 1) {
-            if (t <= im) {
-                const size_t k = (size_t)t * su + j;
-                q_t = a.Tt[k]; q_c = a.Ct[k]; q_e = a.eps[k]; q_L = a.lp[k];
-                if (!STEADY) { q_T = a.T[k]; q_C = a.C[k]; }
-                if (j == 2) q_v = chan ? R(0) : R(a.v_i);
-                else if (t >= 2 && t <= im - 1 && j >= 3 && j <= jm - 1) q_v = a.vn[(size_t)t * sv + j];
-            }
-            // interior cells only: the boundary cells of u and v may still be being written
-            // (the two-stream step runs K2 beside the u/v/p boundary pass) and no stage here
-            // reads them
-            if (t > 2 && t < im && j >= 2 && j <= jm - 1) q_u = a.un[(size_t)t * su + j];
-        }
-    };
+    // Loads at clamped (always valid) indices, one site per field; the defaults of rows and
+    // columns outside the arrays are applied where the values are consumed (a select at the
+    // load would wait for it). u and v come from interior cells only: their boundary cells may
+    // still be being written (the two-stream step runs K2 beside the u/v/p boundary pass).
+    const int jc1 = min(max(j, 1), jm), jvi = min(max(j, 3), jm - 1), jui = min(max(j, 2), jm - 1);
+#define K2_FETCH(Q, t)                                                                            \
+    {                                                                                             \
+        const size_t k_ = (size_t)min(max((t), 1), im) * su + jc1;                                \
+        Q##_t = a.Tt[k_]; Q##_c = a.Ct[k_]; Q##_e = a.eps[k_]; Q##_L = a.lp[k_];                  \
+        if (!STEADY) { Q##_T = a.T[k_]; Q##_C = a.C[k_]; } else { Q##_T = R(0); Q##_C = R(0); }  \
+        Q##_v = a.vn[(size_t)min(max((t), 2), im - 1) * sv + jvi];                               \
+        Q##_u = a.un[(size_t)min(max((t), 3), im - 1) * su + jui];                               \
+    }
+    // Rows run two at a time from an even row (as in K1: a row is computed by the same half of
+    // the pair in every block); the extra row at either end has no stage in range.
+    const int t0 = (ilo - 3) - ((ilo - 3) & 1);
     __syncthreads();
-    fetch(ilo - 3);
+    K2_FETCH(qa, t0)
+    K2_FETCH(qb, t0 + 1)
 
-    for (int t = ilo - 3; t <= ihi + 2; ++t) {
+    // argument fields used in the row body, as locals (the row closure then holds no
+    // reference into the argument block)
+    const R v_i = R(a.v_i), u_in = R(a.u_in);
+    R *const Tn_out = a.Tn, *const Cn_out = a.Cn;
+    auto row = [&](const int t, R &q_t, R &q_c, R &q_u, R &q_v, R &q_T, R &q_C, R &q_e, R &q_L)
+                   __attribute__((always_inline)) {
         // ---------------- phase A: row t
-        const R tnew = q_t, cnew = q_c, vnew = q_v, Tp = q_T, Cp = q_C, e = q_e, Lp = q_L;
-        R unew = q_u;
-        fetch(t + 1);
-        if (t <= 2) unew = chan ? R(a.u_in) : R(0);         // u3 boundary rows (boundary.f90:29-32,39-42)
+        const bool inr = jvalid && t >= 1 && t <= im;
+        const R tnew = inr ? q_t : R(0), cnew = inr ? q_c : R(0), e = inr ? q_e : R(1), Lp = inr ? q_L : R(0);
+        const R Tp = (!STEADY && inr) ? q_T : R(0), Cp = (!STEADY && inr) ? q_C : R(0);
+        const R vnew = !inr ? R(0) : (j == 2 ? (chan ? R(0) : v_i)
+                                             : ((t >= 2 && t <= im - 1 && j >= 3 && j <= jm - 1) ? q_v : R(0)));
+        R unew = (jvalid && t > 2 && t < im && j >= 2 && j <= jm - 1) ? q_u : R(0);
+        {   // row t + 2 into the set just consumed
+            const int tn = t + 2;
+            const size_t k_ = (size_t)min(max(tn, 1), im) * su + jc1;
+            q_t = a.Tt[k_]; q_c = a.Ct[k_]; q_e = a.eps[k_]; q_L = a.lp[k_];
+            if (!STEADY) { q_T = a.T[k_]; q_C = a.C[k_]; }
+            q_v = a.vn[(size_t)min(max(tn, 2), im - 1) * sv + jvi];
+            q_u = a.un[(size_t)min(max(tn, 3), im - 1) * su + jui];
+        }
+        if (t <= 2) unew = chan ? u_in : R(0);         // u3 boundary rows (boundary.f90:29-32,39-42)
         else if (t >= im) unew = chan ? uw[0] : R(0);
         tt.push(tnew); ct.push(cnew); uw.push(unew); vw.push(vnew);
-        sTT[s3(t)][PAD + x] = tnew;
-        sCT[s3(t)][PAD + x] = cnew;
+        sTT[s4(t)][PAD + x] = tnew;
+        sCT[s4(t)][PAD + x] = cnew;
         sV[s2(t)][PAD + x] = vnew;
         __syncthreads();
 
@@ -943,12 +1008,14 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
             bool interior = false;                                                                \
             if (r >= ilo - (3 - K) && r <= ihi - 1 + (3 - K)) {                                   \
                 if (r >= 2 && r <= im - 1) {                                                      \
-                    if (scol) {                                                                   \
-                        interior = true;                                                          \
+                    /* every lane computes; columns outside the update keep 0 */                 \
+                    {                                                                             \
+                        interior = scol;                                                          \
                         const R zE = WP[0], zP = WP[1], zW = WP[2];                               \
                         const R *c = CROW;                                                        \
-                        const R zS = ISC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, tinf, tb, ttop); \
-                        const R zN = ISC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, tinf, tb, ttop);   \
+                        R zS = c[PAD + x - 1], zN = c[PAD + x + 1];                               \
+                        if (ISC) nb_c(zS, zN, zP, j, jm, chan);                                   \
+                        else nb_t(zS, zN, zP, j, jm, chan, tinf, tb, ttop);                       \
                         const R avn_r = R(pr[RR + r]);                                            \
                         const R diff = (KAPPA * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
                                      + (KAPPA * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP)); \
@@ -960,10 +1027,12 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
                                          : vols * (diff - oml[bq] * conv) * id[bq];               \
                         const R rs = ((STEADY ? R(0) : DW[bq]) + Rr * dt) * dtau;                 \
                         const R tau = TAUW[bq];                                                   \
-                        if (K == 1) val = tau + rs;                                               \
-                        else if (K == 2) val = R(0.75) * tau + R(0.25) * (zP + rs);               \
-                        else val = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (zP + rs);                 \
-                        if (r == 2) WOUT.set(0, (ISC && chan) ? cslot : val);  /* row 1 */        \
+                        R cv;                                                                     \
+                        if (K == 1) cv = tau + rs;                                                \
+                        else if (K == 2) cv = R(0.75) * tau + R(0.25) * (zP + rs);                \
+                        else cv = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (zP + rs);                  \
+                        val = scol ? cv : R(0);                                                   \
+                        if (r == 2) WOUT.set(0, scol ? ((ISC && chan) ? cslot : cv) : WOUT[0]);  /* row 1 */ \
                     }                                                                             \
                 } else if (r == im) {                                                             \
                     val = scol ? WOUT[0] : R(0);                                                  \
@@ -978,14 +1047,19 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
         }
 
         R *const noout = nullptr;
-        SCAL_STAGE(1, tt, sTT[s3(t - 1)], false, z1, { sZ1[s2(t - 1)][PAD + x] = val; }, noout, kt, dT, tt)
-        SCAL_STAGE(1, ct, sCT[s3(t - 1)], true, c1, { sC1[s2(t - 1)][PAD + x] = val; }, noout, kc, dC, ct)
+        SCAL_STAGE(1, tt, sTT[s4(t - 1)], false, z1, { sZ1[s2(t - 1)][PAD + x] = val; }, noout, kt, dT, tt)
+        SCAL_STAGE(1, ct, sCT[s4(t - 1)], true, c1, { sC1[s2(t - 1)][PAD + x] = val; }, noout, kc, dC, ct)
         SCAL_STAGE(2, z1, sZ1[s2(t - 2)], false, z2, { sZ2[s2(t - 2)][PAD + x] = val; }, noout, kt, dT, tt)
         SCAL_STAGE(2, c1, sC1[s2(t - 2)], true, c2, { sC2[s2(t - 2)][PAD + x] = val; }, noout, kc, dC, ct)
-        SCAL_STAGE(3, z2, sZ2[s2(t - 3)], false, z3, {}, a.Tn, kt, dT, tt)
-        SCAL_STAGE(3, c2, sC2[s2(t - 3)], true, c3, {}, a.Cn, kc, dC, ct)
+        SCAL_STAGE(3, z2, sZ2[s2(t - 3)], false, z3, {}, Tn_out, kt, dT, tt)
+        SCAL_STAGE(3, c2, sC2[s2(t - 3)], true, c3, {}, Cn_out, kc, dC, ct)
 #undef SCAL_STAGE
+    };
+    for (int t = t0; t <= ihi + 2; t += 2) {
+        row(t, qa_t, qa_c, qa_u, qa_v, qa_T, qa_C, qa_e, qa_L);
+        row(t + 1, qb_t, qb_c, qb_u, qb_v, qb_T, qb_C, qb_e, qb_L);
     }
+#undef K2_FETCH
 }
 
 // ---------------------------------------------------------------------------------------
