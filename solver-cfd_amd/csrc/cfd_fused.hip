@@ -69,6 +69,13 @@ struct Win {
 __device__ __forceinline__ int s2(int r) { return (r + 24) & 1; }
 __device__ __forceinline__ int s3(int r) { return (r + 24) % 3; }
 __device__ __forceinline__ int s4(int r) { return (r + 24) & 3; }
+__device__ __forceinline__ int s8(int r) { return (r + 24) & 7; }
+// ring slot of row t + d (d a compile-time offset, -10 < d < 10) from m5 = (t + 40) % 5
+__device__ __forceinline__ int s5(int m5, int d)
+{
+    const int q = m5 + (d + 10) % 5;
+    return q >= 5 ? q - 5 : q;
+}
 
 __device__ __forceinline__ unsigned long long dbits(double v)
 {
@@ -125,10 +132,18 @@ struct K1Args {
     double mu[5];
 };
 
-template <typename R, int NT>
+// Two layouts of the stage values of u and v. RINGS = false (the one-wave kernel): each lane
+// keeps its own column's recent stage rows in register windows, and 4-row LDS rings serve
+// only the j-neighbours. RINGS = true (the two-wave kernel, 256 registers): the stage rows
+// live only in LDS, 5-row rings (slot = row mod 5) that hold rows r-2..r+2 for the next
+// stage, and the u_tau ring keeps rows t..t-6 so that the v stages' u_tau(j) + u_tau(j-1)
+// sums are read rather than kept; the workgroup then needs just under 80 KB, so that two fit
+// on a CU.
+template <typename R, int NT, bool RINGS>
 struct Smem1 {
-    R sUT[4][lrow(NT)], sVT[4][lrow(NT)], sT[2][lrow(NT)];
-    R sU1[4][lrow(NT)], sU2[4][lrow(NT)], sV1[4][lrow(NT)], sV2[4][lrow(NT)], sV3[2][lrow(NT)];
+    static constexpr int UTR = RINGS ? 8 : 4, SR = RINGS ? 5 : 4;
+    R sUT[UTR][lrow(NT)], sVT[4][lrow(NT)], sT[2][lrow(NT)];
+    R sU1[SR][lrow(NT)], sU2[SR][lrow(NT)], sV1[SR][lrow(NT)], sV2[SR][lrow(NT)], sV3[2][lrow(NT)];
     double sP[2][lrow(NT)];
     // per-row constants, read from here once per row rather than held in scalar registers
     // for the whole march
@@ -171,8 +186,8 @@ __device__ __forceinline__ void fix_v(R *row, int x, int j, int jm)
     }
 }
 
-template <typename R, bool STEADY, bool MUPOLY, int NT>
-__device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Smem1<R, NT> &sm)
+template <typename R, bool STEADY, bool MUPOLY, int NT, bool RINGS>
+__device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Smem1<R, NT, RINGS> &sm)
 {
     const int im = a.imax, jm = a.jmax;
     const int ilo = 2 + bi * a.hb, ihi = min(ilo + a.hb, im);        // output rows [ilo, ihi)
@@ -193,18 +208,20 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     }
     const double *cst = sm.cst;
     for (int k = x; k < lrow(NT); k += NT) {
-        for (int q = 0; q < 4; ++q) { sUT[q][k] = R(0); sVT[q][k] = R(0); }
-        for (int q = 0; q < 4; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
+        for (int q = 0; q < sm.UTR; ++q) sUT[q][k] = R(0);
+        for (int q = 0; q < 4; ++q) sVT[q][k] = R(0);
+        for (int q = 0; q < sm.SR; ++q) { sU1[q][k] = R(0); sU2[q][k] = R(0); sV1[q][k] = R(0); sV2[q][k] = R(0); }
         for (int q = 0; q < 2; ++q) { sP[q][k] = 0.0; sT[q][k] = R(1); sV3[q][k] = R(0); }
     }
 
     // per-column geometry (clamped indices; values outside the ranges are never used)
     const bool jvalid = j >= 1 && j <= jm + 1;
     const int jc = min(max(j, 2), jm);
-    const R haue_j = R(0.5 * pc[min(max(j, 1), jm)]);
     const double aue_jd = pc[min(max(j, 1), jm)];
     const R aue_j = R(aue_jd);
-    const R ave_j = R(pc[Cc + jc]), have_j = R(0.5 * pc[Cc + jc]);
+    const R ave_j = R(pc[Cc + jc]);
+    // (the one-wave form keeps the half face areas; the two-wave one forms them at use)
+    const R haue_j = RINGS ? R(0) : R(0.5) * aue_j, have_j = RINGS ? R(0) : R(0.5) * ave_j;
     const R rdy_j = R(pc[2 * Cc + jc]), rdy_n = R(pc[2 * Cc + min(jc + 1, jm)]);
     const R rdym_j = R(pc[3 * Cc + jc]), rdym_n = R(pc[3 * Cc + min(jc + 1, jm + 1)]);
     const bool ucol = j >= 2 && j <= jm - 1;           // u / p interior column
@@ -212,18 +229,20 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     const bool u_upw_col = upwind_all || j == 2 || j == jm - 1;
     const bool out_col = j >= jlo && j < jhi;
 
-    Win<7, R> ut, Su;     // u_tau rows t-6..t ; Su = u_tau(r,j) + u_tau(r,j-1)
+    Win<7, R> ut;         // u_tau rows t-6..t
+    Win<7, R> Su;         // one-wave form: u_tau(r,j) + u_tau(r,j-1), rows t-6..t
+    Win<5, R> w1, w2, x1, x2;   // one-wave form: u1, u2, v1, v2 own windows
     Win<8, R> vt;         // v_tau rows t-7..t
     Win<6, R> Svn;        // v_tau(r,j+1) + v_tau(r-1,j+1), rows t-6..t-1
     Win<6, double> pw;    // p rows t-7..t-2
     Win<5, R> ie, ew, nuw, k1w, k2w, dpj, bw, du, dv;   // per-cell data, rows t-6..t-2
-    Win<5, R> w1, w2, x1, x2;                            // u1, u2, v1, v2 own windows
     Win<2, R> w3, x3;                                    // u3, v3
-    ut.fill(R(0)); Su.fill(R(0)); vt.fill(R(0)); Svn.fill(R(0)); pw.fill(0.0);
+    ut.fill(R(0)); vt.fill(R(0)); Svn.fill(R(0)); pw.fill(0.0);
+    if (!RINGS) { Su.fill(R(0)); w1.fill(R(0)); w2.fill(R(0)); x1.fill(R(0)); x2.fill(R(0)); }
     ie.fill(R(1)); ew.fill(R(1)); k1w.fill(R(0)); k2w.fill(R(0)); dpj.fill(R(0)); bw.fill(R(0));
     if (MUPOLY) nuw.fill(R(0));
     if (!STEADY) { du.fill(R(0)); dv.fill(R(0)); }
-    w1.fill(R(0)); w2.fill(R(0)); x1.fill(R(0)); x2.fill(R(0)); w3.fill(R(0)); x3.fill(R(0));
+    w3.fill(R(0)); x3.fill(R(0));
     unsigned long long mu_ = 0, mv_ = 0, mp_ = 0;
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
@@ -259,6 +278,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     fetch(t0);
 
     auto row = [&](const int t) __attribute__((always_inline)) {
+        const int m5 = (t + 40) % 5;     // slot of row t in the 5-row stage rings
+        // slots of row t + d in the u_tau ring and in the stage rings
+#define UTS(d) (RINGS ? s8(t + (d)) : s4(t + (d)))
+#define STS(d) (RINGS ? s5(m5, (d)) : s4(t + (d)))
         // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
         // the defaults of rows and columns outside the arrays, applied where the values are
         // consumed (a select at the load would wait for it)
@@ -272,7 +295,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         const int rp = t - 2;
         ut.push(ut_new);
         vt.push(vt_new);
-        sUT[s4(t)][PAD + x] = ut_new;
+        sUT[UTS(0)][PAD + x] = ut_new;
         sVT[s4(t)][PAD + x] = vt_new;
         sP[s2(rp)][PAD + x] = p_rp;
         sT[s2(rp)][PAD + x] = T_rp;
@@ -280,12 +303,12 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
 
         // ---------------- phase B
         if (!chan) {   // boundary columns of the ring rows produced in the previous iteration
-            fix_u(sU1[s4(t - 3)], x, j, jm);
-            fix_u(sU2[s4(t - 5)], x, j, jm);
-            fix_v(sV1[s4(t - 3)], x, j, jm);
-            fix_v(sV2[s4(t - 5)], x, j, jm);
+            fix_u(sU1[STS(-3)], x, j, jm);
+            fix_u(sU2[STS(-5)], x, j, jm);
+            fix_v(sV1[STS(-3)], x, j, jm);
+            fix_v(sV2[STS(-5)], x, j, jm);
         }
-        Su.push(ut_new + sUT[s4(t)][PAD + x - 1]);
+        if (!RINGS) Su.push(ut_new + sUT[UTS(0)][PAD + x - 1]);
         Svn.push(sVT[s4(t - 1)][PAD + x + 1] + sVT[s4(t - 2)][PAD + x + 1]);
         pw.push(p_rp);
         {
@@ -311,7 +334,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         }
 
         // ---------------- U stage K at row r = t - 2K (u update: i = 3..imax-1, j = 2..jmax-1)
-#define U_STAGE(K, WP, RING_C, WOUT, RING_OUT)                                                   \
+#define U_STAGE(K, IN, RING_C, PREV, RING_OUT, PUSH)                                             \
         {                                                                                         \
             const int r = t - 2 * K;                                                              \
             const int bk = 2 * K - 2; /* back index of row r in the rows t-6..t-2 windows */      \
@@ -323,13 +346,13 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                     /* columns outside the u update keep 0 */                                     \
                     {                                                                             \
                         interior = ucol;                                                          \
-                        const R uP = WP[2], uW = WP[3], uWW = WP[4], uE = WP[1], uEE = WP[0];     \
+                        const R uP = IN(0), uW = IN(-1), uWW = IN(-2), uE = IN(1), uEE = IN(2);  \
                         const R *c = &RING_C[PAD + x];                                            \
                         const R uS = c[-1], uN = c[1];                                            \
                         const R iev = ie[bk], ev = ew[bk];                                        \
                         const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                              \
                         const R aun_r = R(pr[r]), rdx_r = R(pr[2 * RR + r]);                      \
-                        const R cns = R(0.5) * aun_r * iev, cew = haue_j * iev;                   \
+                        const R cns = R(0.5) * aun_r * iev, cew = (RINGS ? R(0.5) * aue_j : haue_j) * iev; \
                         const R sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;           \
                         const R fn = Svn[2 * K - 1] * cns;                                        \
                         const R fs = (vt[2 * K] + vt[2 * K + 1]) * cns;                           \
@@ -365,11 +388,11 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 } else if (r <= 2) {                                                              \
                     val = chan ? R(cst[CST_U_IN]) : R(0);                                                \
                 } else {                                                                          \
-                    val = (chan && ucol) ? WOUT[0] : R(0);                                        \
+                    val = (chan && ucol) ? PREV : R(0);                                           \
                 }                                                                                 \
                 RING_OUT                                                                          \
             }                                                                                     \
-            WOUT.push(val);                                                                       \
+            PUSH                                                                                  \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.un[(size_t)r * su + j] = val;                                                   \
                 const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_;       \
@@ -377,7 +400,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         }
 
         // ---------------- V stage K at row r = t - 2K (v update: i = 2..imax-1, j = 3..jmax-1)
-#define V_STAGE(K, WP, RING_C, WOUT, RING_OUT)                                                   \
+#define V_STAGE(K, IN, RING_C, PREV, RING_OUT, PUSH, SET_ROW1)                                   \
         {                                                                                         \
             const int r = t - 2 * K;                                                              \
             const int bk = 2 * K - 2;                                                             \
@@ -389,16 +412,16 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                     /* columns j <= 2 take their boundary value below */                         \
                     {                                                                             \
                         interior = vcol;                                                          \
-                        const R vP = WP[2], vW = WP[3], vWW = WP[4], vE = WP[1], vEE = WP[0];     \
+                        const R vP = IN(0), vW = IN(-1), vWW = IN(-2), vE = IN(1), vEE = IN(2);  \
                         const R *c = &RING_C[PAD + x];                                            \
                         const R vS = c[-1], vN = c[1];                                            \
                         const R iev = ie[bk], ev = ew[bk];                                        \
                         const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                              \
                         const R avn_r = R(pr[RR + r]), rdx_r = R(pr[2 * RR + r]);                 \
-                        const R cns = R(0.5) * avn_r * iev, cew = have_j * iev;                   \
+                        const R cns = R(0.5) * avn_r * iev, cew = (RINGS ? R(0.5) * ave_j : have_j) * iev; \
                         const R sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;           \
                         const R fn = sN * cns, fs = sS * cns;                                     \
-                        const R fe = Su[2 * K - 1] * cew, fw = Su[2 * K] * cew;                   \
+                        const R fe = SU(2 * K - 1) * cew, fw = SU(2 * K) * cew;                   \
                         const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[2 * RR + r + 1]) * (vE - vP)) \
                                      + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP));  \
                         R pw_, pe_, ps_, pn_;                                                     \
@@ -427,27 +450,71 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         else cv = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (vP + rs);                  \
                         val = vcol ? cv : R(0);                                                   \
                         /* row 1 = row 2 (boundary.f90:34-36), in the v update's columns */       \
-                        if (r == 2 && !chan) WOUT.set(0, vcol ? cv : WOUT[0]);                    \
+                        if (r == 2 && !chan) { SET_ROW1; }                                        \
                     }                                                                             \
                 } else if (r == im) {                                                             \
-                    val = vcol ? WOUT[0] : R(0);                                                  \
+                    val = vcol ? PREV : R(0);                                                     \
                 }                                                                                 \
                 if (j <= 2) val = chan ? R(0) : vi;                                               \
                 RING_OUT                                                                          \
             }                                                                                     \
-            WOUT.push(val);                                                                       \
+            PUSH                                                                                  \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.vn[(size_t)r * sv + j] = val;                                                   \
                 const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_;       \
             }                                                                                     \
         }
 
-        U_STAGE(1, ut, sUT[s4(t - 2)], w1, { sU1[s4(t - 2)][PAD + x] = val; })
-        V_STAGE(1, vt, sVT[s4(t - 2)], x1, { sV1[s4(t - 2)][PAD + x] = val; })
-        U_STAGE(2, w1, sU1[s4(t - 4)], w2, { sU2[s4(t - 4)][PAD + x] = val; })
-        V_STAGE(2, x1, sV1[s4(t - 4)], x2, { sV2[s4(t - 4)][PAD + x] = val; })
-        U_STAGE(3, w2, sU2[s4(t - 6)], w3, {})
-        V_STAGE(3, x2, sV2[s4(t - 6)], x3, { sV3[s2(t - 6)][PAD + x] = val; })
+        // u_tau(row t - q, j) + u_tau(row t - q, j - 1)
+#define SU(q) (RINGS ? ut[q] + sUT[UTS(-(q))][PAD + x - 1] : Su[q])
+        if (RINGS) {
+            // stage 1 reads the tau windows (rows t..t-4 = r+2..r-2), stages 2 and 3 the
+            // rings of the stage before, in the lane's own column; PREV is the stage's own
+            // previous row
+#define TAU_U(q) ut[2 - (q)]
+#define TAU_V(q) vt[2 - (q)]
+#define RU1(q) sU1[STS(-4 + (q))][PAD + x]
+#define RU2(q) sU2[STS(-6 + (q))][PAD + x]
+#define RV1(q) sV1[STS(-4 + (q))][PAD + x]
+#define RV2(q) sV2[STS(-6 + (q))][PAD + x]
+            U_STAGE(1, TAU_U, sUT[UTS(-2)], sU1[STS(-3)][PAD + x], { sU1[STS(-2)][PAD + x] = val; }, )
+            V_STAGE(1, TAU_V, sVT[s4(t - 2)], sV1[STS(-3)][PAD + x], { sV1[STS(-2)][PAD + x] = val; }, ,
+                    (sV1[STS(-3)][PAD + x] = vcol ? cv : sV1[STS(-3)][PAD + x]))
+            U_STAGE(2, RU1, sU1[STS(-4)], sU2[STS(-5)][PAD + x], { sU2[STS(-4)][PAD + x] = val; }, )
+            V_STAGE(2, RV1, sV1[STS(-4)], sV2[STS(-5)][PAD + x], { sV2[STS(-4)][PAD + x] = val; }, ,
+                    (sV2[STS(-5)][PAD + x] = vcol ? cv : sV2[STS(-5)][PAD + x]))
+            U_STAGE(3, RU2, sU2[STS(-6)], w3[0], {}, w3.push(val);)
+            V_STAGE(3, RV2, sV2[STS(-6)], x3[0], { sV3[s2(t - 6)][PAD + x] = val; }, x3.push(val);,
+                    (x3.set(0, vcol ? cv : x3[0])))
+#undef RU1
+#undef RU2
+#undef RV1
+#undef RV2
+        } else {
+            // register windows: the lane's own column of the stage before comes from them
+#define RW1(q) w1[2 - (q)]
+#define RW2(q) w2[2 - (q)]
+#define RX1(q) x1[2 - (q)]
+#define RX2(q) x2[2 - (q)]
+            U_STAGE(1, TAU_U, sUT[UTS(-2)], w1[0], { sU1[STS(-2)][PAD + x] = val; }, w1.push(val);)
+            V_STAGE(1, TAU_V, sVT[s4(t - 2)], x1[0], { sV1[STS(-2)][PAD + x] = val; }, x1.push(val);,
+                    (x1.set(0, vcol ? cv : x1[0])))
+            U_STAGE(2, RW1, sU1[STS(-4)], w2[0], { sU2[STS(-4)][PAD + x] = val; }, w2.push(val);)
+            V_STAGE(2, RX1, sV1[STS(-4)], x2[0], { sV2[STS(-4)][PAD + x] = val; }, x2.push(val);,
+                    (x2.set(0, vcol ? cv : x2[0])))
+            U_STAGE(3, RW2, sU2[STS(-6)], w3[0], {}, w3.push(val);)
+            V_STAGE(3, RX2, sV2[STS(-6)], x3[0], { sV3[s2(t - 6)][PAD + x] = val; }, x3.push(val);,
+                    (x3.set(0, vcol ? cv : x3[0])))
+#undef RW1
+#undef RW2
+#undef RX1
+#undef RX2
+        }
+#undef TAU_U
+#undef TAU_V
+#undef SU
+#undef UTS
+#undef STS
 #undef U_STAGE
 #undef V_STAGE
 
@@ -483,11 +550,11 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
 template <typename R, bool STEADY, bool MUPOLY, int OCC, int NT>
 __global__ __launch_bounds__(NT, OCC) void k_uvp(const K1Args<R> a)
 {
-    __shared__ Smem1<R, NT> sm;
+    __shared__ Smem1<R, NT, (OCC > 1)> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
     if (L >= G) return;
-    uvp_tile<R, STEADY, MUPOLY, NT>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1)>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 // Persistent batch: `cases` holds two argument sets per case (even and odd tau parity);
@@ -497,13 +564,13 @@ template <typename R, bool STEADY, bool MUPOLY, int OCC, int NT>
 __global__ __launch_bounds__(NT, OCC) void k_uvp_batch(const K1Args<R> *cases, const BatchTile *tiles,
                                                        int ntiles, const int *ctl)
 {
-    __shared__ Smem1<R, NT> sm;
+    __shared__ Smem1<R, NT, (OCC > 1)> sm;
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile tl = tiles[w];
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile<R, STEADY, MUPOLY, NT>(a, tl.bi, tl.bj, sm);
+        uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1)>(a, tl.bi, tl.bj, sm);
         __syncthreads();   // the next tile re-initialises the LDS rings
     }
 }
@@ -530,7 +597,6 @@ struct Smem1S {
 
 // The v ring is read down to row t-7 by the u stages while a v wave may already write row
 // t+1, so it holds 16 rows; the u ring is read down to row t-6 and holds 8.
-__device__ __forceinline__ int s8(int r) { return (r + 24) & 7; }
 __device__ __forceinline__ int s16(int r) { return (r + 32) & 15; }
 
 template <typename R, bool STEADY, bool MUPOLY, int NT>
@@ -1328,14 +1394,15 @@ __host__ __device__ inline int materialize_tc_start(int im, int jm)
 // ---------------------------------------------------------------------------------------
 namespace {
 
-// Row-block height: `want` > 0 (from the autotuner) or one block per CU, at least min_rows.
-int pick_rows(int rows, int nbj, int min_rows, int want, int *hb)
+// Row-block height: `want` > 0 (from the autotuner) or `per_cu` blocks per CU (the blocks
+// that fit on one CU at once), at least min_rows.
+int pick_rows(int rows, int nbj, int min_rows, int want, int per_cu, int *hb)
 {
     if (want > 0) {
         *hb = want < rows ? want : rows;
         return cdiv(rows, *hb);
     }
-    int nbi = 256 / nbj;
+    int nbi = 256 * per_cu / nbj;
     if (nbi < 1) nbi = 1;
     if (rows / nbi < min_rows) nbi = rows / min_rows > 0 ? rows / min_rows : 1;
     *hb = cdiv(rows, nbi);
@@ -1358,7 +1425,8 @@ void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params
     k1.invfr2 = P->invfr2; k1.ri = P->ri; k1.t_ref = P->t_ref; k1.v_i = P->v_i; k1.u_in = P->u_in;
     for (int q = 0; q < 5; ++q) k1.mu[q] = P->mu_coef[q];
     k1.nbj = cdiv(jm - 2, w1(nt));
-    k1.nbi = pick_rows(im - 2, k1.nbj, a->lone ? 4 : 24, a->hb1, &k1.hb);
+    // the two-wave kernel fits two workgroups on a CU
+    k1.nbi = pick_rows(im - 2, k1.nbj, a->lone ? 4 : 24, a->hb1, a->occ1 == 2 ? 2 : 1, &k1.hb);
 }
 
 template <typename R>
@@ -1375,7 +1443,9 @@ void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack,
     k2.tinf = P->tinf; k2.t_bottom = P->t_bottom; k2.t_top = P->t_top; k2.v_i = P->v_i; k2.u_in = P->u_in;
     k2.slot_lo = P->slot_lo; k2.slot_hi = P->slot_hi; k2.slot_value = P->slot_value;
     k2.nbj = cdiv(jm - 2, w2(nt));
-    k2.nbi = pick_rows(im - 2, k2.nbj, a->lone ? 4 : 16, a->hb2, &k2.hb);
+    // beside the one-wave u/v/p kernel (the two-stream step) one T/C workgroup per CU fits;
+    // after the two-wave one, which fills the CUs, two
+    k2.nbi = pick_rows(im - 2, k2.nbj, a->lone ? 4 : 16, a->hb2, a->occ1 == 2 ? 2 : 1, &k2.hb);
 }
 
 // Column span of a single grid's workgroups: 64 when 64-column strips need no more lanes
@@ -1809,7 +1879,7 @@ int cfdk_batch_case_tiles(int kernel, int32_t span, int32_t imax, int32_t jmax, 
     const int W = kernel == 1 ? w1(span) : w2(span);
     const int nbj = cdiv(jmax - 2, W);
     int h = 0;
-    const int nbi = pick_rows(imax - 2, nbj, kernel == 1 ? 24 : 16, hb, &h);
+    const int nbi = pick_rows(imax - 2, nbj, kernel == 1 ? 24 : 16, hb, 1, &h);
     if (out) {
         int k = 0;
         for (int bi = 0; bi < nbi; ++bi)

@@ -1102,7 +1102,7 @@ static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_
 static const cfd_tiles CAND_UVP[] = {
     {0, 0, 1, 0}, {4, 0, 1, 0}, {8, 0, 1, 0}, {12, 0, 1, 0}, {24, 0, 1, 0}, {32, 0, 1, 0}, {48, 0, 1, 0},
     {64, 0, 1, 0}, {96, 0, 1, 0}, {128, 0, 1, 0},
-    {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
+    {0, 0, 2, 0}, {16, 0, 2, 0}, {24, 0, 2, 0}, {32, 0, 2, 0}, {40, 0, 2, 0}, {48, 0, 2, 0}, {64, 0, 2, 0},
     {0, 0, 3, 0}, {4, 0, 3, 0}, {8, 0, 3, 0}, {12, 0, 3, 0}, {16, 0, 3, 0}, {24, 0, 3, 0}, {32, 0, 3, 0},
     {48, 0, 3, 0}, {64, 0, 3, 0}, {96, 0, 3, 0},
 };
