@@ -180,6 +180,43 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_batch_steps(args):
+    """Throughput of batched pseudo-steps on the seeded 256-case mix (no convergence test
+    stops a case: tol 0): cell-updates/s of the persistent kernels. --batch-eager issues the
+    launches directly instead of replaying a hipGraph (for kernel tracing)."""
+    prec = PREC_FP64 if args.precision == "fp64" else PREC_FP32
+    cs = C.case_mixed_batch(n=args.batch_cases, beta=args.batch_beta)
+    cells = int(sum(c["params"].imax * c["params"].jmax for c in cs))
+    b = _build_batch(cs, prec)
+    b.set_graph(not args.batch_eager)
+    if args.tiles:
+        b.set_tiles(*(int(v) for v in args.tiles.split(",")))
+    elif not args.no_autotune:
+        b.autotune(steps=20)
+    tiles = b.get_tiles()
+    log("batch of %d cases, %d cells, %s, tiles %s, %s" % (len(cs), cells, args.precision, tiles,
+                                                       "eager launches" if args.batch_eager else "graph replays"))
+    spg = min(50, max(1, args.steps))
+    b.run(0.0, max(args.warmup, 1), steps_per_graph=min(spg, max(args.warmup, 1)))
+    t0 = time.perf_counter()
+    code, it, res, st = b.run(0.0, args.steps, steps_per_graph=spg)
+    wall = time.perf_counter() - t0
+    ms, steps, cu = b.last_timing()
+    if not np.all(np.isfinite(res)):
+        raise SystemExit("non-finite residual in the batch")
+    b.close()
+    out = {"metric": "cell-updates/s", "value": cu / wall, "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if prec == PREC_FP64 else "f32+f64",
+           "data": "synthetic (seeded mixed batch: %d cases)" % len(cs),
+           "config": {"workload": "mixed batch of %d steady channel cases, grids {64..512}x{64..256}, %s"
+                                  % (len(cs), args.precision), "cells": cells, "tiles": list(tiles),
+                      "graph": not args.batch_eager},
+           "event_ms_per_step": ms / max(steps, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_warmstart(args):
     """Learned warm start on held-out cases: solver-generated steady states (one grid size)
     train the U-Net with the library's own kernels; the held-out cases are then solved from
@@ -420,7 +457,9 @@ def main():
     ap.add_argument("--pipeline", type=int, default=1, help="two-stream pipelined pseudo-steps (1) or one stream (0)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with plain copy kernels of known size (memory-counter calibration)")
-    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "warmstart", "unet", "fp32steady"],
+    ap.add_argument("--batch-eager", action="store_true", help="batch launches issued directly (no hipGraph)")
+    ap.add_argument("--workload", default="grid", choices=["grid", "batch", "batchsteps", "warmstart", "unet",
+                                                           "fp32steady"],
                     help="grid: one large grid (the cell-update rate); batch: time to residual of the "
                          "256-case mixed batch against the plain FP64 path")
     ap.add_argument("--batch-tol", type=float, default=1e-6)
@@ -440,6 +479,8 @@ def main():
     args = ap.parse_args()
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "batchsteps":
+        return bench_batch_steps(args)
     if args.workload == "warmstart":
         return bench_warmstart(args)
     if args.workload == "unet":
@@ -546,6 +587,12 @@ def main():
         "kernel_ms": kern,
     }
     ctx.close()
+    # measured copy bandwidth of this GPU (1 GiB buffers), reported beside the nominal peak
+    copy_gbs = L.hbm_copy_bandwidth(1 << 30, 20)
+    log("stream copy: %.0f GB/s (read + write) against the nominal %.0f GB/s" % (copy_gbs, HBM_PEAK_GBS))
+    out["roofline"]["measured_copy_gbs"] = copy_gbs
+    out["roofline"]["frac_of_measured_copy"] = achieved / copy_gbs
+    out["step_roofline"]["frac_of_measured_copy"] = step_gbs / copy_gbs
     if not args.no_cpu:
         log("cpu baseline (%.0f s sample)" % args.cpu_seconds)
         port = cpu_baseline(cfg, args.cpu_seconds)

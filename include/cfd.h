@@ -129,6 +129,11 @@ typedef struct cfd_params {
  * dtau=0.05, dt=0.005, beta=1300, b_art=0, QUICK, reference BCs. */
 void cfd_params_default(cfd_params *p, int32_t imax, int32_t jmax);
 
+/* Measured HBM bandwidth of this device: a plain copy kernel over `bytes` (two buffers of
+ * that size), `reps` timed launches; *gbs = bytes read + bytes written per second / 1e9. The
+ * roofline figures of the benchmark report it beside the nominal 8 TB/s. */
+int cfd_hbm_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs);
+
 /* ---- geometry ---------------------------------------------------------------------- */
 /* Node coordinates x[1..imax], y[1..jmax] (arrays of imax+1 / jmax+1 doubles, index 0 unused)
  * give the staggered midpoints and face areas by the reference formulas
@@ -331,6 +336,10 @@ int  cfd_batch_resume(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_p
  * grid sizes), converting precision on the device: e.g. an FP32 batch that has reached the
  * residual its storage can resolve hands over to an FP64 batch for the last decades. */
 int  cfd_batch_copy_state(cfd_batch *dst, cfd_batch *src);
+/* use_graph 1 (default): a run captures steps_per_graph pseudo-steps in a hipGraph once and
+ * replays it; 0: the same launches are issued directly each round (same kernels, same
+ * results; for profilers and callers that do not use graphs). */
+int  cfd_batch_set_graph(cfd_batch *b, int32_t use_graph);
 /* Wall time of the last run (HIP events on the batch stream), graph steps replayed, and
  * cell-updates done (sum over cases of pseudo-steps x imax x jmax). */
 int  cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates);

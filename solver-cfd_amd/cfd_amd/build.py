@@ -16,6 +16,9 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libcfd_amd.so")
 REPO = os.path.dirname(ROOT)
 FORTRAN_MOD = os.path.join(ROOT, "fortran", "cfd_amd.f90")
+FORTRAN_DRIVER = os.path.join(ROOT, "fortran", "cfd_fortran_driver.f90")
+# the Fortran main program that runs main.f90's loop through the library (built next to it)
+DRIVER = os.path.join(PKG, "cfd_fortran_driver")
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -36,7 +39,7 @@ HIP_SOURCES = [
     ("cfd_unet.hip", ["-ffp-contract=fast"]),
     ("cfd_noalloc.hip", ["-ffp-contract=off"]),
 ]
-C_SOURCES = ["cfd_host.c", "cfd_batch.c", "cfd_mesh_io.c"]
+C_SOURCES = ["cfd_host.c", "cfd_batch.c", "cfd_mesh_io.c", "cfd_fortran.c"]
 
 
 def _run(cmd):
@@ -64,10 +67,16 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
          ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-lm", "-lpthread", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
-    # the Fortran binding module is compile-checked when a Fortran compiler is present
+    # the Fortran binding module and the Fortran driver (main.f90's loop calling the
+    # library's Fortran entry points) are built when a Fortran compiler is present
     fc = os.path.join(ROCM, "lib", "llvm", "bin", "flang")
     if os.path.exists(FORTRAN_MOD) and os.path.exists(fc):
-        _run([fc, "-c", FORTRAN_MOD, "-module-dir", BUILD, "-o", os.path.join(BUILD, "cfd_amd_f.o")])
+        modo = os.path.join(BUILD, "cfd_amd_f.o")
+        _run([fc, "-c", FORTRAN_MOD, "-module-dir", BUILD, "-o", modo])
+        tmpd = DRIVER + ".tmp"
+        _run([fc, "-O2", "-module-dir", BUILD, "-o", tmpd, FORTRAN_DRIVER, modo, "-L" + PKG, "-lcfd_amd",
+              "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+        os.replace(tmpd, DRIVER)
     if verbose:
         print("built", LIB)
     return LIB
@@ -75,8 +84,9 @@ def build(verbose=False):
 
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
-    if os.path.exists(LIB):
-        os.remove(LIB)
+    for f in (LIB, DRIVER):
+        if os.path.exists(f):
+            os.remove(f)
 
 
 if __name__ == "__main__":

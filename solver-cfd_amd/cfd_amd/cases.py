@@ -183,11 +183,36 @@ def case_channel_upwind():
     return dict(name="channel_upwind_51x51", params=prm, x=x, y=y, eps=eps, lp=lp, state=st)
 
 
-def case_porous_buoyant(n=204):
+def ac_beta_limit(x, y, dt, dtau):
+    """Largest artificial compressibility beta for which the fastest pressure-velocity mode
+    of one SSP-RK3 pseudo-step stays inside the method's stability interval on the
+    imaginary axis (|z| <= sqrt(3)). Per stage the u update carries dtau dt eps dp/dx
+    (equations.f90:96, 286-297) and the p update dtau beta RP with RP ~ du * dy (the face
+    areas, equations.f90:618-622), so the highest checkerboard mode has
+    z^2 = 4 dtau^2 dt beta (dy/dx + dx/dy); with the widest aspect ratio of the mesh:
+    beta_max = 3 / (4 dtau^2 dt (max dy / min dx + max dx / min dy))."""
+    dx = np.diff(np.asarray(x, dtype=np.float64)[1:])
+    dy = np.diff(np.asarray(y, dtype=np.float64)[1:])
+    r = float(dy.max() / dx.min() + dx.max() / dy.min())
+    return 3.0 / (4.0 * dtau * dtau * dt * r)
+
+
+def case_porous_buoyant(n=204, beta=None):
     """204x204 uniform, QUICK, Re=200, porous middle third eps~U[0.4,0.9] (seed 2),
-    Da=1e-3, Cf=0.55, Boussinesq Ri=1 with hot bottom (1) and cold top (0)."""
+    Da=1e-3, Cf=0.55, Boussinesq Ri=1 with hot bottom (1) and cold top (0); dual time with
+    the reference's dtau = 0.05, dt = 0.005, from zero velocities.
+
+    beta (artificial compressibility): the continuity update pn = p + dtau beta RP carries
+    the face areas (RP ~ div(u) dx dy), so the slowest pressure mode of the pseudo-time
+    iteration decays at a rate ~ dtau dt beta pi^2 dx dy per step, while the fastest
+    pressure-velocity mode stays stable only below ac_beta_limit (3e4 here, independent of
+    the grid size). beta = 1 needs more than 2e6 pseudo-steps to reach a residual of 1e-8 at
+    204x204 (measured: 6.3e-6 after 2e6); the default, half the limit, converges
+    geometrically."""
     x = uniform_nodes(n); y = uniform_nodes(n)
-    prm = make_params(n, n, bc=BC_CHANNEL, re=200.0, pe=140.0, beta=1.0, g=0.0, darcy=1e-3, cf=0.55,
+    if beta is None:
+        beta = 0.5 * ac_beta_limit(x, y, 0.005, 0.05)
+    prm = make_params(n, n, bc=BC_CHANNEL, re=200.0, pe=140.0, beta=beta, g=0.0, darcy=1e-3, cf=0.55,
                       buoyancy=BUOY_BOUSSINESQ, ri=1.0, t_ref=0.5, invfr2=0.0,
                       u_in=1.0, t_bottom=1.0, t_top=0.0, slot_lo=1.0, slot_hi=0.0)
     eps, lp = porous_block(x, y, 1 / 3, 2 / 3, 1 / 3, 2 / 3, 0.4, 0.9, seed=2)
@@ -246,7 +271,13 @@ def case_mixed_batch(n=256, seed=4, sizes_i=(64, 128, 256, 512), sizes_j=(64, 12
     porosity per cell uniform [0.3, 0.95] inside one random porous rectangle covering 10-40 %
     of the unit square, Boussinesq buoyancy with Ri uniform [0, 5] (hot bottom, cold top),
     species slot Z = 1 of width 0.1 on the inflow wall, Sc = 0.7, channel BC set, zero
-    initial fields. Returns a list of dicts (name, params, x, y, eps, lp, state)."""
+    initial fields. Returns a list of dicts (name, params, x, y, eps, lp, state).
+
+    A larger artificial compressibility (beta up to a fraction of ac_beta_limit, which speeds
+    up the pressure modes) does not make this mix converge: with beta at 0.03, 0.1 and 0.3 of
+    each case's limit no case reaches a residual of 1e-8 within 1e5 pseudo-steps in FP64
+    (the residuals stay at 1e-5 to 1: at Re up to 500 and Ri up to 5 with a heated bottom
+    wall most cases have no steady state to converge to)."""
     r = uniform(seed, 12 * n).reshape(n, 12)
     out = []
     for c in range(n):

@@ -1,4 +1,4 @@
-"""ctypes binding of libcfd_amd.so (include/cfd.h, include/cfd_legacy.h).
+"""ctypes binding of libcfd_amd.so (include/cfd.h, include/cfd_legacy.h, include/cfd_fortran.h).
 
 The library is loaded from this package directory; a missing library is an error, never a
 silent fallback. ``Context`` wraps one cfd_ctx: numpy arrays in the C layout go in and out,
@@ -28,12 +28,12 @@ EXPORTS_CFD = [
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
-    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_mesh_reference", "cfd_porous_flags_reference",
+    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_batch_set_graph", "cfd_mesh_reference", "cfd_porous_flags_reference",
     "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_num_params", "cfd_unet_get_params", "cfd_unet_set_params",
     "cfd_unet_predict", "cfd_unet_loss_grad", "cfd_unet_train_step", "cfd_unet_last_timing", "cfd_nn_conv3x3", "cfd_nn_conv3x3_wgrad",
     "cfd_nn_flip_weights", "cfd_nn_maxpool2", "cfd_nn_maxpool2_bwd", "cfd_nn_upsample2", "cfd_nn_upsample2_bwd",
     "cfd_nodal_velocity", "cfd_write_state_table", "cfd_read_state_table", "cfd_write_restart", "cfd_read_restart",
-    "cfd_write_vtk",
+    "cfd_write_vtk", "cfd_hbm_copy_bandwidth",
     "cfd_last_error", "cfd_version",
 ]
 EXPORTS_LEGACY = [
@@ -41,6 +41,10 @@ EXPORTS_LEGACY = [
     "solve_Z", "solve_C", "bcUV", "bcP", "bcZ", "bcC", "max_reduce", "alocar_globais", "desalocar_globais",
     "cfd_legacy_init", "cfd_legacy_set_params", "cfd_legacy_set_mesh", "cfd_legacy_context",
     "max_reduce_noalloc",
+]
+EXPORTS_FORTRAN = [
+    "cfd_fortran_setup", "cfd_fortran_finalize", "solve_u_", "solve_v_", "solve_p_", "solve_z_", "solve_c_",
+    "resu_", "resv_", "resz_", "resc_", "bcuv_", "bcp_", "bcz_", "bcc_",
 ]
 
 
@@ -120,6 +124,7 @@ def load():
         "cfd_batch_run": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_batch_last_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_batch_copy_state": (ctypes.c_int, [_vp, _vp]),
+        "cfd_batch_set_graph": (ctypes.c_int, [_vp, i32]),
         "cfd_batch_resume": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_mesh_reference": (ctypes.c_int, [i32, i32, dbl, dbl, dbl, dbl, dbl, _dp, _dp]),
         "cfd_porous_flags_reference": (ctypes.c_int, [i32, i32, _dp, _dp, dbl, dbl, dbl, _dp, _dp,
@@ -146,6 +151,7 @@ def load():
         "cfd_nn_maxpool2_bwd": (ctypes.c_int, [_vp, _vp, _vp, i32, i32, i32, i32, _vp]),
         "cfd_nn_upsample2": (ctypes.c_int, [_vp, i32, i32, i32, i32, _vp, i32, i32]),
         "cfd_nn_upsample2_bwd": (ctypes.c_int, [_vp, i32, i32, i32, i32, i32, i32, _vp]),
+        "cfd_hbm_copy_bandwidth": (ctypes.c_int, [i64, i32, _dp]),
         "cfd_last_error": (ctypes.c_char_p, []),
         "cfd_version": (ctypes.c_char_p, []),
         "cfd_legacy_init": (ctypes.c_int, [P]),
@@ -160,6 +166,12 @@ def load():
         "max_reduce": (dbl, [_vp, ctypes.c_int, ctypes.c_int]),
         "max_reduce_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
         "alocar_globais": (None, []), "desalocar_globais": (None, []),
+        # Fortran entry points: every argument by reference, host arrays in column-major order
+        "cfd_fortran_setup": (ctypes.c_int, [P, _dp, _dp, _dp, _dp]), "cfd_fortran_finalize": (None, []),
+        "solve_u_": (None, [_dp] * 8), "solve_v_": (None, [_dp] * 9), "solve_p_": (None, [_dp] * 5),
+        "solve_z_": (None, [_dp] * 5), "solve_c_": (None, [_dp] * 5),
+        "resu_": (None, [_dp] * 4), "resv_": (None, [_dp] * 5), "resz_": (None, [_dp] * 4), "resc_": (None, [_dp] * 4),
+        "bcuv_": (None, [_dp] * 2), "bcp_": (None, [_dp]), "bcz_": (None, [_dp]), "bcc_": (None, [_dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -167,6 +179,13 @@ def load():
         f.argtypes = args
     _lib = L
     return L
+
+
+def hbm_copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Measured copy bandwidth (GB/s, read + write) of the current device."""
+    g = ctypes.c_double(0.0)
+    check(load().cfd_hbm_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g)))
+    return g.value
 
 
 def last_error():
@@ -403,6 +422,10 @@ class Batch:
         if raise_on_fail:
             check(code)
         return code, np.array(it[:], dtype=np.int64), res, np.array(st[:], dtype=np.int32)
+
+    def set_graph(self, on):
+        """True (default): runs replay a captured hipGraph; False: direct launches."""
+        check(load().cfd_batch_set_graph(self.h, 1 if on else 0))
 
     def copy_state_from(self, other):
         """Take every case's current iterates from another batch of the same cases

@@ -55,6 +55,7 @@ struct cfd_batch {
     int32_t span;                      /* columns per workgroup of the batch kernels */
     cfd_tiles tiles;
     int args_dirty;
+    int eager;                  /* 1: launch the steps directly instead of replaying a graph */
     hipEvent_t ev0, ev1;
     double last_ms;
     int64_t last_steps, last_cells;
@@ -530,34 +531,54 @@ static int batch_run(cfd_batch *b, int resume, double tol, int64_t itc_max, int3
         free(ctl); free(st);
         if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch resume: %s", hipGetErrorString(e));
     }
-    /* capture steps_per_graph batched steps + the active count */
+    /* capture steps_per_graph batched steps + the active count (eager mode launches the
+     * same sequence directly each round) */
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
-    BHIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-    int r = CFD_OK;
-    for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) r = cfdk_batch_step(&L, b->s);
-    if (r == CFD_OK) r = cfdk_batch_count(&L, b->s);
-    hipError_t e = hipStreamEndCapture(b->s, &graph);
-    if (r != CFD_OK) { if (graph) hipGraphDestroy(graph); return cfd__fail(r, "batch capture failed"); }
-    if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&exec, graph, NULL, NULL, 0);
-    if (e != hipSuccess) { hipGraphDestroy(graph); return cfd__fail(CFD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
-    int32_t active = b->n;
+    hipError_t e = hipSuccess;
+    if (!b->eager) {
+        BHIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        int r = CFD_OK;
+        for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) r = cfdk_batch_step(&L, b->s);
+        if (r == CFD_OK) r = cfdk_batch_count(&L, b->s);
+        e = hipStreamEndCapture(b->s, &graph);
+        if (r != CFD_OK) { if (graph) hipGraphDestroy(graph); return cfd__fail(r, "batch capture failed"); }
+        if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, NULL, NULL, 0);
+        if (e != hipSuccess) { hipGraphDestroy(graph); return cfd__fail(CFD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+    }
+    /* the active count is read back into pinned memory */
+    int32_t *active = NULL;
+    if (hipHostMalloc((void **)&active, sizeof *active, hipHostMallocDefault) != hipSuccess) {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        return cfd__fail(CFD_ERR_HIP, "batch run: pinned allocation failed");
+    }
+    *active = b->n;
     int64_t replays = 0;
+    int r = CFD_OK;
     const int64_t max_replays = (itc_max + steps_per_graph - 1) / steps_per_graph;
     e = hipEventRecord(b->ev0, b->s);
-    while (e == hipSuccess && active > 0 && replays < max_replays) {
-        e = hipGraphLaunch(exec, b->s);
-        if (e == hipSuccess) e = hipMemcpyAsync(&active, b->nactive, sizeof active, hipMemcpyDeviceToHost, b->s);
-        if (e == hipSuccess) e = hipStreamSynchronize(b->s);
+    while (e == hipSuccess && r == CFD_OK && *active > 0 && replays < max_replays) {
+        if (!b->eager) {
+            e = hipGraphLaunch(exec, b->s);
+        } else {
+            for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) r = cfdk_batch_step(&L, b->s);
+            if (r == CFD_OK) r = cfdk_batch_count(&L, b->s);
+        }
+        if (e == hipSuccess && r == CFD_OK)
+            e = hipMemcpyAsync(active, b->nactive, sizeof *active, hipMemcpyDeviceToHost, b->s);
+        if (e == hipSuccess && r == CFD_OK) e = hipStreamSynchronize(b->s);
         ++replays;
     }
-    if (e == hipSuccess) e = hipEventRecord(b->ev1, b->s);
-    if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+    if (e == hipSuccess && r == CFD_OK) e = hipEventRecord(b->ev1, b->s);
+    if (e == hipSuccess && r == CFD_OK) e = hipEventSynchronize(b->ev1);
     float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
+    if (e == hipSuccess && r == CFD_OK) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    hipHostFree(active);
+    if (r != CFD_OK) return cfd__fail(r, "batch step launch failed");
     if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch run: %s", hipGetErrorString(e));
     /* results */
     const int32_t n = b->n;
@@ -598,6 +619,13 @@ int cfd_batch_resume(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_pe
                      int64_t *iters, double *res, int32_t *status)
 {
     return batch_run(b, 1, tol, itc_max, steps_per_graph, diverge_factor, iters, res, status);
+}
+
+int cfd_batch_set_graph(cfd_batch *b, int32_t use_graph)
+{
+    if (!b) return cfd__fail(CFD_ERR_STATE, "no batch");
+    b->eager = use_graph ? 0 : 1;
+    return CFD_OK;
 }
 
 int cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates)

@@ -218,6 +218,25 @@ __global__ void k_copy(double *dst, const double *src, long n)
         dst[t] = src[t];
 }
 
+// Stream copy for the bandwidth probe: 16-byte elements, each 256-lane block moves 4 KB per
+// lane group (four coalesced 16 B loads in flight per lane, then four stores).
+constexpr int SC_PER = 4;
+__global__ __launch_bounds__(256) void k_stream_copy(uint4 *__restrict__ dst, const uint4 *__restrict__ src, long n)
+{
+    const long base = (long)blockIdx.x * (256 * SC_PER) + threadIdx.x;
+    uint4 v[SC_PER];
+#pragma unroll
+    for (int k = 0; k < SC_PER; ++k) {
+        const long t = base + (long)k * 256;
+        if (t < n) v[k] = src[t];
+    }
+#pragma unroll
+    for (int k = 0; k < SC_PER; ++k) {
+        const long t = base + (long)k * 256;
+        if (t < n) dst[t] = v[k];
+    }
+}
+
 __global__ void k_fill(double *dst, double v, long n)
 {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
@@ -345,6 +364,36 @@ int cfdk_copy(double *dst, const double *src, long n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy, dim3(reduce_blocks(n)), dim3(256), 0, s, dst, src, n);
     return launched();
+}
+
+/* HBM bandwidth probe: a plain copy of `bytes` (rounded down to 16 B) between two fresh
+ * device buffers, `reps` timed launches after one warm-up; *gbs = 2 * bytes * reps / time. */
+int cfd_hbm_copy_bandwidth(int64_t bytes, int32_t reps, double *gbs)
+{
+    if (bytes < 16 || reps < 1 || !gbs) return CFD_ERR_ARG;
+    const long n = (long)(bytes / 16);
+    uint4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int r = CFD_ERR_HIP;
+    if (hipMalloc((void **)&a, (size_t)n * 16) == hipSuccess && hipMalloc((void **)&b, (size_t)n * 16) == hipSuccess &&
+        hipMemset(a, 0, (size_t)n * 16) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        const dim3 grid(cdiv(n, 256 * SC_PER));
+        hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, 0, b, a, n);
+        hipEventRecord(e0, 0);
+        for (int q = 0; q < reps; ++q) hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, 0, b, a, n);
+        hipEventRecord(e1, 0);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f) {
+            *gbs = 2.0 * (double)n * 16.0 * reps / ((double)ms * 1e-3) * 1e-9;
+            r = CFD_OK;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return r;
 }
 
 int cfdk_fill(double *dst, double v, long n, hipStream_t s)

@@ -105,5 +105,18 @@ module cfd_amd
         type(c_ptr) function cfd_last_error() bind(C, name="cfd_last_error")
             import :: c_ptr
         end function
+
+        ! include/cfd_fortran.h: hands the module state of comum.f90 (parameters, nodes
+        ! x(1:imax), y(1:jmax), epsilon1 and liga_poros) to the library once; afterwards a
+        ! driver CALLs solve_U, solve_V, solve_P, solve_Z, solve_C, RESU, ..., bcUV, ... as
+        ! the reference's main.f90 does (external procedures, implicit interface, host arrays).
+        integer(c_int) function cfd_fortran_setup(p, x, y, epsilon1, liga_poros) bind(C, name="cfd_fortran_setup")
+            import :: cfd_params, c_int, c_double
+            type(cfd_params), intent(in) :: p
+            real(c_double), intent(in) :: x(*), y(*), epsilon1(*), liga_poros(*)
+        end function
+
+        subroutine cfd_fortran_finalize() bind(C, name="cfd_fortran_finalize")
+        end subroutine
     end interface
 end module cfd_amd

@@ -27,11 +27,32 @@ def lib():
 
 
 def test_exports_every_declared_symbol(lib):
-    for header in ("cfd.h", "cfd_legacy.h"):
+    for header in ("cfd.h", "cfd_legacy.h", "cfd_fortran.h"):
         for name in sorted(_declared(header)):
             assert hasattr(lib, name), "%s declares %s but the library does not export it" % (header, name)
     assert set(L.EXPORTS_CFD) <= _declared("cfd.h")
     assert set(L.EXPORTS_LEGACY) <= _declared("cfd_legacy.h")
+    assert set(L.EXPORTS_FORTRAN) == _declared("cfd_fortran.h")
+
+
+def test_fortran_driver_links_the_reference_call_names(lib):
+    """The Fortran driver (main.f90's loop) is built and resolves solve_U ... solve_C to the
+    library's Fortran-ABI symbols; without a GPU, a call reports that setup is missing."""
+    import subprocess
+    from cfd_amd import build as B
+    if not os.path.exists("/opt/rocm/lib/llvm/bin/flang"):
+        pytest.skip("no Fortran compiler in this image")
+    assert os.path.exists(B.DRIVER), "build() did not produce the Fortran driver"
+    und = subprocess.run(["nm", "-D", "--undefined-only", B.DRIVER], capture_output=True, text=True).stdout
+    for name in ("solve_u_", "solve_v_", "solve_p_", "solve_z_", "solve_c_", "cfd_fortran_setup"):
+        assert re.search(r"\b%s\b" % name, und), name
+    defined = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True).stdout
+    for name in L.EXPORTS_FORTRAN:
+        assert re.search(r" T %s$" % name, defined, flags=re.M), name
+    r = ctypes.c_double(0.0)
+    z = np.zeros(4)
+    lib.solve_p_(*([z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))] * 4), ctypes.byref(r))
+    assert np.isnan(r.value) and "cfd_fortran_setup" in L.last_error()
 
 
 def test_params_struct_layout_matches_header(lib):
