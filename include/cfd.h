@@ -239,6 +239,16 @@ int  cfd_ctx_autotune(cfd_ctx *ctx, int32_t steps, cfd_tiles *out, double *out_m
 /* Pseudo-steps taken by each path since the context was created: fused kernels and the
  * exact per-operator path (the latter runs the first step after new inputs, and every step
  * with b_art != 0 or CFD_COMPAT_CUDA). */
+/* L2 residual norms (no counterpart in the reference, whose equations.f90:298,600,662 take
+ * MAXVAL(ABS(res_*)) only). With on = 1 every later pseudo-step also folds, in the same pass
+ * as the maxima, the sums of squares of the final-stage increments r3 of u, v and p over the
+ * update regions (the entries whose maximum is the step's residual); FP64 in both
+ * precisions, summed in no fixed order (the last bits may vary from run to run). Off by
+ * default: the folding costs the two-wave u/v/p kernel register spills.
+ * cfd_ctx_last_residual_l2 returns sqrt of the sums of the last step (CFD_ERR_STATE when
+ * off). */
+int  cfd_ctx_set_residual_l2(cfd_ctx *c, int32_t on);
+int  cfd_ctx_last_residual_l2(cfd_ctx *c, double l2[3]);
 int  cfd_ctx_step_counts(const cfd_ctx *ctx, int64_t *fused, int64_t *exact);
 
 /* `nsteps` pseudo-steps like cfd_pseudo_steps, with events around each launch of the fused

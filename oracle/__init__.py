@@ -77,6 +77,7 @@ def lib():
         L.orc_mesh_ref_x.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_maxabs.restype = ctypes.c_double
         L.orc_maxabs.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_last_res_l2.argtypes = [vp, _dp]
         _lib = L
     return _lib
 
@@ -183,6 +184,12 @@ class Oracle:
         s["T_n_tau"] = np.zeros_like(s["T"]); s["C_n_tau"] = np.zeros_like(s["C"])
         cs = OrcState(*[_ptr(s[n]) for n, _ in OrcState._fields_])
         return s, cs
+
+    def last_res_l2(self):
+        """L2 norms of the last solve_u / solve_v / solve_p final-stage residuals."""
+        r = np.zeros(3)
+        lib().orc_last_res_l2(self.h, _ptr(r))
+        return r
 
     def pseudo_steps(self, s, cs, n, history=True):
         h = np.zeros((n, 3)) if history else None

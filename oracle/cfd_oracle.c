@@ -449,6 +449,28 @@ double orc_maxabs(const double *a, int ld, int i0, int i1, int j0, int j1)
     return nan ? NAN : m;
 }
 
+static double sumsq_(const double *a, int ld, int i0, int i1, int j0, int j1)
+{
+    double s = 0.0;
+    for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j) {
+            const double v = a[(size_t)i * ld + j];
+            s += v * v;
+        }
+    return s;
+}
+
+/* Root of the sum of squares of the last solve_U / solve_V / solve_P final-stage residual
+ * arrays over the update regions, the entries MAXVAL(ABS(res_*)) reads (equations.f90:298,
+ * 600,662). The reference computes the maximum only; the L2 norm is an addition. */
+void orc_last_res_l2(const orc_ctx *o, double l2[3])
+{
+    const int im = o->imax, jm = o->jmax;
+    l2[0] = sqrt(sumsq_(o->res_u, jm + 1, 3, im - 1, 2, jm - 1));
+    l2[1] = sqrt(sumsq_(o->res_v, jm + 2, 2, im - 1, 3, jm - 1));
+    l2[2] = sqrt(sumsq_(o->res_p, jm + 1, 2, im - 1, 2, jm - 1));
+}
+
 /* CUDA max_reduce (max_reduce.cu:14-41): max|a| over the first n*m linear entries. */
 static double maxabs_linear_(const double *a, long n)
 {

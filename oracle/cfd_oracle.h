@@ -90,6 +90,8 @@ void orc_mesh_ref_x(int n, double L, double P, double Q, double *x);
 
 /* Max-abs over the Fortran update region of an array (i0..i1, j0..j1), row stride ld. */
 double orc_maxabs(const double *a, int ld, int i0, int i1, int j0, int j1);
+/* L2 norms of the last solve_U / solve_V / solve_P residuals over their update regions. */
+void orc_last_res_l2(const orc_ctx *o, double l2[3]);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,7 @@ EXPORTS_CFD = [
     "cfd_params_default", "cfd_mesh_1d", "cfd_geometry_derive", "cfd_ctx_create", "cfd_ctx_destroy",
     "cfd_ctx_get_params", "cfd_ctx_set_params", "cfd_ctx_set_mesh", "cfd_ctx_upload", "cfd_ctx_download",
     "cfd_ctx_device_ptr", "cfd_ctx_sync_levels", "cfd_pseudo_steps", "cfd_run_to_tol", "cfd_physical_step",
-    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_set_kernel", "cfd_ctx_set_pipeline", "cfd_ctx_step_counts",
+    "cfd_pseudo_steps_graph", "cfd_ctx_last_timing", "cfd_ctx_last_residual_l2", "cfd_ctx_set_residual_l2", "cfd_ctx_set_kernel", "cfd_ctx_set_pipeline", "cfd_ctx_step_counts",
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
@@ -105,6 +105,8 @@ def load():
         "cfd_physical_step": (ctypes.c_int, [_vp, dbl, i64, i32, ctypes.POINTER(i64), _dp]),
         "cfd_pseudo_steps_graph": (ctypes.c_int, [_vp, i32, i32, _dp]),
         "cfd_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
+        "cfd_ctx_last_residual_l2": (ctypes.c_int, [_vp, _dp]),
+        "cfd_ctx_set_residual_l2": (ctypes.c_int, [_vp, i32]),
         "cfd_ctx_set_kernel": (ctypes.c_int, [_vp, i32]),
         "cfd_ctx_set_pipeline": (ctypes.c_int, [_vp, i32]),
         "cfd_ctx_step_counts": (ctypes.c_int, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
@@ -306,6 +308,16 @@ class Context:
         v = ctypes.c_double(0)
         check(load().cfd_ctx_last_timing(self.h, ctypes.byref(v)))
         return v.value
+
+    def set_residual_l2(self, on):
+        """Fold the L2 norms of the final-stage increments from the next step on."""
+        check(load().cfd_ctx_set_residual_l2(self.h, 1 if on else 0))
+
+    def last_residual_l2(self):
+        """L2 norms of the last pseudo-step's final-stage increments of u, v, p."""
+        r = np.zeros(3)
+        check(load().cfd_ctx_last_residual_l2(self.h, _ptr(r)))
+        return r
 
     def step_counts(self):
         """(fused, exact): pseudo-steps taken by each path so far."""

@@ -383,6 +383,7 @@ static int build_tables(cfd_batch *b)
             a->pn = (double *)k->buf[n_p];
             a->epsr = k->buf[B_EPS]; a->lpr = k->buf[B_LP];
             a->res_max = b->res + 3 * c;
+            a->res_l2 = NULL;      /* batches report the maxima only */
             a->prec = b->prec;
         }
         nt1 += cfdk_batch_case_tiles(1, b->span, k->P.imax, k->P.jmax, hb1[c], c, NULL);

@@ -82,6 +82,13 @@ __device__ __forceinline__ unsigned long long dbits(double v)
     return (unsigned long long)__double_as_longlong(fabs(v));
 }
 
+// Sum over the wavefront, one atomic add per wave (skipped when the sum is zero).
+__device__ __forceinline__ void fold_sum(double *out, double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(out, v);
+}
+
 __device__ __forceinline__ void fold_max(unsigned long long *out, unsigned long long m)
 {
     for (int o = 32; o > 0; o >>= 1) {
@@ -126,6 +133,7 @@ struct K1Args {
     R *un, *vn;                                            // outputs (interior cells)
     double *pn;
     unsigned long long *res;                               // max |r3| bit patterns: u, v, p
+    double *l2;                                            // sums of r3^2 (u, v, p) or null
     const double *pk;                                      // packed geometry (cfd_devfactors.pack)
     int imax, jmax, nbi, nbj, hb, chan, upwind_all, bouss;
     double dt, dtau, beta, inv_re, inv_reda, cfsd, g, invfr2, ri, t_ref, v_i, u_in;
@@ -186,7 +194,9 @@ __device__ __forceinline__ void fix_v(R *row, int x, int j, int jm)
     }
 }
 
-template <typename R, bool STEADY, bool MUPOLY, int NT, bool RINGS>
+// L2: also fold the sums of r3^2 (single grids; batches report the maxima only, and their
+// register budget stays as it was)
+template <typename R, bool STEADY, bool MUPOLY, int NT, bool RINGS, bool L2>
 __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Smem1<R, NT, RINGS> &sm)
 {
     const int im = a.imax, jm = a.jmax;
@@ -244,6 +254,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     if (!STEADY) { du.fill(R(0)); dv.fill(R(0)); }
     w3.fill(R(0)); x3.fill(R(0));
     unsigned long long mu_ = 0, mv_ = 0, mp_ = 0;
+    double qu_ = 0.0, qv_ = 0.0, qp_ = 0.0;        // sums of r3^2 (FP64 in both precisions)
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
     // next-row registers: the loads for row t+1 are issued before row t is computed
@@ -395,7 +406,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             PUSH                                                                                  \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.un[(size_t)r * su + j] = val;                                                   \
-                const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_;       \
+                const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_; if (L2) qu_ += (double)rs * (double)rs;       \
             }                                                                                     \
         }
 
@@ -461,7 +472,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             PUSH                                                                                  \
             if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
                 a.vn[(size_t)r * sv + j] = val;                                                   \
-                const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_;       \
+                const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_; if (L2) qv_ += (double)rs * (double)rs;       \
             }                                                                                     \
         }
 
@@ -532,7 +543,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 double pi = pp + dd;
                 pi = 0.75 * pp + 0.25 * (pi + dd);
                 a.pn[(size_t)r * su + j] = 1.0 / 3.0 * pp + 2.0 / 3.0 * (pi + dd);
-                const unsigned long long bb = dbits(dd); mp_ = bb > mp_ ? bb : mp_;
+                const unsigned long long bb = dbits(dd); mp_ = bb > mp_ ? bb : mp_; if (L2) qp_ += dd * dd;
             }
         }
     };
@@ -543,18 +554,24 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     fold_max(a.res + 0, mu_);
     fold_max(a.res + 1, mv_);
     fold_max(a.res + 2, mp_);
+    if (L2 && a.l2) {
+        fold_sum(a.l2 + 0, qu_);
+        fold_sum(a.l2 + 1, qv_);
+        fold_sum(a.l2 + 2, qp_);
+    }
 }
 
 // OCC: waves per SIMD the register budget is sized for (1: no spills for FP64; 2: 256
-// registers). The autotuner picks the faster one per grid.
-template <typename R, bool STEADY, bool MUPOLY, int OCC, int NT>
+// registers). The autotuner picks the faster one per grid. L2: the form that also folds the
+// sums of squares (contexts that asked for the L2 norms; it costs the two-wave form spills).
+template <typename R, bool STEADY, bool MUPOLY, int OCC, int NT, bool L2>
 __global__ __launch_bounds__(NT, OCC) void k_uvp(const K1Args<R> a)
 {
     __shared__ Smem1<R, NT, (OCC > 1)> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
     if (L >= G) return;
-    uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1)>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1), L2>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 // Persistent batch: `cases` holds two argument sets per case (even and odd tau parity);
@@ -570,7 +587,7 @@ __global__ __launch_bounds__(NT, OCC) void k_uvp_batch(const K1Args<R> *cases, c
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1)>(a, tl.bi, tl.bj, sm);
+        uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1), false>(a, tl.bi, tl.bj, sm);
         __syncthreads();   // the next tile re-initialises the LDS rings
     }
 }
@@ -599,7 +616,7 @@ struct Smem1S {
 // t+1, so it holds 16 rows; the u ring is read down to row t-6 and holds 8.
 __device__ __forceinline__ int s16(int r) { return (r + 32) & 15; }
 
-template <typename R, bool STEADY, bool MUPOLY, int NT>
+template <typename R, bool STEADY, bool MUPOLY, int NT, bool L2>
 __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int bj, Smem1S<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
@@ -642,6 +659,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
     if (!STEADY) dd.fill(R(0));
     w1.fill(R(0)); w2.fill(R(0)); w3.fill(R(0));
     unsigned long long mu_ = 0, mv_ = 0, mp_ = 0;
+    double qu_ = 0.0, qv_ = 0.0, qp_ = 0.0;        // sums of r3^2 (FP64 in both precisions)
 
     const size_t su = (size_t)(jm + 1), sv = (size_t)(jm + 2);
     R q_t, q_T, q_e, q_L, q_m;
@@ -772,7 +790,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                 WOUT.push(val);                                                                   \
                 if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                       \
                     a.un[(size_t)r * su + j] = val;                                               \
-                    const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_;   \
+                    const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_; if (L2) qu_ += (double)rs * (double)rs;   \
                 }                                                                                 \
             }
             US_STAGE(1, own, sUT[s8(t - 2)], w1, { sU1[s3(t - 2)][PAD + x] = val; })
@@ -792,7 +810,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                 double pi = pp + ddp;
                 pi = 0.75 * pp + 0.25 * (pi + ddp);
                 a.pn[(size_t)r * su + j] = 1.0 / 3.0 * pp + 2.0 / 3.0 * (pi + ddp);
-                const unsigned long long bb = dbits(ddp); mp_ = bb > mp_ ? bb : mp_;
+                const unsigned long long bb = dbits(ddp); mp_ = bb > mp_ ? bb : mp_; if (L2) qp_ += ddp * ddp;
             }
         } else {
             const R ave_j = R(pc[Cc + jc]), have_j = R(0.5 * pc[Cc + jc]);
@@ -855,7 +873,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                 WOUT.push(val);                                                                   \
                 if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                       \
                     a.vn[(size_t)r * sv + j] = val;                                               \
-                    const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_;   \
+                    const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_; if (L2) qv_ += (double)rs * (double)rs;   \
                 }                                                                                 \
             }
             VS_STAGE(1, own, sVT[s16(t - 2)], w1, { sV1[s3(t - 2)][PAD + x] = val; })
@@ -867,16 +885,21 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
     fold_max(a.res + 0, mu_);
     fold_max(a.res + 1, mv_);
     fold_max(a.res + 2, mp_);
+    if (L2 && a.l2) {
+        fold_sum(a.l2 + 0, qu_);
+        fold_sum(a.l2 + 1, qv_);
+        fold_sum(a.l2 + 2, qp_);
+    }
 }
 
-template <typename R, bool STEADY, bool MUPOLY, int NT>
+template <typename R, bool STEADY, bool MUPOLY, int NT, bool L2>
 __global__ __launch_bounds__(2 * NT, 1) void k_uvp_split(const K1Args<R> a)
 {
     __shared__ Smem1S<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    uvp_tile_split<R, STEADY, MUPOLY, NT>(a, L / a.nbj, L % a.nbj, sm);
+    uvp_tile_split<R, STEADY, MUPOLY, NT, L2>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, bool MUPOLY, int NT>
@@ -889,7 +912,7 @@ __global__ __launch_bounds__(2 * NT, 1) void k_uvp_split_batch(const K1Args<R> *
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K1Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        uvp_tile_split<R, STEADY, MUPOLY, NT>(a, tl.bi, tl.bj, sm);
+        uvp_tile_split<R, STEADY, MUPOLY, NT, false>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1416,6 +1439,7 @@ void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params
     k1.p = a->p; k1.T = (const R *)a->T;
     k1.eps = (const R *)a->epsr; k1.lp = (const R *)a->lpr;
     k1.un = (R *)a->um_n; k1.vn = (R *)a->vm_n; k1.pn = a->pn; k1.res = (unsigned long long *)a->res_max;
+    k1.l2 = a->res_l2;
     k1.pk = pack;
     k1.imax = im; k1.jmax = jm;
     k1.chan = P->bc == CFD_BC_CHANNEL; k1.upwind_all = P->scheme == CFD_SCHEME_UPWIND;
@@ -1461,13 +1485,20 @@ int lone_span(int jm)
 }
 
 // occ: 1 or 2 select k_uvp's register budget, 3 the split-chain kernel.
+template <typename R, bool STEADY, bool MUPOLY, int NT, bool L2>
+void launch_k1_l2(const K1Args<R> &k1, int occ, int grid1, hipStream_t s)
+{
+    const dim3 g1(grid1);
+    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY, NT, L2>), g1, dim3(2 * NT), 0, s, k1);
+    else if (occ == 2) hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 2, NT, L2>), g1, dim3(NT), 0, s, k1);
+    else hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 1, NT, L2>), g1, dim3(NT), 0, s, k1);
+}
+
 template <typename R, bool STEADY, bool MUPOLY, int NT>
 void launch_k1_v(const K1Args<R> &k1, int occ, int grid1, hipStream_t s)
 {
-    const dim3 g1(grid1);
-    if (occ == 3) hipLaunchKernelGGL((k_uvp_split<R, STEADY, MUPOLY, NT>), g1, dim3(2 * NT), 0, s, k1);
-    else if (occ == 2) hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 2, NT>), g1, dim3(NT), 0, s, k1);
-    else hipLaunchKernelGGL((k_uvp<R, STEADY, MUPOLY, 1, NT>), g1, dim3(NT), 0, s, k1);
+    if (k1.l2) launch_k1_l2<R, STEADY, MUPOLY, NT, true>(k1, occ, grid1, s);
+    else launch_k1_l2<R, STEADY, MUPOLY, NT, false>(k1, occ, grid1, s);
 }
 
 template <typename R, int NT>

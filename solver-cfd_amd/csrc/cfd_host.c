@@ -161,6 +161,8 @@ struct cfd_ctx {
     uint64_t *d_hist;      /* 3 uint64 per step */
     int64_t hist_cap;
     uint64_t *h_hist;      /* pinned mirror */
+    double *d_l2;          /* sums of r3^2 of the last step (u, v, p) */
+    int l2_on;             /* fold them (cfd_ctx_set_residual_l2) */
     hipEvent_t ev0, ev1;
     hipEvent_t evk[4];     /* per-kernel timing events, created on first use */
     double last_ms_per_step;
@@ -246,6 +248,7 @@ static int ctx_alloc(cfd_ctx *c)
         c->geo32_stale = 1;
     }
     CHK(ensure_hist(c, 64));
+    CHK(dalloc(&c->d_l2, 3));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
     if (c->owns_stream) {
@@ -274,6 +277,7 @@ static void ctx_free(cfd_ctx *c)
     if (c->lp32) hipFree(c->lp32);
     if (c->d_hist) hipFree(c->d_hist);
     if (c->h_hist) hipHostFree(c->h_hist);
+    if (c->d_l2) hipFree(c->d_l2);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     for (int k = 0; k < 4; ++k) if (c->evk[k]) hipEventDestroy(c->evk[k]);
@@ -515,6 +519,7 @@ typedef struct {
     const cfd_devgeo *g;
     const cfd_params *P;
     hipStream_t s;
+    double *l2;            /* sums of squares of the residuals (u, v, p), or NULL */
 } xenv;
 
 static int x_residual(const xenv *e, int region, const double *res, uint64_t *slot)
@@ -525,11 +530,19 @@ static int x_residual(const xenv *e, int region, const double *res, uint64_t *sl
          * called with (imax-1, jmax-2) for u (solve_U.cu:58), (imax-1, jmax) for v
          * (solve_V.cu:52) and (imax, jmax) for p (solve_P.cu:50). */
         long n = region == 0 ? (long)(im - 1) * (jm - 2) : (region == 1 ? (long)(im - 1) * jm : (long)im * jm);
+        if (e->l2) {
+            static const int lo_i[3] = {3, 2, 2}, lo_j[3] = {2, 3, 2};
+            CHK(cfdk_sumsq(res, region == 1 ? jm + 2 : jm + 1, lo_i[region], im - 1, lo_j[region], jm - 1,
+                           e->l2 + region, e->s));
+        }
         return cfdk_maxabs_linear(res, n, slot, e->s);
     }
-    if (region == 0) return cfdk_maxabs(res, jm + 1, 3, im - 1, 2, jm - 1, slot, e->s);
-    if (region == 1) return cfdk_maxabs(res, jm + 2, 2, im - 1, 3, jm - 1, slot, e->s);
-    return cfdk_maxabs(res, jm + 1, 2, im - 1, 2, jm - 1, slot, e->s);
+    /* the update regions of equations.f90:298,600,662; the sums of squares (no counterpart in
+     * the reference) run over the same entries */
+    static const int lo_i[3] = {3, 2, 2}, lo_j[3] = {2, 3, 2};
+    const int ld = region == 1 ? jm + 2 : jm + 1;
+    if (e->l2) CHK(cfdk_sumsq(res, ld, lo_i[region], im - 1, lo_j[region], jm - 1, e->l2 + region, e->s));
+    return cfdk_maxabs(res, ld, lo_i[region], im - 1, lo_j[region], jm - 1, slot, e->s);
 }
 
 /* equations.f90:244-301; T (may be NULL) feeds the mu(T) model only. */
@@ -620,7 +633,7 @@ static void tau_swap(cfd_ctx *c)
 /* One pseudo-step (main.f90:122-142) through the per-operator kernels; residuals into slot[0..2]. */
 static int step_exact(cfd_ctx *c, uint64_t *slot)
 {
-    xenv e = {&c->g, &c->P, c->s};
+    xenv e = {&c->g, &c->P, c->s, c->l2_on ? c->d_l2 : NULL};
     const int st = c->P.steady;
     double **S = c->st, **X = c->scr;
     const double *um = st ? S[S_UMT] : S[S_UM];
@@ -642,7 +655,7 @@ static int step_exact(cfd_ctx *c, uint64_t *slot)
             CHK(cfdk_copy(S[S_VMN] + k, S[S_VMT] + k, 2, c->s));
         }
     }
-    CHK(cfdk_zero_u64(slot, 3, c->s));
+    CHK(cfdk_zero_slots(slot, 3, e.l2, c->s));
     CHK(x_solve_u(&e, um, S[S_UMT], S[S_VMT], S[S_UMN], p_uv, T, X[X_UI], X[X_RU], X[X_RESU], slot + 0));
     CHK(x_solve_v(&e, vm, S[S_UMT], S[S_VMT], S[S_VMN], p_uv, T, X[X_VI], X[X_RV], X[X_RESV], slot + 1));
     CHK(x_solve_p(&e, S[S_P], S[S_UMN], S[S_VMN], S[S_PN], X[X_RP], X[X_PI], X[X_RESP], slot + 2));
@@ -692,6 +705,7 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     a.T_tau = F[S_TT]; a.C_tau = F[S_CT];
     a.um_n = F[S_UMN]; a.vm_n = F[S_VMN]; a.pn = S[S_PN]; a.T_n = F[S_TN]; a.C_n = F[S_CN];
     a.res_max = slot;
+    a.res_l2 = c->l2_on ? c->d_l2 : NULL;
     a.hb1 = c->tiles.rows_uvp; a.hb2 = c->tiles.rows_tc;
     a.occ1 = c->tiles.waves_uvp; a.occ2 = c->tiles.waves_tc;
     a.lone = 1;
@@ -706,7 +720,7 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     } else {
         CHK(pipe_join(c));
     }
-    CHK(cfdk_zero_u64(slot, 3, c->s));
+    CHK(cfdk_zero_slots(slot, 3, a.res_l2, c->s));
     CHK(cfdk_fused_step(&c->g, &c->fac, &c->P, &a, c->kernel, c->s));
     if (two) {
         const int q = (int)(c->pipe_n & 1);
@@ -1214,6 +1228,28 @@ int cfd_pseudo_steps_timed(cfd_ctx *c, int32_t nsteps, double res[3], double ms[
     for (int q = 0; q < 4; ++q) ms[q] = nf ? acc[q] / nf : 0.0;
     if (res) for (int q = 0; q < 3; ++q) res[q] = u64_to_res(c->h_hist[q]);
     if (!nf) return fail(CFD_ERR_UNSUPPORTED, "no step took the fused path");
+    return CFD_OK;
+}
+
+int cfd_ctx_set_residual_l2(cfd_ctx *c, int32_t on)
+{
+    if (!c) return fail(CFD_ERR_STATE, "no context");
+    CHK(pipe_join(c));
+    HIPCHK(hipMemsetAsync(c->d_l2, 0, 3 * sizeof(double), c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->l2_on = on ? 1 : 0;
+    return CFD_OK;
+}
+
+int cfd_ctx_last_residual_l2(cfd_ctx *c, double l2[3])
+{
+    if (!c || !l2) return fail(CFD_ERR_ARG, "NULL argument");
+    if (!c->l2_on) return fail(CFD_ERR_STATE, "the L2 norms are not folded: enable them with cfd_ctx_set_residual_l2");
+    CHK(pipe_join(c));
+    double h[3];
+    HIPCHK(hipMemcpyAsync(h, c->d_l2, sizeof h, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int q = 0; q < 3; ++q) l2[q] = sqrt(h[q]);
     return CFD_OK;
 }
 

@@ -45,6 +45,12 @@ int cfdk_bc_s(const cfd_devgeo *g, const cfd_params *P, int kind, double *a, hip
 /* max |a(i,j)| over i0..i1, j0..j1 (row stride ld) folded into *out (uint64 bit pattern of a
  * non-negative double; NaN propagates as the largest pattern). *out must be cleared first. */
 int cfdk_zero_u64(uint64_t *out, int n, hipStream_t s);
+/* One launch that clears n residual slots and, when l2 is not NULL, the 3 sum-of-squares
+ * accumulators beside them. */
+int cfdk_zero_slots(uint64_t *slot, int n, double *l2, hipStream_t s);
+/* sum of a(i,j)^2 over i0..i1, j0..j1 (row stride ld) added to *out (atomically, in no fixed
+ * order: the last bits depend on the schedule). */
+int cfdk_sumsq(const double *a, int ld, int i0, int i1, int j0, int j1, double *out, hipStream_t s);
 int cfdk_maxabs(const double *a, int ld, int i0, int i1, int j0, int j1, uint64_t *out, hipStream_t s);
 int cfdk_maxabs_linear(const double *a, long n, uint64_t *out, hipStream_t s);
 int cfdk_copy(double *dst, const double *src, long n, hipStream_t s);
@@ -62,8 +68,10 @@ typedef struct cfd_fused_args {
     void *um_n, *vm_n, *T_n, *C_n;
     double *pn;
     const void *epsr, *lpr;        /* porosity fields in the field precision */
-    /* residual accumulators: res_max[0..2] (uint64 bit patterns of max|r3| for u, v, p) */
+    /* residual accumulators: res_max[0..2] (uint64 bit patterns of max|r3| for u, v, p) and,
+     * when res_l2 is not NULL, the sums of r3^2 over the same entries (3 doubles) */
     uint64_t *res_max;
+    double *res_l2;
     int32_t prec;
     /* row-block heights of K1 / K2 (0: built-in heuristic) and the register-budget variants
      * (occ1 in {1, 2} waves per SIMD or 3 = split u/v chains, occ2 in {1, 4} waves per SIMD or

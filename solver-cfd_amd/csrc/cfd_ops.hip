@@ -212,6 +212,31 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
     if (t < n) p[t] = 0ull;
 }
 
+__global__ void k_zero_slots(unsigned long long *p, int n, double *l2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0ull;
+    if (l2 && t < 3) l2[t] = 0.0;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void k_sumsq(const double *a, int ld, int i0, int j0, int ni, int nj, double *out)
+{
+    double s = 0.0;
+    const long n = (long)ni * nj;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const double v = a[(size_t)(i0 + t / nj) * ld + (j0 + t % nj)];
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(out, s);
+}
+
 __global__ void k_copy(double *dst, const double *src, long n)
 {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
@@ -350,6 +375,21 @@ int cfdk_maxabs(const double *a, int ld, int i0, int i1, int j0, int j1, uint64_
     if (ni <= 0 || nj <= 0) return CFD_OK;
     hipLaunchKernelGGL(k_maxabs, dim3(reduce_blocks((long)ni * nj)), dim3(256), 0, s, a, ld, i0, j0, ni, nj,
                        (unsigned long long *)out);
+    return launched();
+}
+
+int cfdk_zero_slots(uint64_t *slot, int n, double *l2, hipStream_t s)
+{
+    const int m = n > 3 ? n : 3;
+    hipLaunchKernelGGL(k_zero_slots, dim3(cdiv(m, 256)), dim3(256), 0, s, (unsigned long long *)slot, n, l2);
+    return launched();
+}
+
+int cfdk_sumsq(const double *a, int ld, int i0, int i1, int j0, int j1, double *out, hipStream_t s)
+{
+    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
+    if (ni <= 0 || nj <= 0) return CFD_OK;
+    hipLaunchKernelGGL(k_sumsq, dim3(reduce_blocks((long)ni * nj)), dim3(256), 0, s, a, ld, i0, j0, ni, nj, out);
     return launched();
 }
 

@@ -385,9 +385,11 @@ __global__ __launch_bounds__(WG_THREADS, WG_WAVES_PER_SIMD) void k_conv3x3_wgrad
     }
 }
 
-// dw[i] += sum over g < G of part[g stride + i], i < count; blockIdx.y takes a slice of the
-// groups (atomic adds when there are several slices)
-__global__ void k_wsum(const float *__restrict__ part, int G, int gper, long count, long stride, float *__restrict__ dw)
+// dw[i] += sum over g < G of part[g stride + i], i < count, in a fixed order (the training
+// is reproducible run to run). blockIdx.y takes a slice of the groups; with several slices
+// each one leaves its sum in the first row of its slice (only this thread reads column i
+// of those rows) and k_wsum_final adds the slice sums in order.
+__global__ void k_wsum(float *__restrict__ part, int G, int gper, long count, long stride, float *__restrict__ dw)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -395,7 +397,17 @@ __global__ void k_wsum(const float *__restrict__ part, int G, int gper, long cou
     float s = 0.f;
     for (int g = g0; g < g1; ++g) s += part[(size_t)g * stride + i];
     if (gridDim.y == 1) dw[i] += s;
-    else atomicAdd(&dw[i], s);
+    else part[(size_t)g0 * stride + i] = s;
+}
+
+__global__ void k_wsum_final(const float *__restrict__ part, int nslices, int gper, long count, long stride,
+                             float *__restrict__ dw)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.f;
+    for (int q = 0; q < nslices; ++q) s += part[(size_t)q * gper * stride + i];
+    dw[i] += s;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -786,8 +798,12 @@ int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W
     if (r != CFD_OK) return r;
     if (db && !joint) {
         const int gyb = G < 64 ? G : 64, gperb = cdiv(G, gyb);
-        hipLaunchKernelGGL(k_wsum, dim3(cdiv(Cout, 256), cdiv(G, gperb)), dim3(256), 0, 0, (const float *)part_b, G,
-                           gperb, (long)Cout, (long)Cout, db);
+        const int gyb2 = cdiv(G, gperb);
+        hipLaunchKernelGGL(k_wsum, dim3(cdiv(Cout, 256), gyb2), dim3(256), 0, 0, part_b, G, gperb, (long)Cout,
+                           (long)Cout, db);
+        if (gyb2 > 1)
+            hipLaunchKernelGGL(k_wsum_final, dim3(cdiv(Cout, 256)), dim3(256), 0, 0, (const float *)part_b, gyb2,
+                               gperb, (long)Cout, (long)Cout, db);
         r = launched();
         if (r != CFD_OK) return r;
     }
@@ -799,7 +815,9 @@ int wgrad(const uint16_t *x, const uint16_t *dy, int32_t B, int32_t H, int32_t W
     if (gy < 1) gy = 1;
     const int gper = cdiv(G, gy);
     gy = cdiv(G, gper);
-    hipLaunchKernelGGL(k_wsum, dim3(nbx, gy), dim3(256), 0, 0, (const float *)part, G, gper, count, pstride, dw);
+    hipLaunchKernelGGL(k_wsum, dim3(nbx, gy), dim3(256), 0, 0, part, G, gper, count, pstride, dw);
+    if (gy > 1)
+        hipLaunchKernelGGL(k_wsum_final, dim3(nbx), dim3(256), 0, 0, (const float *)part, gy, gper, count, pstride, dw);
     return launched();
 }
 

@@ -327,3 +327,37 @@ def test_pipeline_matches_single_stream_bitwise(waves):
     assert np.array_equal(h0, h1) and np.array_equal(r0, r1) and np.array_equal(g0, g1)
     for k in FIELDS:
         assert np.array_equal(f0[k], f1[k]), (k, max_rel(f1[k], f0[k]))
+
+
+@pytest.mark.parametrize("steady,prec,waves", [(0, PREC_FP64, 1), (0, PREC_FP64, 2), (0, PREC_FP64, 3),
+                                                (1, PREC_FP64, 0), (0, PREC_FP32, 0)])
+def test_residual_l2_matches_oracle(steady, prec, waves):
+    """The L2 norms of the final-stage increments, folded in the same pass as the maxima
+    (cfd_ctx_last_residual_l2), against the oracle's after the exact first step and after
+    40 fused steps (FP64 1e-11 relative: the increments differ by rounding only and the sums
+    run in another order; FP32 storage 1e-4)."""
+    say("case: residual L2, steady", steady, "precision", prec, "u/v/p waves", waves)
+    prm, x, y, eps, lp, st, out, meta = load_golden("quick_porous48")
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "steady")}
+    prm = make_params(prm.imax, prm.jmax, steady=steady, **d)
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    ctx = L.Context(prm, x, y, eps, lp, precision=prec)
+    ctx.set_state(st)
+    with pytest.raises(L.CfdError):
+        ctx.last_residual_l2()          # off by default
+    ctx.set_residual_l2(True)
+    if waves:
+        ctx.set_tiles(0, 0, waves, 1)     # the one-wave, two-wave and split-chain u/v/p kernels
+    bar = 1e-11 if prec == PREC_FP64 else 1e-4
+    for n in (1, 40):
+        o.pseudo_steps(s, cs, n, history=False)
+        ref = o.last_res_l2()
+        ctx.pseudo_steps(n)
+        got = ctx.last_residual_l2()
+        rel = np.abs(got - ref) / ref
+        say("after", n, "steps: gpu", got, "oracle", ref, "rel", rel)
+        assert np.all(rel <= bar), (n, rel)
+    nf, ne = ctx.step_counts()
+    assert ne == 1 and nf == 40
+    ctx.close()

@@ -454,3 +454,33 @@ def test_steady_mode_is_dual_time_with_tracking_level():
         o0.pseudo_steps(s0, c0, 1)
     for k in ("um_tau", "vm_tau", "p", "T_tau", "C_tau"):
         assert np.array_equal(s1[k], s0[k]), k
+
+
+def test_residual_l2_closed_forms():
+    """The oracle's L2 residual norms (an addition: the reference reports max|r3| only). With
+    dt = 0 every RK stage's increment is (phi - phi_tau) dtau, so the u and v norms are
+    dtau ||phi - phi_tau|| over the update regions (u: i=3..imax-1, j=2..jmax-1; v: i=2..imax-1,
+    j=3..jmax-1); the p norm is ||pn - p|| over i, j = 2..max-1 (solve_P's three stages give
+    pn = p + dtau beta RP). A wrong region, a missing square root or a dropped dtau fails."""
+    prm, x, y, eps, lp = small_case()
+    im, jm = prm.imax, prm.jmax
+    d = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "mu_coef", "dt")}
+    o = O.Oracle(make_params(im, jm, dt=0.0, **d), x, y, eps, lp)
+    st = random_state(im, jm, seed=43)
+    phys = random_state(im, jm, seed=44)
+    s, cs = o.make_state(dict(um=phys["um"], vm=phys["vm"], p=st["p"], T=st["T"], C=st["C"],
+                              um_tau=st["um"], vm_tau=st["vm"]))
+    p0 = s["p"].copy()
+    h = o.pseudo_steps(s, cs, 1)
+    l2 = o.last_res_l2()
+    dtau = prm.dtau
+    du = (phys["um"] - st["um"])[3:im, 2:jm]
+    dv = (phys["vm"] - st["vm"])[2:im, 3:jm]
+    assert l2[0] == pytest.approx(dtau * np.sqrt(np.sum(du * du)), rel=1e-13)
+    assert l2[1] == pytest.approx(dtau * np.sqrt(np.sum(dv * dv)), rel=1e-13)
+    # after the step p holds pn (tau update)
+    dp = (s["p"] - p0)[2:im, 2:jm]
+    assert l2[2] == pytest.approx(np.sqrt(np.sum(dp * dp)), rel=1e-12)
+    # and the maxima of the same entries
+    assert h[0, 0] == pytest.approx(dtau * np.max(np.abs(du)), rel=1e-15)
+    assert h[0, 1] == pytest.approx(dtau * np.max(np.abs(dv)), rel=1e-15)
