@@ -73,6 +73,9 @@ extern "C" {
 /* Viscosity model in the momentum diffusion coefficients. */
 #define CFD_MU_CONST         0   /* eps/Re */
 #define CFD_MU_POLY4         1   /* eps*mu(T)/Re, mu(T) = sum_k mu_coef[k] T^k, T of the physical level */
+#define CFD_K_CONST          0   /* energy diffusivity 1/Pe */
+#define CFD_K_POLY4          1   /* k(T)/Pe, k(T) = sum_k k_coef[k] T^k (conductivity over its reference
+                                    value), T of the physical level */
 
 /* Storage precision of the fused pseudo-step. */
 #define CFD_PREC_FP64        0   /* FP64 fields and arithmetic (the drop-in) */
@@ -122,6 +125,9 @@ typedef struct cfd_params {
     double slot_lo, slot_hi;    /* channel BC set: y-range of the species slot on the left wall */
     double slot_value;          /* channel BC set: species value inside the slot (0 outside) */
     double mu_coef[5];          /* CFD_MU_POLY4 coefficients c0..c4 */
+    double k_coef[5];           /* CFD_K_POLY4 coefficients c0..c4 */
+    int32_t k_model;            /* CFD_K_* */
+    int32_t reserved;           /* 0 */
 } cfd_params;
 
 /* Reference defaults (OpenMp_Multicore/comum.f90:29-115, main.f90:105-106, input/iterations.dat):
@@ -305,7 +311,8 @@ int cfd_write_vtk(const char *path, const char *title, int32_t imax, int32_t jma
  * exit test of the pseudo-time loop runs on the device, and a run replays a captured
  * hipGraph of `steps_per_graph` pseudo-steps until every case has stopped. Fields are
  * stored in the batch precision (CFD_PREC_FP64, or CFD_PREC_FP32 with FP64 pressure and
- * residual maxima). Every case must share `steady` and `mu_model`; b_art must be 0 and
+ * residual maxima). Every case must share `steady` and `mu_model`, k_model must be
+ * CFD_K_CONST; b_art must be 0 and
  * compat CFD_COMPAT_FORTRAN (batches run the fused kernels only). Arrays use the C layout
  * of one case (see above); uploads and downloads are host arrays of doubles. */
 typedef struct cfd_batch cfd_batch;

@@ -53,6 +53,7 @@ def lib():
         L.orc_get_geometry.argtypes = [vp] + [_dp] * 6
         for n in ("orc_resu", "orc_resv"):
             getattr(L, n).argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_resz_kt.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
         for n in ("orc_resz", "orc_resc"):
             getattr(L, n).argtypes = [vp, _dp, _dp, _dp, _dp]
         for n in ("orc_upwind_ui", "orc_upwind_uj", "orc_upwind_vi", "orc_upwind_vj"):
@@ -126,6 +127,10 @@ class Oracle:
 
     def resz(self, um, vm, z):
         r = np.zeros_like(z); lib().orc_resz(self.h, _ptr(um), _ptr(vm), _ptr(z), _ptr(r)); return r
+
+    def resz_kt(self, um, vm, z, T):
+        """RESZ with the k(T) diffusivity at the temperature T (k_model K_POLY4)."""
+        r = np.zeros_like(z); lib().orc_resz_kt(self.h, _ptr(um), _ptr(vm), _ptr(z), _ptr(T), _ptr(r)); return r
 
     def resc(self, um, vm, c):
         r = np.zeros_like(c); lib().orc_resc(self.h, _ptr(um), _ptr(vm), _ptr(c), _ptr(r)); return r

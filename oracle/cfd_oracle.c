@@ -25,6 +25,8 @@ struct orc_ctx {
     double *eps, *lp;
     /* scratch (module arrays RU/RV/RZ/RC/Pi/RP/res_* of comum.f90:121-146) */
     double *ru, *rv, *rz, *rc, *ui, *vi, *zi, *ci, *pi, *rp, *res_u, *res_v, *res_p;
+    /* physical-level temperature of the k(T) diffusivity while solve_Z runs (else NULL) */
+    const double *kT;
 };
 
 #define IU(i, j) ((size_t)(i) * (size_t)(o->jmax + 1) + (size_t)(j))   /* p-like and u */
@@ -307,10 +309,18 @@ void orc_resv(orc_ctx *o, const double *um, const double *vm, const double *p, c
 void orc_resz(orc_ctx *o, const double *um, const double *vm, const double *z, double *rz)
 {
     const int im = o->imax, jm = o->jmax;
-    const double k = 1.0 / o->P.pe;
+    const double k0 = 1.0 / o->P.pe;
+    const double *kT = o->P.k_model == CFD_K_POLY4 ? o->kT : NULL;
+    const double *kc = o->P.k_coef;
     #pragma omp parallel for schedule(static)
     for (int i = 2; i <= im - 1; ++i)
         for (int j = 2; j <= jm - 1; ++j) {
+            /* k(T)/Pe of the cell (the reference: 1/Pe, equations.f90:737) */
+            double k = k0;
+            if (kT) {
+                const double t = kT[IU(i, j)];
+                k = ((((kc[4] * t + kc[3]) * t + kc[2]) * t + kc[1]) * t + kc[0]) / o->P.pe;
+            }
             const double dzudx = 0.5 * (z[IU(i + 1, j)] + z[IU(i, j)]) * um[IU(i + 1, j)] * o->aue[j]
                                - 0.5 * (z[IU(i - 1, j)] + z[IU(i, j)]) * um[IU(i, j)] * o->auw[j];
             const double dzvdy = 0.5 * (z[IU(i, j + 1)] + z[IU(i, j)]) * vm[IV(i, j + 1)] * o->avn[i]
@@ -629,7 +639,17 @@ static void solve_scalar_(orc_ctx *o, res_fn_ resf, bc_fn_ bcf, double *r, doubl
 void orc_solve_z(orc_ctx *o, const double *um_n, const double *vm_n, const double *z,
                  double *z_n_tau, const double *z_tau)
 {
+    /* k(T) takes the physical-level temperature z, fixed through the three stages */
+    o->kT = z;
     solve_scalar_(o, orc_resz, orc_bcz, o->rz, o->zi, um_n, vm_n, z, z_n_tau, z_tau);
+    o->kT = NULL;
+}
+
+void orc_resz_kt(orc_ctx *o, const double *um, const double *vm, const double *z, const double *T, double *rz)
+{
+    o->kT = T;
+    orc_resz(o, um, vm, z, rz);
+    o->kT = NULL;
 }
 
 void orc_solve_c(orc_ctx *o, const double *um_n, const double *vm_n, const double *c,

@@ -43,6 +43,8 @@ void orc_get_geometry(const orc_ctx *o, double *xm, double *ym, double *aun, dou
 void orc_resu(orc_ctx *o, const double *um, const double *vm, const double *p, const double *T, double *ru);
 void orc_resv(orc_ctx *o, const double *um, const double *vm, const double *p, const double *T, double *rv);
 void orc_resz(orc_ctx *o, const double *um, const double *vm, const double *z, double *rz);
+/* RESZ with the k(T) diffusivity (k_model CFD_K_POLY4) evaluated at the temperature T. */
+void orc_resz_kt(orc_ctx *o, const double *um, const double *vm, const double *z, const double *T, double *rz);
 void orc_resc(orc_ctx *o, const double *um, const double *vm, const double *c, double *rc);
 /* Upwind rows/columns (equations.f90:120-240, 423-542). */
 void orc_upwind_ui(orc_ctx *o, const double *um, const double *vm, const double *p, const double *T, double *ru, int j);

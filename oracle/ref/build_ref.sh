@@ -30,7 +30,7 @@ FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
 mkdir -p "$OUT"
 W=$(mktemp -d /tmp/cfdref.XXXXXX)
 trap 'rm -rf "$W"' EXIT
-for f in comum.f90 boundary.f90 equations.f90 nonsymetric_mesh.f90; do
+for f in comum.f90 boundary.f90 equations.f90 nonsymetric_mesh.f90 properties_CH4.f90; do
     cp "$REF/$f" "$W/$f"
 done
 
@@ -57,6 +57,6 @@ sed -i -E '/^SUBROUTINE bcC/,/^END SUBROUTINE bcC/ s/\bCn\b/Cn_d/g' "$W/boundary
 
 cp "$HERE/harness.f90" "$W/harness.f90"
 ( cd "$W" && "$FC" -O2 -ffp-contract=off -fopenmp ${REF_FFLAGS:-} -o "$OUT/ref_$TAG" \
-      comum.f90 boundary.f90 equations.f90 nonsymetric_mesh.f90 harness.f90 )
+      comum.f90 boundary.f90 equations.f90 nonsymetric_mesh.f90 properties_CH4.f90 harness.f90 )
 grep -E "PARAMETER :: (imax|jmax|Re|Pe|Sc|Darcy_number|Cf|g|Fr|v_i|Px_grid|Lhori|y_up|y_down) " "$W/comum.f90" > "$OUT/ref_$TAG.params"
 echo "built $OUT/ref_$TAG"

@@ -7,7 +7,8 @@
 ! tests/golden/make_golden.py, to produce the golden vectors under tests/golden/.
 !
 ! Input  (stream, native endian): harness_in.bin
-!   int32  mode            1 = operators, 2 = pseudo-steps, 3 = physical steps, 4 = mesh
+!   int32  mode            1 = operators, 2 = pseudo-steps, 3 = physical steps, 4 = mesh,
+!                          5 = mixture properties (properties_CH4.f90) at nsteps temperatures
 !   int32  nsteps, nphys, itc_max, steady
 !   real64 dtau, dt, beta, b_art, tinf, eps_conv
 !   geometry (modes 1-3): x(imax), y(jmax), xm(imax+1), ym(jmax+1), areau_n(2:imax),
@@ -39,6 +40,20 @@ program ref_harness
     if (mode == 4) then
         call mesh
         write(22) x, y, xm, ym, areau_n, areau_e, areav_n, areav_e, epsilon1, liga_poros
+        close(22)
+        stop
+    end if
+
+    if (mode == 5) then
+        ! Too(k), k = 1..nsteps, follow the header; per temperature the reference's mixture
+        ! properties: k_tot, rho_tot, cp_tot, nu_tot, alpha_tot, Pr
+        YF_b = 1.d0
+        do k = 1, nsteps
+            read(21) Too
+            call properties
+            write(22) k_tot, rho_tot, cp_tot, nu_tot, alpha_tot, Pr
+        end do
+        close(21)
         close(22)
         stop
     end if

@@ -61,6 +61,14 @@ def run(binary, header_i, header_d, payload, threads=1):
         return open(os.path.join(d, "harness_out.bin"), "rb").read()
 
 
+def properties(binary, temps):
+    """The reference's mixture properties (properties_CH4.f90:8-58) at each ambient
+    temperature Too in temps: an array [n, 6] of k_tot, rho_tot, cp_tot, nu_tot, alpha_tot, Pr."""
+    temps = np.asarray(temps, dtype=np.float64)
+    buf = run(binary, [5, len(temps), 0, 0, 0], [0.05, 0.005, 1.0, 0.0, 1.0, 0.0], [temps])
+    return np.frombuffer(buf, dtype=np.float64).reshape(len(temps), 6)
+
+
 def time_pseudo_steps(binary, header_d, payload, n_short, n_long, threads):
     """Seconds per pseudo-step of the harness's mode-2 loop (the reference's solve_U, solve_V,
     solve_P, solve_Z, solve_C per step): the difference of two runs of n_long and n_short

@@ -227,6 +227,30 @@ def mu_poly_coefficients(seed=3):
     return tuple(base * (1.0 + uniform(seed, 5, -0.02, 0.02)))
 
 
+# Thermal conductivity fits of the reference's mixture properties (properties_CH4.f90:13-14,
+# 34, 44): air (cubic) and methane (quadratic) in W/(m K) against T in K, mixed with the
+# mole fractions X_met = 0.25, X_air = 0.75 (properties_CH4.f90:6). Kept as the reference
+# states them: they are the data of that fit.
+_K_AIR = (0.0009128206, 9.04946015844389e-5, -0.000000029, 4.63985599705762e-12)
+_K_MET = (0.0037188796, 0.000064197, 1.30126895841081e-7)
+_X_MET, _X_AIR = 0.25, 0.75
+
+
+def conductivity_poly_ch4_air(t0=300.0, dT=300.0):
+    """Coefficients c0..c4 of k(T)/k(t0) for the nondimensional temperature T, where the
+    dimensional one is t0 + dT * T, from the reference's mixture conductivity
+    k_tot = X_met k_met + X_air k_air (properties_CH4.f90, evaluated there once at the
+    ambient temperature Too). For cfd_params.k_coef with k_model K_POLY4."""
+    d = [_X_AIR * _K_AIR[n] + (_X_MET * _K_MET[n] if n < 3 else 0.0) for n in range(4)]
+    # k_tot(t0 + dT T) = sum_n d_n (t0 + dT T)^n, expanded in powers of T
+    c = [0.0] * 5
+    for n, dn in enumerate(d):
+        for m in range(n + 1):
+            c[m] += dn * math.comb(n, m) * t0 ** (n - m) * dT ** m
+    k0 = c[0]
+    return tuple(v / k0 for v in c)
+
+
 def case_full_physics(imax=2048, jmax=2048):
     """Full physics at scale: tanh-clustered mesh (Q=2) toward the walls, QUICK, porous
     middle-third block (eps~U[0.4,0.9]), Boussinesq Ri=2, species slot Z=1 of width 0.1 on

@@ -12,6 +12,7 @@ BC_REFERENCE, BC_CHANNEL = 0, 1
 BUOY_REFERENCE, BUOY_BOUSSINESQ = 0, 1
 COMPAT_FORTRAN, COMPAT_CUDA = 0, 1
 MU_CONST, MU_POLY4 = 0, 1
+K_CONST, K_POLY4 = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 
 # field ids (include/cfd.h)
@@ -37,6 +38,8 @@ class CfdParams(ctypes.Structure):
         ("u_in", ctypes.c_double), ("t_bottom", ctypes.c_double), ("t_top", ctypes.c_double),
         ("slot_lo", ctypes.c_double), ("slot_hi", ctypes.c_double), ("slot_value", ctypes.c_double),
         ("mu_coef", ctypes.c_double * 5),
+        ("k_coef", ctypes.c_double * 5),
+        ("k_model", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -47,6 +50,7 @@ DEFAULTS = dict(
     ri=0.0, t_ref=0.0, v_i=0.5, tinf=1.0, dtau=5.0e-2, dt=0.5e-2, beta=1300.0, b_art=0.0,
     u_in=1.0, t_bottom=1.0, t_top=0.0, slot_lo=0.0, slot_hi=0.0, slot_value=1.0,
     mu_coef=(1.0, 0.0, 0.0, 0.0, 0.0),
+    k_coef=(1.0, 0.0, 0.0, 0.0, 0.0), k_model=K_CONST, reserved=0,
 )
 
 
@@ -60,9 +64,9 @@ def make_params(imax, jmax, **kw):
     p = CfdParams()
     p.imax, p.jmax = int(imax), int(jmax)
     for k, v in d.items():
-        if k == "mu_coef":
+        if k in ("mu_coef", "k_coef"):
             for n in range(5):
-                p.mu_coef[n] = float(v[n])
+                getattr(p, k)[n] = float(v[n])
         elif isinstance(getattr(p, k), int):
             setattr(p, k, int(v))
         else:
@@ -74,5 +78,5 @@ def params_to_dict(p):
     out = {}
     for name, _ in CfdParams._fields_:
         v = getattr(p, name)
-        out[name] = list(v) if name == "mu_coef" else v
+        out[name] = list(v) if name in ("mu_coef", "k_coef") else v
     return out

@@ -83,6 +83,8 @@ int cfd_batch_create(int32_t n, const cfd_params *params, int32_t precision, cfd
         if (P->imax < 8 || P->jmax < 8) return cfd__fail(CFD_ERR_ARG, "case %d: grid %dx%d too small", c, P->imax, P->jmax);
         if (P->b_art != 0.0 || P->compat != CFD_COMPAT_FORTRAN)
             return cfd__fail(CFD_ERR_UNSUPPORTED, "case %d: batches run the fused path (b_art = 0, Fortran semantics)", c);
+        if (P->k_model != CFD_K_CONST)
+            return cfd__fail(CFD_ERR_UNSUPPORTED, "case %d: batches use the constant energy diffusivity 1/Pe", c);
         if (P->steady != params[0].steady || P->mu_model != params[0].mu_model)
             return cfd__fail(CFD_ERR_ARG, "case %d: steady and mu_model must be the same for every case", c);
         if (!(P->re > 0 && P->darcy > 0 && P->pe > 0 && P->sc > 0 && P->dtau > 0 && P->dt > 0))

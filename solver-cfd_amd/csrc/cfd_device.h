@@ -144,6 +144,15 @@ __device__ __forceinline__ double resv_lit(const Geo &g, const cfd_params &P, co
          - q_art + buoy - e * drag * g.lp[iu(g, i, j)];
 }
 
+// k(T)/Pe of the energy equation's diffusion (k_model CFD_K_POLY4): the conductivity over its
+// reference value as a polynomial in the temperature (no counterpart in the reference's
+// RESZ, whose diffusivity is the constant 1/Pe, equations.f90:737)
+__device__ __forceinline__ double cond_lit(const cfd_params &P, double t)
+{
+    const double *c = P.k_coef;
+    return ((((c[4] * t + c[3]) * t + c[2]) * t + c[1]) * t + c[0]) / P.pe;
+}
+
 // equations.f90:735-762 (k = 1/Pe, conv_in_porous = false) and 836-864 (k = 1/Re/Sc, true)
 __device__ __forceinline__ double ressc_lit(const Geo &g, double k, bool species, const double *um, const double *vm,
                                             const double *z, int i, int j)

@@ -932,7 +932,16 @@ struct K2Args {
     const double *pk;
     int imax, jmax, nbi, nbj, hb, chan;
     double dt, dtau, kt, kc, tinf, t_bottom, t_top, v_i, u_in, slot_lo, slot_hi, slot_value;
+    double kcoef[5];             // k(T) polynomial (k_model CFD_K_POLY4): kappa = kt * k(T)
 };
+
+// k(T) of the energy diffusivity at the physical-level temperature t (the conductivity over
+// its reference value; the reference's RESZ has the constant 1/Pe)
+template <typename R>
+__device__ __forceinline__ R kpoly(const double *c, R t)
+{
+    return (((R(c[4]) * t + R(c[3])) * t + R(c[2])) * t + R(c[1])) * t + R(c[0]);
+}
 
 template <typename R, int NT>
 struct Smem2 {
@@ -988,7 +997,9 @@ __device__ __forceinline__ void nb_c(R &zs, R &zn, R zp, int j, int jm, bool cha
     zs = (j - 1 == 1) ? s1 : zs;
 }
 
-template <typename R, bool STEADY, int NT>
+// KPOLY: the T chain's diffusivity is kt * k(T) per cell, T of the physical level (the
+// iterate in steady mode), fixed through the three stages
+template <typename R, bool STEADY, int NT, bool KPOLY>
 __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Smem2<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
@@ -1018,9 +1029,10 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
     const double yj = a.y[min(max(j, 1), jm)];
     const R cslot = chan ? ((yj >= a.slot_lo && yj <= a.slot_hi) ? R(a.slot_value) : R(0)) : R(0);
 
-    Win<4, R> tt, ct, uw, vw, vnw, id, oml, dT, dC;
+    Win<4, R> tt, ct, uw, vw, vnw, id, oml, dT, dC, kw;
     Win<3, R> z1, z2, c1, c2, z3, c3;
     tt.fill(R(0)); ct.fill(R(0)); uw.fill(R(0)); vw.fill(R(0)); vnw.fill(R(0)); id.fill(R(1)); oml.fill(R(1));
+    if (KPOLY) kw.fill(kt);
     if (!STEADY) { dT.fill(R(0)); dC.fill(R(0)); }
     z1.fill(R(0)); z2.fill(R(0)); c1.fill(R(0)); c2.fill(R(0)); z3.fill(R(0)); c3.fill(R(0));
 
@@ -1086,6 +1098,7 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
             id.push(Lp != R(0) ? R(1) / (Lp * (e - R(1)) + R(1)) : R(1));
             oml.push(R(1) - Lp);
             if (!STEADY) { dT.push(Tp - tnew); dC.push(Cp - cnew); }
+            if (KPOLY) kw.push(kt * kpoly(a.kcoef, STEADY ? tnew : Tp));
         }
 
         // stage K of a scalar at row r = t - K; newest own row of WP is t-K+1
@@ -1136,11 +1149,11 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
         }
 
         R *const noout = nullptr;
-        SCAL_STAGE(1, tt, sTT[s4(t - 1)], false, z1, { sZ1[s2(t - 1)][PAD + x] = val; }, noout, kt, dT, tt)
+        SCAL_STAGE(1, tt, sTT[s4(t - 1)], false, z1, { sZ1[s2(t - 1)][PAD + x] = val; }, noout, (KPOLY ? kw[bq] : kt), dT, tt)
         SCAL_STAGE(1, ct, sCT[s4(t - 1)], true, c1, { sC1[s2(t - 1)][PAD + x] = val; }, noout, kc, dC, ct)
-        SCAL_STAGE(2, z1, sZ1[s2(t - 2)], false, z2, { sZ2[s2(t - 2)][PAD + x] = val; }, noout, kt, dT, tt)
+        SCAL_STAGE(2, z1, sZ1[s2(t - 2)], false, z2, { sZ2[s2(t - 2)][PAD + x] = val; }, noout, (KPOLY ? kw[bq] : kt), dT, tt)
         SCAL_STAGE(2, c1, sC1[s2(t - 2)], true, c2, { sC2[s2(t - 2)][PAD + x] = val; }, noout, kc, dC, ct)
-        SCAL_STAGE(3, z2, sZ2[s2(t - 3)], false, z3, {}, Tn_out, kt, dT, tt)
+        SCAL_STAGE(3, z2, sZ2[s2(t - 3)], false, z3, {}, Tn_out, (KPOLY ? kw[bq] : kt), dT, tt)
         SCAL_STAGE(3, c2, sC2[s2(t - 3)], true, c3, {}, Cn_out, kc, dC, ct)
 #undef SCAL_STAGE
     };
@@ -1159,7 +1172,7 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
 // neighbour. Same expressions as scal_tile.
 // ---------------------------------------------------------------------------------------
 
-template <typename R, bool STEADY, int NT>
+template <typename R, bool STEADY, int NT, bool KPOLY>
 __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int bj, Smem2<R, NT> &sm)
 {
     const int im = a.imax, jm = a.jmax;
@@ -1195,9 +1208,10 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
     auto ring1 = isC ? sC1 : sZ1;
     auto ring2 = isC ? sC2 : sZ2;
 
-    Win<4, R> zt, uw, vw, vnw, id, oml, dZ;
+    Win<4, R> zt, uw, vw, vnw, id, oml, dZ, kw;
     Win<3, R> y1, y2, y3;
     zt.fill(R(0)); uw.fill(R(0)); vw.fill(R(0)); vnw.fill(R(0)); id.fill(R(1)); oml.fill(R(1));
+    if (KPOLY) kw.fill(kap);
     if (!STEADY) dZ.fill(R(0));
     y1.fill(R(0)); y2.fill(R(0)); y3.fill(R(0));
 
@@ -1238,6 +1252,7 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
             id.push(Lp != R(0) ? R(1) / (Lp * (e - R(1)) + R(1)) : R(1));
             oml.push(R(1) - Lp);
             if (!STEADY) dZ.push(Zp - znew);
+            if (KPOLY) kw.push(isC ? kap : kap * kpoly(a.kcoef, STEADY ? znew : Zp));
         }
 #define SCS_STAGE(K, WP, CROW, WOUT, RINGW, OUTP)                                                \
         {                                                                                         \
@@ -1254,8 +1269,9 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
                         const R zS = isC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, tinf, tb, ttop); \
                         const R zN = isC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, tinf, tb, ttop);   \
                         const R avn_r = R(pr[RR + r]);                                            \
-                        const R diff = (kap * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
-                                     + (kap * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP));   \
+                        const R kq = KPOLY ? kw[bq] : kap;                                        \
+                        const R diff = (kq * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
+                                     + (kq * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP));    \
                         const R uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];          \
                         const R conv = R(0.5) * ((zE + zP) * uE - (zW + zP) * uP) * aue_j         \
                                      + R(0.5) * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;        \
@@ -1288,14 +1304,14 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
     }
 }
 
-template <typename R, bool STEADY, int NT>
+template <typename R, bool STEADY, int NT, bool KPOLY>
 __global__ __launch_bounds__(2 * NT, 4) void k_scal_split(const K2Args<R> a)
 {
     __shared__ Smem2<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    scal_tile_split<R, STEADY, NT>(a, L / a.nbj, L % a.nbj, sm);
+    scal_tile_split<R, STEADY, NT, KPOLY>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, int NT>
@@ -1308,19 +1324,19 @@ __global__ __launch_bounds__(2 * NT, 4) void k_scal_split_batch(const K2Args<R> 
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        scal_tile_split<R, STEADY, NT>(a, tl.bi, tl.bj, sm);
+        scal_tile_split<R, STEADY, NT, false>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
 
-template <typename R, bool STEADY, int OCC, int NT>
+template <typename R, bool STEADY, int OCC, int NT, bool KPOLY>
 __global__ __launch_bounds__(NT, OCC) void k_scal(const K2Args<R> a)
 {
     __shared__ Smem2<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
-    scal_tile<R, STEADY, NT>(a, L / a.nbj, L % a.nbj, sm);
+    scal_tile<R, STEADY, NT, KPOLY>(a, L / a.nbj, L % a.nbj, sm);
 }
 
 template <typename R, bool STEADY, int OCC, int NT>
@@ -1333,7 +1349,7 @@ __global__ __launch_bounds__(NT, OCC) void k_scal_batch(const K2Args<R> *cases, 
         const int cs = ctl[tl.c];
         if (!(cs & 1)) continue;
         const K2Args<R> a = cases[2 * tl.c + ((cs >> 1) & 1)];
-        scal_tile<R, STEADY, NT>(a, tl.bi, tl.bj, sm);
+        scal_tile<R, STEADY, NT, false>(a, tl.bi, tl.bj, sm);
         __syncthreads();
     }
 }
@@ -1466,6 +1482,7 @@ void fill_k2(K2Args<R> &k2, int im, int jm, const double *y, const double *pack,
     k2.dt = P->dt; k2.dtau = P->dtau; k2.kt = 1.0 / P->pe; k2.kc = 1.0 / P->re / P->sc;
     k2.tinf = P->tinf; k2.t_bottom = P->t_bottom; k2.t_top = P->t_top; k2.v_i = P->v_i; k2.u_in = P->u_in;
     k2.slot_lo = P->slot_lo; k2.slot_hi = P->slot_hi; k2.slot_value = P->slot_value;
+    for (int q = 0; q < 5; ++q) k2.kcoef[q] = P->k_coef[q];
     k2.nbj = cdiv(jm - 2, w2(nt));
     // beside the one-wave u/v/p kernel (the two-stream step) one T/C workgroup per CU fits;
     // after the two-wave one, which fills the CUs, two
@@ -1522,21 +1539,28 @@ void launch_k1(const K1Args<R> &k1, const cfd_params *P, int occ, hipStream_t s,
     else launch_k1_nt<R, NT1>(k1, P, occ, s);
 }
 
-template <typename R, int NT>
-void launch_k2_nt(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
+template <typename R, int NT, bool KP>
+void launch_k2_kp(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
 {
     const int G2 = k2.nbi * k2.nbj, grid2 = ((G2 + 7) / 8) * 8;
     const dim3 g2(grid2), b2(NT);
     if (occ == 2) {           // split chains: T and C in separate waves
-        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true, NT>), g2, dim3(2 * NT), 0, s, k2);
-        else hipLaunchKernelGGL((k_scal_split<R, false, NT>), g2, dim3(2 * NT), 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal_split<R, true, NT, KP>), g2, dim3(2 * NT), 0, s, k2);
+        else hipLaunchKernelGGL((k_scal_split<R, false, NT, KP>), g2, dim3(2 * NT), 0, s, k2);
     } else if (occ == 4) {
-        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4, NT>), g2, b2, 0, s, k2);
-        else hipLaunchKernelGGL((k_scal<R, false, 4, NT>), g2, b2, 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 4, NT, KP>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<R, false, 4, NT, KP>), g2, b2, 0, s, k2);
     } else {
-        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 1, NT>), g2, b2, 0, s, k2);
-        else hipLaunchKernelGGL((k_scal<R, false, 1, NT>), g2, b2, 0, s, k2);
+        if (P->steady) hipLaunchKernelGGL((k_scal<R, true, 1, NT, KP>), g2, b2, 0, s, k2);
+        else hipLaunchKernelGGL((k_scal<R, false, 1, NT, KP>), g2, b2, 0, s, k2);
     }
+}
+
+template <typename R, int NT>
+void launch_k2_nt(const K2Args<R> &k2, const cfd_params *P, int occ, hipStream_t s)
+{
+    if (P->k_model == CFD_K_POLY4) launch_k2_kp<R, NT, true>(k2, P, occ, s);
+    else launch_k2_kp<R, NT, false>(k2, P, occ, s);
 }
 
 template <typename R>

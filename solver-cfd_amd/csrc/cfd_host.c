@@ -82,6 +82,7 @@ void cfd_params_default(cfd_params *p, int32_t imax, int32_t jmax)
     p->u_in = 1.0; p->t_bottom = 1.0; p->t_top = 0.0;
     p->slot_lo = 0.0; p->slot_hi = 0.0; p->slot_value = 1.0;
     p->mu_coef[0] = 1.0;
+    p->k_coef[0] = 1.0; p->k_model = CFD_K_CONST;
 }
 
 static int check_params(const cfd_params *p)
@@ -94,6 +95,7 @@ static int check_params(const cfd_params *p)
     if (p->buoyancy < 0 || p->buoyancy > 1) return fail(CFD_ERR_ARG, "unknown buoyancy model %d", p->buoyancy);
     if (p->compat < 0 || p->compat > 1) return fail(CFD_ERR_ARG, "unknown compat %d", p->compat);
     if (p->mu_model < 0 || p->mu_model > 1) return fail(CFD_ERR_ARG, "unknown mu model %d", p->mu_model);
+    if (p->k_model < 0 || p->k_model > 1) return fail(CFD_ERR_ARG, "unknown conductivity model %d", p->k_model);
     if (!(p->re > 0) || !(p->pe > 0) || !(p->sc > 0) || !(p->darcy > 0))
         return fail(CFD_ERR_ARG, "Re, Pe, Sc and Da must be positive");
     return CFD_OK;
@@ -599,13 +601,13 @@ static int x_solve_s(const xenv *e, int species, const double *um_n, const doubl
 {
     const cfd_devgeo *g = e->g; const cfd_params *P = e->P;
     const int kind = species ? 2 : 1;
-    CHK(cfdk_ressc(g, P, species, um_n, vm_n, z_tau, rz, e->s));
+    CHK(cfdk_ressc(g, P, species, um_n, vm_n, z_tau, z, rz, e->s));
     CHK(cfdk_stage(g, P, 2, 1, z, z_tau, rz, NULL, zi, NULL, e->s));
     CHK(cfdk_bc_s(g, P, kind, zi, e->s));
-    CHK(cfdk_ressc(g, P, species, um_n, vm_n, zi, rz, e->s));
+    CHK(cfdk_ressc(g, P, species, um_n, vm_n, zi, z, rz, e->s));
     CHK(cfdk_stage(g, P, 2, 2, z, z_tau, rz, zi, zi, NULL, e->s));
     CHK(cfdk_bc_s(g, P, kind, zi, e->s));
-    CHK(cfdk_ressc(g, P, species, um_n, vm_n, zi, rz, e->s));
+    CHK(cfdk_ressc(g, P, species, um_n, vm_n, zi, z, rz, e->s));
     CHK(cfdk_stage(g, P, 2, 3, z, z_tau, rz, zi, z_n_tau, NULL, e->s));
     CHK(cfdk_bc_s(g, P, kind, z_n_tau, e->s));
     return CFD_OK;
@@ -1349,13 +1351,13 @@ void RESV(double *dev_um, double *dev_vm, double *dev_p, double *dev_t, double *
 void RESZ(double *dev_um_n, double *dev_vm_n, double *dev_z, double *dev_rz)
 {
     LEGACY_CTX();
-    cfdk_ressc(&c->g, &c->P, 0, dev_um_n, dev_vm_n, dev_z, dev_rz, c->s);
+    cfdk_ressc(&c->g, &c->P, 0, dev_um_n, dev_vm_n, dev_z, NULL, dev_rz, c->s);
 }
 
 void RESC(double *dev_um_n, double *dev_vm_n, double *dev_c, double *dev_rc)
 {
     LEGACY_CTX();
-    cfdk_ressc(&c->g, &c->P, 1, dev_um_n, dev_vm_n, dev_c, dev_rc, c->s);
+    cfdk_ressc(&c->g, &c->P, 1, dev_um_n, dev_vm_n, dev_c, NULL, dev_rc, c->s);
 }
 
 void bcUV(double *dev_um, double *dev_vm)

@@ -25,9 +25,10 @@ int cfdk_resu(const cfd_devgeo *g, const cfd_params *P, int mode, const double *
               const double *p, const double *T, double *ru, hipStream_t s);
 int cfdk_resv(const cfd_devgeo *g, const cfd_params *P, int mode, const double *um, const double *vm,
               const double *p, const double *T, double *rv, hipStream_t s);
-/* species = 0: RESZ (1/Pe), 1: RESC (1/(Re Sc)) */
+/* species = 0: RESZ (1/Pe, or k(T)/Pe with k_model CFD_K_POLY4 at the temperature kT),
+ * 1: RESC (1/(Re Sc)); kT may be NULL (constant diffusivity) */
 int cfdk_ressc(const cfd_devgeo *g, const cfd_params *P, int species, const double *um, const double *vm,
-               const double *z, double *rz, hipStream_t s);
+               const double *z, const double *kT, double *rz, hipStream_t s);
 /* RK3 stage k (1,2,3) on the u region (i=3..imax-1, j=2..jmax-1), v region (i=2..imax-1,
  * j=3..jmax-1) or scalar region (i=2..imax-1, j=2..jmax-1):
  *   res = ((phys - tau) + r*dt)*dtau ; out = tau + res | 0.75 tau + 0.25 (prev + res) |

@@ -58,13 +58,15 @@ __global__ void k_resv(Geo g, cfd_params P, int mode, const double *um, const do
     rv[iv(g, i, j)] = resv_lit(g, P, um, vm, p, T, i, j, quick);
 }
 
+// kT (may be null): physical-level temperature of the k(T) diffusivity, k(T)/Pe per cell
 __global__ void k_ressc(Geo g, double k, int species, const double *um, const double *vm,
-                        const double *z, double *rz)
+                        const double *z, const double *kT, cfd_params P, double *rz)
 {
     const int j = blockIdx.x * BX + threadIdx.x + 2;
     const int i = blockIdx.y * BY + threadIdx.y + 2;
     if (i > g.imax - 1 || j > g.jmax - 1) return;
-    rz[iu(g, i, j)] = ressc_lit(g, k, species != 0, um, vm, z, i, j);
+    const double kk = kT ? cond_lit(P, kT[iu(g, i, j)]) : k;
+    rz[iu(g, i, j)] = ressc_lit(g, kk, species != 0, um, vm, z, i, j);
 }
 
 // region 0: u (i=3..imax-1, j=2..jmax-1, stride jmax+1); 1: v (i=2..imax-1, j=3..jmax-1,
@@ -311,11 +313,12 @@ int cfdk_resv(const cfd_devgeo *g, const cfd_params *P, int mode, const double *
 }
 
 int cfdk_ressc(const cfd_devgeo *g, const cfd_params *P, int species, const double *um, const double *vm,
-               const double *z, double *rz, hipStream_t s)
+               const double *z, const double *kT, double *rz, hipStream_t s)
 {
     const double k = species ? 1.0 / P->re / P->sc : 1.0 / P->pe;
+    const double *kt = (!species && P->k_model == CFD_K_POLY4) ? kT : nullptr;
     dim3 grid(cdiv(g->jmax - 2, BX), cdiv(g->imax - 2, BY));
-    hipLaunchKernelGGL(k_ressc, grid, dim3(BX, BY), 0, s, mk(g), k, species, um, vm, z, rz);
+    hipLaunchKernelGGL(k_ressc, grid, dim3(BX, BY), 0, s, mk(g), k, species, um, vm, z, kt, *P, rz);
     return launched();
 }
 

@@ -14,6 +14,7 @@ module cfd_amd
     integer(c_int32_t), parameter :: CFD_BUOY_REFERENCE = 0, CFD_BUOY_BOUSSINESQ = 1
     integer(c_int32_t), parameter :: CFD_COMPAT_FORTRAN = 0, CFD_COMPAT_CUDA = 1
     integer(c_int32_t), parameter :: CFD_PREC_FP64 = 0, CFD_PREC_FP32 = 1
+    integer(c_int32_t), parameter :: CFD_MU_CONST = 0, CFD_MU_POLY4 = 1, CFD_K_CONST = 0, CFD_K_POLY4 = 1
     integer(c_int32_t), parameter :: CFD_LAYOUT_C = 0, CFD_LAYOUT_FORTRAN = 1
     integer(c_int32_t), parameter :: CFD_F_UM = 0, CFD_F_VM = 1, CFD_F_UM_TAU = 2, CFD_F_VM_TAU = 3, &
                                      CFD_F_P = 4, CFD_F_T = 5, CFD_F_T_TAU = 6, CFD_F_C = 7, &
@@ -25,6 +26,8 @@ module cfd_amd
         real(c_double) :: dtau, dt, beta, b_art, u_in, t_bottom, t_top
         real(c_double) :: slot_lo, slot_hi, slot_value
         real(c_double) :: mu_coef(5)
+        real(c_double) :: k_coef(5)
+        integer(c_int32_t) :: k_model, reserved
     end type cfd_params
 
     interface

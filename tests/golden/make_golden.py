@@ -280,7 +280,18 @@ def case_refvtk51x84():
     print("wrote refvtk51x84")
 
 
-CASES = dict(refout51x84=case_refout51x84, refvtk51x84=case_refvtk51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
+def case_properties_ch4():
+    """The reference's mixture properties (properties_CH4.f90, built with the harness) at
+    ambient temperatures 250-900 K: k_tot, rho_tot, cp_tot, nu_tot, alpha_tot, Pr."""
+    b = build("t24", **T24)
+    temps = np.linspace(250.0, 900.0, 27)
+    props = H.properties(b, temps)
+    np.savez_compressed(os.path.join(OUT, "properties_ch4.npz"), temps=temps, props=props,
+                        columns=np.array(["k_tot", "rho_tot", "cp_tot", "nu_tot", "alpha_tot", "Pr"]))
+    print("wrote properties_ch4")
+
+
+CASES = dict(properties_ch4=case_properties_ch4, refout51x84=case_refout51x84, refvtk51x84=case_refvtk51x84, ops24=case_ops24, steps24=case_steps24, steady24=case_steady24,
              phys24=case_phys24, uniform51_pe100=case_uniform51_pe100, steady51_converged=case_steady51_converged,
              quick_porous204=case_quick_porous204, quick_porous48=case_quick_porous48, mesh51=case_mesh51)
 

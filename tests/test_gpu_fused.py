@@ -14,7 +14,7 @@ import oracle as O
 from cfd_amd import lib as L
 from cfd_amd import cases as C
 from cfd_amd.params import (make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4, PREC_FP32,
-                             PREC_FP64)
+                             PREC_FP64, K_POLY4)
 from helpers import load_golden, max_rel, field_rel
 
 pytestmark = pytest.mark.gpu
@@ -360,4 +360,52 @@ def test_residual_l2_matches_oracle(steady, prec, waves):
         assert np.all(rel <= bar), (n, rel)
     nf, ne = ctx.step_counts()
     assert ne == 1 and nf == 40
+    ctx.close()
+
+
+@pytest.mark.parametrize("steady,prec,waves_tc", [(0, PREC_FP64, 1), (0, PREC_FP64, 2), (0, PREC_FP64, 4),
+                                                  (1, PREC_FP64, 0), (0, PREC_FP32, 0)])
+def test_conductivity_poly_matches_oracle(steady, prec, waves_tc):
+    """k(T) (k_model K_POLY4, the reference's methane/air conductivity fit as a polynomial in
+    T, with mu(T) too) through the exact first step and 150 fused steps, against the oracle:
+    FP64 1e-10 per field (normwise), FP32 storage 1e-5; every T/C kernel variant."""
+    say("case: k(T), steady", steady, "precision", prec, "T/C waves", waves_tc)
+    im, jm = 70, 52
+    x = C.tanh_nodes(im, 0.0, 1.0, 0.3, 2.0)
+    y = C.tanh_nodes(jm, 0.0, 1.0, 0.3, 2.0)
+    eps, lp = C.porous_block(x, y, 0.3, 0.6, 0.25, 0.7, 0.4, 0.9, seed=31)
+    dt = C.stable_dt(x, y, 2.0 / 80.0, 1.0 / (0.7 * 80.0) * 2.5, 1.5, 0.05)
+    prm = make_params(im, jm, re=80.0, pe=0.7 * 80.0, sc=0.7, beta=1.0, darcy=1e-2, cf=0.55, g=0.0, dt=dt,
+                      bc=BC_CHANNEL, buoyancy=BUOY_BOUSSINESQ, ri=1.0, t_ref=0.5, invfr2=0.0, u_in=1.0,
+                      t_bottom=1.0, t_top=0.0, slot_lo=0.45, slot_hi=0.55, slot_value=1.0, steady=steady,
+                      mu_model=MU_POLY4, mu_coef=C.mu_poly_coefficients(3),
+                      k_model=K_POLY4, k_coef=C.conductivity_poly_ch4_air(300.0, 300.0))
+    st = C.perturbed_state(im, jm, seed=32, amp=5e-2, u0=0.4, v0=0.05, p0=0.2, T0=0.6, C0=0.1,
+                           perturb_scalars=True)
+    o = O.Oracle(prm, x, y, eps, lp)
+    s, cs = o.make_state(st)
+    ctx = L.Context(prm, x, y, eps, lp, precision=prec)
+    ctx.set_state(st)
+    if waves_tc:
+        ctx.set_tiles(0, 0, 1, waves_tc)
+    o.pseudo_steps(s, cs, 1, history=False)
+    ctx.pseudo_steps(1)
+    if prec == PREC_FP64:
+        # the exact first step reproduces the oracle's arithmetic
+        e1 = {k: max_rel(ctx.download(k), s[k]) for k in FIELDS}
+        say("after the exact step", e1)
+        assert e1["T_tau"] == 0.0
+    o.pseudo_steps(s, cs, 150, history=False)
+    ctx.pseudo_steps(150)
+    errs = {k: max_rel(ctx.download(k), s[k]) for k in FIELDS}
+    nf, ne = ctx.step_counts()
+    say("fused", nf, "exact", ne, "fields", errs)
+    assert nf == 150 and ne == 1
+    assert max(errs.values()) <= (1e-10 if prec == PREC_FP64 else 1e-5)
+    # and k(T) matters: the constant-diffusivity solve differs
+    o2 = O.Oracle(make_params(im, jm, **{k: getattr(prm, k) for k, _ in type(prm)._fields_
+                                         if k not in ("imax", "jmax", "k_model")}), x, y, eps, lp)
+    s2, cs2 = o2.make_state(st)
+    o2.pseudo_steps(s2, cs2, 151, history=False)
+    assert max_rel(s2["T_tau"], s["T_tau"]) > 1e-6
     ctx.close()

@@ -4,12 +4,13 @@ The golden vectors of tests/golden/ were produced by the flang-built OpenMp_Mult
 path (tests/golden/make_golden.py). Both sides compute with one IEEE rounding per
 operation in the same order, so the oracle must match them exactly.
 """
+import os
 import numpy as np
 import pytest
 
 import oracle as O
 from cfd_amd import cases as C
-from cfd_amd.params import make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4
+from cfd_amd.params import make_params, SCHEME_UPWIND, BC_CHANNEL, BUOY_BOUSSINESQ, MU_POLY4, K_POLY4
 from helpers import load_golden, small_case, random_state
 
 
@@ -484,3 +485,87 @@ def test_residual_l2_closed_forms():
     # and the maxima of the same entries
     assert h[0, 0] == pytest.approx(dtau * np.max(np.abs(du)), rel=1e-15)
     assert h[0, 1] == pytest.approx(dtau * np.max(np.abs(dv)), rel=1e-15)
+
+
+# ---- k(T): the energy diffusivity k(T)/Pe (k_model K_POLY4; the reference's RESZ has 1/Pe) --
+def _kt_params(prm, **kw):
+    base = {k: getattr(prm, k) for k, _ in type(prm)._fields_ if k not in ("imax", "jmax", "k_coef", "k_model")}
+    base.update(kw)
+    return make_params(prm.imax, prm.jmax, **base)
+
+
+@pytest.mark.parametrize("k,t", [(0, 2.0), (1, 2.0), (2, 3.0), (3, 2.0), (4, 2.0), (1, 0.5)])
+def test_conductivity_each_power_scales_peclet(k, t):
+    """k(T) = T^k at a uniform temperature t is the constant-diffusivity RESZ at Pe / t^k; a
+    wrong Horner order, a dropped power or the iterate in place of the physical-level
+    temperature gives another power and fails."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    coef = [0.0] * 5
+    coef[k] = 1.0
+    ok = O.Oracle(_kt_params(prm, k_model=K_POLY4, k_coef=coef), x, y, eps, lp)
+    oc = O.Oracle(_kt_params(prm, pe=prm.pe / t ** k), x, y, eps, lp)
+    um, vm, T = st["um"], st["vm"], st["T"]
+    Tu = np.full_like(T, t)
+    a, b = ok.resz_kt(um, vm, T, Tu), oc.resz(um, vm, T)
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.max(np.abs(b))), (k, t)
+    # the reference's constant form where k(T) = 1
+    one = O.Oracle(_kt_params(prm, k_model=K_POLY4, k_coef=(1.0, 0, 0, 0, 0)), x, y, eps, lp)
+    assert np.array_equal(one.resz_kt(um, vm, T, T), O.Oracle(prm, x, y, eps, lp).resz(um, vm, T))
+
+
+def test_conductivity_reads_the_temperature_of_its_own_cell():
+    """Changing the temperature of the k(T) evaluation in one cell changes RZ there only."""
+    prm, x, y, eps, lp, st, out, meta = load_golden("ops24")
+    o = O.Oracle(_kt_params(prm, k_model=K_POLY4, k_coef=(0.8, 0.3, 0.1, 0.02, 0.001)), x, y, eps, lp)
+    um, vm, T = st["um"], st["vm"], st["T"]
+    for (i0, j0) in ((5, 7), (13, 3), (22, 18)):
+        T2 = T.copy()
+        T2[i0, j0] *= 1.7
+        d = np.argwhere(o.resz_kt(um, vm, T, T2) != o.resz_kt(um, vm, T, T)).tolist()
+        assert d == [[i0, j0]], (i0, j0, d)
+
+
+def test_conductivity_takes_the_physical_level_in_solve_z():
+    """In a dual-time pseudo-step k(T) takes the physical-level T (fixed through the stages,
+    like mu(T)): solve_Z equals the three stages with RESZ at k(T^n), and changing T_tau
+    changes the result only through the transported field."""
+    prm, x, y, eps, lp = small_case()
+    p2 = _kt_params(prm, k_model=K_POLY4, k_coef=(0.8, 0.3, 0.1, 0.02, 0.001), b_art=0.0)
+    o = O.Oracle(p2, x, y, eps, lp)
+    st = random_state(prm.imax, prm.jmax, seed=51)
+    st2 = random_state(prm.imax, prm.jmax, seed=52)
+    um, vm, Tn, Tt = st["um"], st["vm"], st["T"], st2["T"]
+    zn = np.zeros_like(Tn)
+    o.solve_z(um, vm, Tn, zn, Tt)
+    # the same three stages by hand with RESZ at k(T^n)
+    dt, dtau = p2.dt, p2.dtau
+    im, jm = prm.imax, prm.jmax
+    r = o.resz_kt(um, vm, Tt, Tn)
+    zi = Tt.copy()
+    zi[2:im, 2:jm] = Tt[2:im, 2:jm] + ((Tn - Tt) + r * dt)[2:im, 2:jm] * dtau
+    o.bcz(zi)
+    r = o.resz_kt(um, vm, zi, Tn)
+    zi[2:im, 2:jm] = 0.75 * Tt[2:im, 2:jm] + 0.25 * (zi + ((Tn - Tt) + r * dt) * dtau)[2:im, 2:jm]
+    o.bcz(zi)
+    r = o.resz_kt(um, vm, zi, Tn)
+    z3 = np.zeros_like(Tt)
+    z3[2:im, 2:jm] = 1.0 / 3.0 * Tt[2:im, 2:jm] + 2.0 / 3.0 * (zi + ((Tn - Tt) + r * dt) * dtau)[2:im, 2:jm]
+    o.bcz(z3)
+    assert np.array_equal(zn[1:, 1:], z3[1:, 1:])
+
+
+def test_conductivity_polynomial_matches_reference_properties():
+    """cases.conductivity_poly_ch4_air restates the reference's mixture conductivity
+    (properties_CH4.f90) as a polynomial in the nondimensional temperature; evaluated at
+    t0 + dT T it equals the reference's own k_tot(Too) (the properties_ch4 fixture, made by
+    the flang-built reference routine) over 250-900 K, normalised by k_tot(t0)."""
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "properties_ch4.npz"))
+    temps, k_ref = z["temps"], z["props"][:, 0]
+    for t0, dT in ((300.0, 300.0), (250.0, 650.0)):
+        c = C.conductivity_poly_ch4_air(t0, dT)
+        assert c[4] == 0.0 and c[0] == 1.0
+        th = (temps - t0) / dT
+        k0 = k_ref[np.argmin(np.abs(temps - t0))]
+        assert abs(temps[np.argmin(np.abs(temps - t0))] - t0) < 1e-12
+        got = k0 * sum(c[n] * th ** n for n in range(5))
+        assert np.max(np.abs(got - k_ref) / k_ref) < 1e-13
