@@ -103,14 +103,34 @@ def _build_batch(cs, prec):
     return b
 
 
+def run_plain(cases, tol, itc_max, log=log, check_every=500):
+    """The plain FP64 path, run for real: one context per case from its initial state, its
+    tile shape autotuned (cached per grid class for the process), cfd_run_to_tol to tol or
+    itc_max. Returns per case (pseudo-steps, seconds of device time between the run's HIP
+    events, converged)."""
+    out = []
+    for c in cases:
+        ctx = L.Context(c["params"], c["x"], c["y"], c["eps"], c["lp"])
+        ctx.set_state(c["state"])
+        ctx.autotune(steps=10)
+        code, it, res = ctx.run_to_tol(tol, itc_max, check_every=check_every, raise_on_fail=False)
+        secs = ctx.last_ms_per_step() * it * 1e-3 if code == 0 else ctx.last_ms_per_step() * itc_max * 1e-3
+        out.append((int(it), secs, code == 0))
+        ctx.close()
+        if log:
+            log("  plain %s: %d steps, %.2f s, %s" % (c["name"], it, secs, "converged" if code == 0 else "not converged"))
+    return out
+
+
 def bench_batch(args):
     """Time to residual `tol` for every case of the seeded mixed batch (grids {64..512} x
     {64..256}, Re, Da, porosity, Ri drawn per case; steady channel flow from zero fields).
 
-    plain FP64 path: one context per case, cfd_run_to_tol, autotuned tiles. Its arithmetic is
-    bit-identical to the FP64 batch (tested), so its per-case step counts are the FP64
-    batch's; its time is the sum over cases of steps x the context's measured time per step
-    for that grid class. batched: FP32 batch (persistent launches, graph replay) down to
+    plain FP64 path: one context per case, cfd_run_to_tol, autotuned tiles; the first case of
+    each grid class runs for real to the same tolerance and cap, and the class's other cases
+    are priced at its measured time per step and their own FP64-batch step counts (the batch
+    and a context step the same arithmetic; the sample reports how many of its step counts
+    equal the batch's). batched: FP32 batch (persistent launches, graph replay) down to
     max(tol, 5e-7) (the FP32 storage floor is ~2e-7), then the FP64 batch from the converted
     state down to tol, both timed with events. Cases that do not converge run to itc_max on
     both paths. Steady fields of cases converged on both paths are compared per field."""
@@ -129,21 +149,23 @@ def bench_batch(args):
         log("  case %s failed (status %d at step %d): Re %.1f Da %.2e Ri %.2f dt %.3e"
             % (c["name"], st64[i], it64[i], c["re"], c["da"], c["ri"], c["params"].dt))
     log("FP64 batch: %d/%d converged, %.1f s, tiles %s" % (int(conv.sum()), len(cs), ms64 * 1e-3, tiles64))
-    # plain path: measured time per step of one context per grid class
-    t_class = {}
-    for c in cs:
+    # plain path, run for real on a stratified sample: the first case of each grid class,
+    # one context each, to the same tolerance and cap; a class's other cases are priced at
+    # its sample's measured time per step times their own FP64 step counts
+    t_class, sample = {}, []
+    for i, c in enumerate(cs):
         k = (c["params"].imax, c["params"].jmax)
-        if k in t_class:
-            continue
-        ctx = L.Context(c["params"], c["x"], c["y"], c["eps"], c["lp"])
-        ctx.set_state(c["state"])
-        ctx.autotune(steps=10)          # the plain path at its best tile shape for this grid
-        ctx.run_to_tol(0.0, 50, check_every=50, raise_on_fail=False)
-        ctx.run_to_tol(0.0, 400, check_every=100, raise_on_fail=False)
-        t_class[k] = ctx.last_ms_per_step()
-        ctx.close()
+        if k not in t_class:
+            t_class[k] = None
+            sample.append(i)
+    runs = run_plain([cs[i] for i in sample], args.batch_tol, args.batch_itc, log=log)
+    for i, (it, secs, ok) in zip(sample, runs):
+        t_class[(cs[i]["params"].imax, cs[i]["params"].jmax)] = secs / max(it, 1) * 1e3
+    same_steps = sum(1 for i, (it, _, _) in zip(sample, runs) if it == int(it64[i]))
     t_plain = sum(int(it64[i]) * t_class[(c["params"].imax, c["params"].jmax)] for i, c in enumerate(cs)) * 1e-3
-    log("plain FP64 path: %.1f s" % t_plain)
+    t_sample = sum(r[1] for r in runs)
+    log("plain FP64 path: sample of %d cases (one per grid class) %.1f s measured; all cases %.1f s"
+        % (len(sample), t_sample, t_plain))
     # batched FP32 (+ FP64 finish below the FP32 floor)
     sol = WS.solve(cs, args.batch_tol, args.batch_itc, log=log)
     stb = sol["status"]
@@ -174,6 +196,11 @@ def bench_batch(args):
                                "fp64_batch": float(np.sum(it64 * cells)) / (ms64 * 1e-3),
                                "fp32_batch": float(np.sum(sol["iters32"] * cells)) / max(sol["seconds32"], 1e-9)},
         "plain_ms_per_step": {"%dx%d" % k: v for k, v in sorted(t_class.items())},
+        "plain_sample": {"cases": len(sample), "seconds_measured": t_sample,
+                         "converged": int(sum(1 for r in runs if r[2])),
+                         "steps_equal_to_fp64_batch": same_steps,
+                         "note": "one case per grid class run for real; the rest priced at their class's measured "
+                                 "time per step and their own FP64-batch step counts"},
     }
     sol["batch"].close()
     b64.close()
@@ -221,7 +248,9 @@ def bench_warmstart(args):
     """Learned warm start on held-out cases: solver-generated steady states (one grid size)
     train the U-Net with the library's own kernels; the held-out cases are then solved from
     the network's prediction (with fallback to zero fields) and from zero fields, and the
-    plain FP64 path (one context per case from zero fields) is priced for comparison."""
+    plain FP64 path (one context per case from zero fields) runs for real on an evenly
+    spaced sample of the held-out cases (--ws-plain-sample), whose mean time per case
+    stands for all of them."""
     from cfd_amd import warmstart as WS
     n = args.ws_train + args.ws_test
     H = args.ws_size
@@ -252,22 +281,21 @@ def bench_warmstart(args):
     solw = WS.solve(test_cases, tol, args.ws_itc, states=states, fallback_factor=10.0, log=log)
     solz = WS.solve(test_cases, tol, args.ws_itc, log=log)
     itw = solw["iters32"] + solw["iters64"]; itz = solz["iters32"] + solz["iters64"]
-    # the plain FP64 path on the held-out cases: one context per case from zero fields. Its
-    # arithmetic is the FP64 batch's (tested bit for bit), so the FP64 batch from zero fields
-    # gives its per-case step counts, priced at one context's measured time per step
+    # the plain FP64 path on the held-out cases: one context per case from zero fields, run
+    # for real on an evenly spaced sample of them (to the same tolerance and cap); the whole
+    # set is the sample's mean measured time per case times the number of cases. The FP64
+    # batch from zero fields gives the step counts and convergence of every case.
     b64 = WS.build_batch(test_cases, PREC_FP64)
     (_, it64, _, st64), ms64 = b64.run_sliced(tol, args.ws_itc, log=log)
-    c0 = test_cases[0]
-    ctx = L.Context(c0["params"], c0["x"], c0["y"], c0["eps"], c0["lp"])
-    ctx.set_state(c0["state"])
-    ctx.autotune(steps=10)
-    ctx.run_to_tol(0.0, 50, check_every=50, raise_on_fail=False)
-    ctx.run_to_tol(0.0, 400, check_every=100, raise_on_fail=False)
-    t_ctx = ctx.last_ms_per_step()
-    ctx.close()
-    t_plain = float(it64.sum()) * t_ctx * 1e-3
-    log("plain FP64 path: %d steps x %.4f ms = %.1f s; warm-started batch %.1f s"
-        % (int(it64.sum()), t_ctx, t_plain, solw["seconds"]))
+    ns = min(args.ws_plain_sample, len(test_cases))
+    sidx = sorted(set(int(round(v)) for v in np.linspace(0, len(test_cases) - 1, ns)))
+    runs = run_plain([test_cases[k] for k in sidx], tol, args.ws_itc, log=log)
+    t_sample = sum(r[1] for r in runs)
+    t_plain = t_sample / len(sidx) * len(test_cases)
+    same_steps = sum(1 for k, r in zip(sidx, runs) if r[0] == int(it64[k]))
+    t_ctx = t_sample / max(sum(r[0] for r in runs), 1) * 1e3
+    log("plain FP64 path: %d sampled cases run for real, %.1f s (%.4f ms per step); all %d cases %.1f s; "
+        "warm-started batch %.1f s" % (len(sidx), t_sample, t_ctx, len(test_cases), t_plain, solw["seconds"]))
 
     def field_diffs(sol_a, b_ref, ok):
         e = {k: 0.0 for k in ("um_tau", "vm_tau", "p", "T_tau")}
@@ -295,6 +323,9 @@ def bench_warmstart(args):
         "solve_s_zero": solz["seconds"], "solve_s_warm": solw["seconds"],
         "steady_field_max_rel_diff_warm_vs_zero": errs,
         "plain_fp64": {"seconds": t_plain, "ms_per_step": t_ctx, "iters_total": int(it64.sum()),
+                       "sample": {"cases": len(sidx), "seconds_measured": t_sample,
+                                  "steps": [r[0] for r in runs], "converged": int(sum(1 for r in runs if r[2])),
+                                  "steps_equal_to_fp64_batch": same_steps},
                        "converged": int((st64 == 1).sum()), "fp64_batch_s": ms64 * 1e-3,
                        "speedup_warm_batched": t_plain / solw["seconds"],
                        "steady_field_max_rel_diff_warm_batched_vs_plain": errs_plain},
@@ -476,6 +507,8 @@ def main():
     ap.add_argument("--ws-epochs", type=int, default=30)
     ap.add_argument("--ws-itc", type=int, default=60000)
     ap.add_argument("--ws-train-tol", type=float, default=1e-5)
+    ap.add_argument("--ws-plain-sample", type=int, default=16,
+                    help="held-out cases run for real on the plain FP64 path (one context each)")
     args = ap.parse_args()
     if args.workload == "batch":
         return bench_batch(args)
