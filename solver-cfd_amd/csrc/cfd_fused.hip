@@ -277,14 +277,17 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         else { q_um = R(0); q_vm = R(0); }
         (void)in_u; (void)in_v; (void)in_p;
     };
-    // Rows run two at a time, so that the window shifts of the first row of a pair are
-    // register renames rather than moves. Pairs start at even rows: a row is always computed
-    // by the same half of the pair, whatever the block, so blocks of different heights (a
-    // batch case against a single context) give bit-identical results. The loop may thus
-    // start one row before ilo - 6 and end one row after ihi + 6; in those rows every stage,
-    // the P update and every global store are outside their row ranges (only ring slots that
-    // no computed row reads are written).
-    const int t0 = (ilo - 6) - ((ilo - 6) & 1);
+    // Rows run four at a time in the register-window form (two in the ring form, whose code
+    // would grow past what the extra unrolling saves), so that most window shifts are
+    // register renames rather than moves. Groups start at rows that are multiples of the
+    // group size: a row is always computed by the same position of the group, whatever the
+    // block, so blocks of different heights (a batch case against a single context) give
+    // bit-identical results. The loop may thus start up to UNR - 1 rows before ilo - 6 and
+    // end up to UNR - 1 rows after ihi + 6; in those rows every stage, the P update and
+    // every global store are outside their row ranges (only ring slots that no computed row
+    // reads are written).
+    constexpr int UNR = RINGS ? 2 : 4;
+    const int t0 = (ilo - 6) - ((ilo - 6) & (UNR - 1));
     __syncthreads();
     fetch(t0);
 
@@ -547,9 +550,13 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             }
         }
     };
-    for (int t = t0; t <= ihi + 6; t += 2) {
+    for (int t = t0; t <= ihi + 6; t += UNR) {
         row(t);
         row(t + 1);
+        if (UNR == 4) {
+            row(t + 2);
+            row(t + 3);
+        }
     }
     fold_max(a.res + 0, mu_);
     fold_max(a.res + 1, mv_);
@@ -571,6 +578,9 @@ __global__ __launch_bounds__(NT, OCC) void k_uvp(const K1Args<R> a)
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
     if (L >= G) return;
+    // issue priority over co-resident waves: in the two-stream step the T/C kernel of the
+    // previous pseudo-step shares the SIMDs, and this kernel is the critical path
+    __builtin_amdgcn_s_setprio(3);
     uvp_tile<R, STEADY, MUPOLY, NT, (OCC > 1), L2>(a, L / a.nbj, L % a.nbj, sm);
 }
 
@@ -899,6 +909,7 @@ __global__ __launch_bounds__(2 * NT, 1) void k_uvp_split(const K1Args<R> a)
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
     if (L >= G) return;
+    __builtin_amdgcn_s_setprio(3);      // as k_uvp
     uvp_tile_split<R, STEADY, MUPOLY, NT, L2>(a, L / a.nbj, L % a.nbj, sm);
 }
 
