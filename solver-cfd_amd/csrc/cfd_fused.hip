@@ -277,16 +277,17 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         else { q_um = R(0); q_vm = R(0); }
         (void)in_u; (void)in_v; (void)in_p;
     };
-    // Rows run four at a time in the register-window form (two in the ring form, whose code
-    // would grow past what the extra unrolling saves), so that most window shifts are
-    // register renames rather than moves. Groups start at rows that are multiples of the
+    // Rows run four at a time in the FP64 register-window form (two in the ring form, whose
+    // code would grow past what the extra unrolling saves, and in FP32, where the extra
+    // registers would leave no room for a T/C wave beside two u/v/p waves), so that most
+    // window shifts are register renames rather than moves. Groups start at rows that are multiples of the
     // group size: a row is always computed by the same position of the group, whatever the
     // block, so blocks of different heights (a batch case against a single context) give
     // bit-identical results. The loop may thus start up to UNR - 1 rows before ilo - 6 and
     // end up to UNR - 1 rows after ihi + 6; in those rows every stage, the P update and
     // every global store are outside their row ranges (only ring slots that no computed row
     // reads are written).
-    constexpr int UNR = RINGS ? 2 : 4;
+    constexpr int UNR = (RINGS || sizeof(R) == 4) ? 2 : 4;
     const int t0 = (ilo - 6) - ((ilo - 6) & (UNR - 1));
     __syncthreads();
     fetch(t0);
