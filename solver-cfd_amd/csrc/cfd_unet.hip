@@ -405,8 +405,17 @@ __global__ void k_wsum_final(const float *__restrict__ part, int nslices, int gp
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
+    // the slice sums are independent loads: eight in flight, added in slice order
     float s = 0.f;
-    for (int q = 0; q < nslices; ++q) s += part[(size_t)q * gper * stride + i];
+    int q = 0;
+    for (; q + 8 <= nslices; q += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = part[(size_t)(q + k) * gper * stride + i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; q < nslices; ++q) s += part[(size_t)q * gper * stride + i];
     dw[i] += s;
 }
 
