@@ -217,10 +217,10 @@ int cfd_batch_set_mesh(cfd_batch *b, int32_t c, const double *x, const double *y
         for (size_t t = 0; t < R; ++t) {
             const double a = (t >= 2 && t <= (size_t)im) ? 1.0 / (x[t] - x[t - 1]) : 0.0;
             const double bb = (t >= 2 && t <= (size_t)im + 1) ? 1.0 / (xm[t] - xm[t - 1]) : 0.0;
-            pr[t] = (t >= 2 && t <= (size_t)im) ? aun[t] : 0.0;
-            pr[R + t] = (t >= 1 && t <= (size_t)im) ? avn[t] : 0.0;
-            pr[2 * R + t] = a;
-            pr[3 * R + t] = bb;
+            pr[4 * t] = (t >= 2 && t <= (size_t)im) ? aun[t] : 0.0;
+            pr[4 * t + 1] = (t >= 1 && t <= (size_t)im) ? avn[t] : 0.0;
+            pr[4 * t + 2] = a;
+            pr[4 * t + 3] = bb;
         }
         for (size_t t = 0; t < C; ++t) {
             const double a = (t >= 2 && t <= (size_t)jm) ? 1.0 / (y[t] - y[t - 1]) : 0.0;

@@ -207,7 +207,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     const bool chan = a.chan != 0, upwind_all = a.upwind_all != 0, bouss = a.bouss != 0;
     const R dt = R(a.dt), dtau = R(a.dtau), vi = R(a.v_i), inv_re = R(a.inv_re), gg = R(a.g);
     const int RR = im + 2, Cc = jm + 2;
-    const double *pr = a.pk, *pc = a.pk + 4 * RR;    // rows [aun|avn|rdx|rdxm], cols [aue|ave|rdy|rdym]
+    const double *pr = a.pk, *pc = a.pk + 4 * RR;    // rows (aun avn rdx rdxm) x i, cols [aue|ave|rdy|rdym]
     auto sUT = sm.sUT; auto sVT = sm.sVT; auto sT = sm.sT; auto sP = sm.sP;
     auto sU1 = sm.sU1; auto sU2 = sm.sU2; auto sV1 = sm.sV1; auto sV2 = sm.sV2; auto sV3 = sm.sV3;
 
@@ -366,13 +366,13 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         const R uS = c[-1], uN = c[1];                                            \
                         const R iev = ie[bk], ev = ew[bk];                                        \
                         const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                              \
-                        const R aun_r = R(pr[r]), rdx_r = R(pr[2 * RR + r]);                      \
+                        const R aun_r = R(pr[4 * r]), rdx_r = R(pr[4 * r + 2]);                      \
                         const R cns = R(0.5) * aun_r * iev, cew = (RINGS ? R(0.5) * aue_j : haue_j) * iev; \
                         const R sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;           \
                         const R fn = Svn[2 * K - 1] * cns;                                        \
                         const R fs = (vt[2 * K] + vt[2 * K + 1]) * cns;                           \
                         const R fe = sE * cew, fw = sW * cew;                                     \
-                        const R diff = (nu * aue_j) * (R(pr[3 * RR + r]) * (uW - uP) + R(pr[3 * RR + r + 1]) * (uE - uP)) \
+                        const R diff = (nu * aue_j) * (R(pr[4 * r + 3]) * (uW - uP) + R(pr[4 * r + 7]) * (uE - uP)) \
                                      + (nu * aun_r) * (rdy_j * (uS - uP) + rdy_n * (uN - uP));    \
                         R pw_, pe_, ps_, pn_;                                                     \
                         if (u_upw_col) {                                                          \
@@ -432,12 +432,12 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         const R vS = c[-1], vN = c[1];                                            \
                         const R iev = ie[bk], ev = ew[bk];                                        \
                         const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                              \
-                        const R avn_r = R(pr[RR + r]), rdx_r = R(pr[2 * RR + r]);                 \
+                        const R avn_r = R(pr[4 * r + 1]), rdx_r = R(pr[4 * r + 2]);                 \
                         const R cns = R(0.5) * avn_r * iev, cew = (RINGS ? R(0.5) * ave_j : have_j) * iev; \
                         const R sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;           \
                         const R fn = sN * cns, fs = sS * cns;                                     \
                         const R fe = SU(2 * K - 1) * cew, fw = SU(2 * K) * cew;                   \
-                        const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[2 * RR + r + 1]) * (vE - vP)) \
+                        const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[4 * r + 6]) * (vE - vP)) \
                                      + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP));  \
                         R pw_, pe_, ps_, pn_;                                                     \
                         if (upwind_all || r == 2 || r == im - 1) {                                \
@@ -540,7 +540,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 const double u3p = (double)w3[1], u3e = (double)w3[0];
                 const double v3p = (double)x3[1];
                 const double v3n = (double)lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
-                const double avn_r = pr[RR + r];
+                const double avn_r = pr[4 * r + 1];
                 const double rpv = -((u3e * aue_jd - u3p * aue_jd) + (v3n * avn_r - v3p * avn_r));
                 const double dd = a.dtau * rpv * cst[CST_BETA];
                 const double pp = pw[5];
@@ -760,14 +760,14 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                             const R uS = c[-1], uN = c[1];                                        \
                             const R iev = ie[bk], ev = ew[bk];                                    \
                             const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                          \
-                            const R aun_r = R(pr[r]), rdx_r = R(pr[2 * RR + r]);                  \
+                            const R aun_r = R(pr[4 * r]), rdx_r = R(pr[4 * r + 2]);                  \
                             const R cns = R(0.5) * aun_r * iev, cew = haue_j * iev;               \
                             const R sE = uE + uP, sW = uP + uW, sS = uS + uP, sN = uP + uN;       \
                             const R vr = sVT[s16(r)][PAD + x], vr1 = sVT[s16(r - 1)][PAD + x];    \
                             const R fn = (sVT[s16(r)][PAD + x + 1] + sVT[s16(r - 1)][PAD + x + 1]) * cns; \
                             const R fs = (vr + vr1) * cns;                                        \
                             const R fe = sE * cew, fw = sW * cew;                                 \
-                            const R diff = (nu * aue_j) * (R(pr[3 * RR + r]) * (uW - uP) + R(pr[3 * RR + r + 1]) * (uE - uP)) \
+                            const R diff = (nu * aue_j) * (R(pr[4 * r + 3]) * (uW - uP) + R(pr[4 * r + 7]) * (uE - uP)) \
                                          + (nu * aun_r) * (rdy_j * (uS - uP) + rdy_n * (uN - uP)); \
                             R pw_, pe_, ps_, pn_;                                                 \
                             if (u_upw_col) {                                                      \
@@ -814,7 +814,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                 const double u3p = (double)w3[1], u3e = (double)w3[0];
                 const double v3p = (double)sV3[s2(r)][PAD + x];
                 const double v3n = (double)lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
-                const double avn_r = pr[RR + r];
+                const double avn_r = pr[4 * r + 1];
                 const double rpv = -((u3e * aue_jd - u3p * aue_jd) + (v3n * avn_r - v3p * avn_r));
                 const double ddp = a.dtau * rpv * a.beta;
                 const double pp = aux[5];
@@ -844,14 +844,14 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                             const R vS = c[-1], vN = c[1];                                        \
                             const R iev = ie[bk], ev = ew[bk];                                    \
                             const R nu = MUPOLY ? nuw[bk] : ev * inv_re;                          \
-                            const R avn_r = R(pr[RR + r]), rdx_r = R(pr[2 * RR + r]);             \
+                            const R avn_r = R(pr[4 * r + 1]), rdx_r = R(pr[4 * r + 2]);             \
                             const R cns = R(0.5) * avn_r * iev, cew = have_j * iev;               \
                             const R sN = vP + vN, sS = vP + vS, sE = vE + vP, sW = vP + vW;       \
                             const R fn = sN * cns, fs = sS * cns;                                 \
                             const R ur = sUT[s8(r)][PAD + x];                                     \
                             const R fe = (sUT[s8(r + 1)][PAD + x] + sUT[s8(r + 1)][PAD + x - 1]) * cew; \
                             const R fw = (ur + sUT[s8(r)][PAD + x - 1]) * cew;                    \
-                            const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[2 * RR + r + 1]) * (vE - vP)) \
+                            const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[4 * r + 6]) * (vE - vP)) \
                                          + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP)); \
                             R pw_, pe_, ps_, pn_;                                                 \
                             if (upwind_all || r == 2 || r == im - 1) {                            \
@@ -1130,13 +1130,13 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
                         R zS = c[PAD + x - 1], zN = c[PAD + x + 1];                               \
                         if (ISC) nb_c(zS, zN, zP, j, jm, chan);                                   \
                         else nb_t(zS, zN, zP, j, jm, chan, tinf, tb, ttop);                       \
-                        const R avn_r = R(pr[RR + r]);                                            \
-                        const R diff = (KAPPA * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
+                        const R avn_r = R(pr[4 * r + 1]);                                            \
+                        const R diff = (KAPPA * aue_j) * (R(pr[4 * r + 6]) * (zE - zP) + R(pr[4 * r + 2]) * (zW - zP)) \
                                      + (KAPPA * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP)); \
                         const R uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];          \
                         const R conv = R(0.5) * ((zE + zP) * uE - (zW + zP) * uP) * aue_j         \
                                      + R(0.5) * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;        \
-                        const R vols = R(pr[3 * RR + r + 1]) * rdym_n;                            \
+                        const R vols = R(pr[4 * r + 7]) * rdym_n;                            \
                         const R Rr = ISC ? vols * (diff - conv * id[bq])                          \
                                          : vols * (diff - oml[bq] * conv) * id[bq];               \
                         const R rs = ((STEADY ? R(0) : DW[bq]) + Rr * dt) * dtau;                 \
@@ -1280,14 +1280,14 @@ __device__ __forceinline__ void scal_tile_split(const K2Args<R> &a, int bi, int 
                         const R *c = CROW;                                                        \
                         const R zS = isC ? lds_c(c, x, j, -1, jm, chan) : lds_t(c, x, j, -1, jm, chan, tinf, tb, ttop); \
                         const R zN = isC ? lds_c(c, x, j, 1, jm, chan) : lds_t(c, x, j, 1, jm, chan, tinf, tb, ttop);   \
-                        const R avn_r = R(pr[RR + r]);                                            \
+                        const R avn_r = R(pr[4 * r + 1]);                                            \
                         const R kq = KPOLY ? kw[bq] : kap;                                        \
-                        const R diff = (kq * aue_j) * (R(pr[2 * RR + r + 1]) * (zE - zP) + R(pr[2 * RR + r]) * (zW - zP)) \
+                        const R diff = (kq * aue_j) * (R(pr[4 * r + 6]) * (zE - zP) + R(pr[4 * r + 2]) * (zW - zP)) \
                                      + (kq * avn_r) * (rdy_n * (zN - zP) + rdy_j * (zS - zP));    \
                         const R uE = uw[bq - 1], uP = uw[bq], vP = vw[bq], vN = vnw[bq];          \
                         const R conv = R(0.5) * ((zE + zP) * uE - (zW + zP) * uP) * aue_j         \
                                      + R(0.5) * ((zN + zP) * vN - (zS + zP) * vP) * avn_r;        \
-                        const R vols = R(pr[3 * RR + r + 1]) * rdym_n;                            \
+                        const R vols = R(pr[4 * r + 7]) * rdym_n;                            \
                         const R Rr = isC ? vols * (diff - conv * id[bq])                          \
                                          : vols * (diff - oml[bq] * conv) * id[bq];               \
                         const R rs = ((STEADY ? R(0) : dZ[bq]) + Rr * dt) * dtau;                 \
@@ -1658,10 +1658,10 @@ __global__ void k_factors(Geo g, double *rdx, double *rdy, double *rdxm, double 
         const double b = (t >= 2 && t <= g.imax + 1) ? 1.0 / (g.xm[t] - g.xm[t - 1]) : 0.0;
         if (t >= 2 && t <= g.imax) rdx[t] = a;
         if (t >= 2 && t <= g.imax + 1) rdxm[t] = b;
-        pr[t] = (t >= 2 && t <= g.imax) ? g.aun[t] : 0.0;
-        pr[R + t] = (t >= 1 && t <= g.imax) ? g.avn[t] : 0.0;
-        pr[2 * R + t] = a;
-        pr[3 * R + t] = b;
+        pr[4 * t] = (t >= 2 && t <= g.imax) ? g.aun[t] : 0.0;
+        pr[4 * t + 1] = (t >= 1 && t <= g.imax) ? g.avn[t] : 0.0;
+        pr[4 * t + 2] = a;
+        pr[4 * t + 3] = b;
     }
     if (t < Cc) {
         const double a = (t >= 2 && t <= g.jmax) ? 1.0 / (g.y[t] - g.y[t - 1]) : 0.0;

@@ -99,8 +99,10 @@ typedef struct cfd_devfactors {
     double *rdy;     /* 1/(y_j - y_{j-1})          j = 2..jmax   */
     double *rdxm;    /* 1/(xm_i - xm_{i-1})        i = 2..imax+1 */
     double *rdym;    /* 1/(ym_j - ym_{j-1})        j = 2..jmax+1 */
-    /* packed copy for the fused kernels, 4*(imax+2) + 4*(jmax+2) doubles:
-     * rows [aun | avn | rdx | rdxm] then columns [aue | ave | rdy | rdym] */
+    /* packed copy for the fused kernels, 4*(imax+2) + 4*(jmax+2) doubles: per row i the
+     * four values aun avn rdx rdxm side by side (element 4i+q), so that every row factor a
+     * kernel reads in one row iteration sits at a fixed offset from one address; then the
+     * columns [aue | ave | rdy | rdym], one array after the other */
     double *pack;
 } cfd_devfactors;
 
