@@ -35,7 +35,7 @@ HIP_COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wno-unus
 # they reproduce the CPU oracle bit for bit; the fused kernels may contract to FMA.
 HIP_SOURCES = [
     ("cfd_ops.hip", ["-ffp-contract=off"]),
-    ("cfd_fused.hip", ["-ffp-contract=fast"]),
+    ("cfd_fused.hip", ["-ffp-contract=on"]),
     ("cfd_unet.hip", ["-ffp-contract=fast"]),
     ("cfd_noalloc.hip", ["-ffp-contract=off"]),
 ]

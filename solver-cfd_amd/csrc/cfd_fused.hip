@@ -28,6 +28,8 @@
 // which equals the reference's coefficient assembly of equations.f90:51-72), reciprocals of
 // the 1-D spacings precomputed, FMA contraction allowed: FP64 results differ from the
 // literal reference arithmetic by rounding only (tested at 1e-10 relative).
+#include <type_traits>
+
 #include "cfd_device.h"
 #include "cfd_internal.h"
 
@@ -246,6 +248,11 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     Win<6, R> Svn;        // v_tau(r,j+1) + v_tau(r-1,j+1), rows t-6..t-1
     Win<6, double> pw;    // p rows t-7..t-2
     Win<5, R> ie, ew, nuw, k1w, k2w, dpj, bw, du, dv;   // per-cell data, rows t-6..t-2
+    // The unsteady register-window form keeps four per-cell inputs of rows t-6..t-2 in LDS
+    // (slot r mod 5, each lane its own column) rather than in registers: the source terms
+    // um - u_tau, vm - v_tau, the pressure difference p(j) - p(j-1) and the buoyancy term.
+    constexpr bool DLDS = !RINGS && !STEADY && sizeof(R) == 8;
+    __shared__ R sDUV[DLDS ? 20 : 1][DLDS ? NT : 1];
     Win<2, R> w3, x3;                                    // u3, v3
     ut.fill(R(0)); vt.fill(R(0)); Svn.fill(R(0)); pw.fill(0.0);
     if (!RINGS) { Su.fill(R(0)); w1.fill(R(0)); w2.fill(R(0)); x1.fill(R(0)); x2.fill(R(0)); }
@@ -277,22 +284,26 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         else { q_um = R(0); q_vm = R(0); }
         (void)in_u; (void)in_v; (void)in_p;
     };
-    // Rows run four at a time in the FP64 register-window form (two in the ring form, whose
-    // code would grow past what the extra unrolling saves, and in FP32, where the extra
-    // registers would leave no room for a T/C wave beside two u/v/p waves), so that most
-    // window shifts are register renames rather than moves. Groups start at rows that are multiples of the
-    // group size: a row is always computed by the same position of the group, whatever the
-    // block, so blocks of different heights (a batch case against a single context) give
-    // bit-identical results. The loop may thus start up to UNR - 1 rows before ilo - 6 and
-    // end up to UNR - 1 rows after ihi + 6; in those rows every stage, the P update and
+    // Rows run two at a time, so that half of the window shifts are register renames rather
+    // than moves. Groups start at even rows: a row is always computed by the same position of
+    // the group, whatever the block, so blocks of different heights (a batch case against a
+    // single context) give bit-identical results. The loop may thus start one row before
+    // ilo - 6 and end one row after ihi + 6; in those rows every stage, the P update and
     // every global store are outside their row ranges (only ring slots that no computed row
     // reads are written).
-    constexpr int UNR = (RINGS || sizeof(R) == 4) ? 2 : 4;
+    constexpr int UNR = 2;
+    // FP64 register-window form: the interior groups run with the row guards compiled out
+    // (below). Not in the ring form or in FP32, whose register budgets (two waves per SIMD,
+    // or two u/v/p waves and a T/C wave) the extra live values of the straight-line code
+    // would exceed.
+    constexpr bool PEEL = !RINGS && sizeof(R) == 8;
     const int t0 = (ilo - 6) - ((ilo - 6) & (UNR - 1));
     __syncthreads();
     fetch(t0);
 
-    auto row = [&](const int t) __attribute__((always_inline)) {
+    auto row = [&](const int t, auto fullc) __attribute__((always_inline)) {
+        // FULL: every row guard below holds (the rows of the group lie in [ilo + 7, min(im, ihi + 5)])
+        constexpr bool FULL = decltype(fullc)::value;
         const int m5 = (t + 40) % 5;     // slot of row t in the 5-row stage rings
         // slots of row t + d in the u_tau ring and in the stage rings
 #define UTS(d) (RINGS ? s8(t + (d)) : s4(t + (d)))
@@ -300,7 +311,8 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         // ---------------- phase A: rows t (iterates) and t-2 (pointwise inputs)
         // the defaults of rows and columns outside the arrays, applied where the values are
         // consumed (a select at the load would wait for it)
-        const bool in_u = t >= 1 && t <= im + 1, in_v = t >= 1 && t <= im, in_p = t - 2 >= 1 && t - 2 <= im;
+        const bool in_u = FULL || (t >= 1 && t <= im + 1), in_v = FULL || (t >= 1 && t <= im),
+                   in_p = FULL || (t - 2 >= 1 && t - 2 <= im);
         const bool pu = in_p && lu, pv = in_p && jvalid;
         const R ut_new = (in_u && lu) ? q_ut : R(0), vt_new = (in_v && jvalid) ? q_vt : R(0);
         const R T_rp = pu ? q_T : R(1), e = pu ? q_e : R(1), Lp = pu ? q_L : R(0);
@@ -328,7 +340,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         pw.push(p_rp);
         {
             R iev = R(1), k1 = R(0), k2 = R(0);
-            if (rp >= ilo - 4 && rp <= ihi + 4) {
+            if (FULL || (rp >= ilo - 4 && rp <= ihi + 4)) {
                 iev = e == R(1) ? R(1) : R(1) / e;
                 if (Lp != R(0)) {
                     k1 = e * Lp * R(cst[CST_INV_REDA]);
@@ -341,11 +353,17 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 nuw.push(e * inv_re * ((((R(cst[CST_MU0 + 4]) * tt + R(cst[CST_MU0 + 3])) * tt + R(cst[CST_MU0 + 2])) * tt
                                          + R(cst[CST_MU0 + 1])) * tt + R(cst[CST_MU0])));
             }
-            dpj.push(R(p_rp - sP[s2(rp)][PAD + x - 1]));
+            if (DLDS) sDUV[10 + s5(m5, -2)][x] = R(p_rp - sP[s2(rp)][PAD + x - 1]);
+            else dpj.push(R(p_rp - sP[s2(rp)][PAD + x - 1]));
             const R Tm = sT[s2(rp)][PAD + x - 1];
-            bw.push(bouss ? R(cst[CST_RI]) * ((T_rp + Tm) * R(0.5) - R(cst[CST_T_REF]))
-                          : R(cst[CST_INVFR2]) * (R(1) - R(2) / (T_rp + Tm)));
-            if (!STEADY) { du.push(um_rp - ut[2]); dv.push(vm_rp - vt[2]); }
+            const R bv = bouss ? R(cst[CST_RI]) * ((T_rp + Tm) * R(0.5) - R(cst[CST_T_REF]))
+                               : R(cst[CST_INVFR2]) * (R(1) - R(2) / (T_rp + Tm));
+            if (DLDS) sDUV[15 + s5(m5, -2)][x] = bv;
+            else bw.push(bv);
+            if (!STEADY) {
+                if (DLDS) { sDUV[s5(m5, -2)][x] = um_rp - ut[2]; sDUV[5 + s5(m5, -2)][x] = vm_rp - vt[2]; }
+                else { du.push(um_rp - ut[2]); dv.push(vm_rp - vt[2]); }
+            }
         }
 
         // ---------------- U stage K at row r = t - 2K (u update: i = 3..imax-1, j = 2..jmax-1)
@@ -355,8 +373,8 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             const int bk = 2 * K - 2; /* back index of row r in the rows t-6..t-2 windows */      \
             R val = R(0), rs = R(0);                                                              \
             bool interior = false;                                                                \
-            if (r >= ilo - 2 * (3 - K) && r <= ihi + 2 * (3 - K)) {                               \
-                if (r >= 3 && r <= im - 1) {                                                      \
+            if (FULL || (r >= ilo - 2 * (3 - K) && r <= ihi + 2 * (3 - K))) {                     \
+                if (FULL || (r >= 3 && r <= im - 1)) {                                            \
                     /* every lane computes (inputs outside the arrays hold finite defaults); */   \
                     /* columns outside the u update keep 0 */                                     \
                     {                                                                             \
@@ -392,7 +410,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                             const R vP = vt[2 * K];                                               \
                             Rr -= uP * (k1 + k2 * sqrt(uP * uP + vP * vP));                   \
                         }                                                                         \
-                        rs = ((STEADY ? R(0) : du[bk]) + Rr * dt) * dtau;                         \
+                        rs = ((STEADY ? R(0) : DLDS ? sDUV[s5(m5, -2 * K)][x] : du[bk]) + Rr * dt) * dtau; \
                         const R tau = ut[2 * K];                                                  \
                         R cv;                                                                     \
                         if (K == 1) cv = tau + rs;                                                \
@@ -408,7 +426,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 RING_OUT                                                                          \
             }                                                                                     \
             PUSH                                                                                  \
-            if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
+            if (K == 3 && interior && out_col && (FULL || (r >= ilo && r < ihi))) {               \
                 a.un[(size_t)r * su + j] = val;                                                   \
                 const unsigned long long bb = dbits((double)rs); mu_ = bb > mu_ ? bb : mu_; if (L2) qu_ += (double)rs * (double)rs;       \
             }                                                                                     \
@@ -421,8 +439,8 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             const int bk = 2 * K - 2;                                                             \
             R val = R(0), rs = R(0);                                                              \
             bool interior = false;                                                                \
-            if (r >= ilo - 2 * (3 - K) && r <= ihi - 1 + 2 * (3 - K)) {                           \
-                if (r >= 2 && r <= im - 1) {                                                      \
+            if (FULL || (r >= ilo - 2 * (3 - K) && r <= ihi - 1 + 2 * (3 - K))) {                 \
+                if (FULL || (r >= 2 && r <= im - 1)) {                                            \
                     /* every lane computes; columns outside the v update keep 0, the inflow */    \
                     /* columns j <= 2 take their boundary value below */                         \
                     {                                                                             \
@@ -440,7 +458,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         const R diff = (nu * ave_j) * (rdx_r * (vW - vP) + R(pr[4 * r + 6]) * (vE - vP)) \
                                      + (nu * avn_r) * (rdym_j * (vS - vP) + rdym_n * (vN - vP));  \
                         R pw_, pe_, ps_, pn_;                                                     \
-                        if (upwind_all || r == 2 || r == im - 1) {                                \
+                        if (upwind_all || (!FULL && (r == 2 || r == im - 1))) {                   \
                             pw_ = fw > R(0) ? vW : vP; pe_ = fe > R(0) ? vP : vE;                 \
                             ps_ = fs > R(0) ? vS : vP; pn_ = fn > R(0) ? vP : vN;                 \
                         } else {                                                                  \
@@ -451,13 +469,14 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                             ps_ = quick_face(fs, sS, cS, cPj); pn_ = quick_face(fn, sN, cPj, cN); \
                         }                                                                         \
                         const R conv = fw * pw_ - fe * pe_ + fs * ps_ - fn * pn_;                 \
-                        R Rr = rdy_j * (rdx_r * (diff + conv) - ev * dpj[bk]) + bw[bk];           \
+                        R Rr = rdy_j * (rdx_r * (diff + conv) - ev * (DLDS ? sDUV[10 + s5(m5, -2 * K)][x] : dpj[bk])) \
+                                 + (DLDS ? sDUV[15 + s5(m5, -2 * K)][x] : bw[bk]);                       \
                         const R k1 = k1w[bk], k2 = k2w[bk];                                       \
                         if (k1 != R(0) || k2 != R(0)) {                                           \
                             const R uP = ut[2 * K];                                               \
                             Rr -= vP * (k1 + k2 * sqrt(uP * uP + vP * vP));                   \
                         }                                                                         \
-                        rs = ((STEADY ? R(0) : dv[bk]) + Rr * dt) * dtau;                         \
+                        rs = ((STEADY ? R(0) : DLDS ? sDUV[5 + s5(m5, -2 * K)][x] : dv[bk]) + Rr * dt) * dtau; \
                         const R tau = vt[2 * K];                                                  \
                         R cv;                                                                     \
                         if (K == 1) cv = tau + rs;                                                \
@@ -465,7 +484,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         else cv = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (vP + rs);                  \
                         val = vcol ? cv : R(0);                                                   \
                         /* row 1 = row 2 (boundary.f90:34-36), in the v update's columns */       \
-                        if (r == 2 && !chan) { SET_ROW1; }                                        \
+                        if (!FULL && r == 2 && !chan) { SET_ROW1; }                               \
                     }                                                                             \
                 } else if (r == im) {                                                             \
                     val = vcol ? PREV : R(0);                                                     \
@@ -474,7 +493,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 RING_OUT                                                                          \
             }                                                                                     \
             PUSH                                                                                  \
-            if (K == 3 && interior && out_col && r >= ilo && r < ihi) {                           \
+            if (K == 3 && interior && out_col && (FULL || (r >= ilo && r < ihi))) {               \
                 a.vn[(size_t)r * sv + j] = val;                                                   \
                 const unsigned long long bb = dbits((double)rs); mv_ = bb > mv_ ? bb : mv_; if (L2) qv_ += (double)rs * (double)rs;       \
             }                                                                                     \
@@ -536,7 +555,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
         // ---------------- P at row r = t - 7 (equations.f90:620-658), FP64 accumulation
         {
             const int r = t - 7;
-            if (r >= 2 && r <= im - 1 && ucol && out_col && r >= ilo && r < ihi) {
+            if ((FULL || (r >= 2 && r <= im - 1 && r >= ilo && r < ihi)) && ucol && out_col) {
                 const double u3p = (double)w3[1], u3e = (double)w3[0];
                 const double v3p = (double)x3[1];
                 const double v3n = (double)lds_v(sV3[s2(r)], x, j, 1, jm, chan, vi);
@@ -551,13 +570,30 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
             }
         }
     };
-    for (int t = t0; t <= ihi + 6; t += UNR) {
-        row(t);
-        row(t + 1);
+    // Groups whose rows all lie in [ilo + 7, min(im, ihi + 5)] run the form with the row
+    // guards compiled out (every stage, the P update and the stores are in range there); the
+    // first and last groups of a block keep them. The arithmetic of a row is the same
+    // expressions in both forms (the file is compiled with -ffp-contract=on, so each
+    // expression's FMA contraction is fixed by the source), so a row gives the same bits
+    // whichever form computes it.
+    const int tfull_lo = ilo + 7, tfull_hi = min(im, ihi + 5);
+    using full_t = std::integral_constant<bool, true>;
+    using guarded_t = std::integral_constant<bool, false>;
+    auto group = [&](const int t, auto fullc) __attribute__((always_inline)) {
+        row(t, fullc);
+        row(t + 1, fullc);
         if (UNR == 4) {
-            row(t + 2);
-            row(t + 3);
+            row(t + 2, fullc);
+            row(t + 3, fullc);
         }
+    };
+    if (PEEL) {
+        int t = t0;
+        for (; t <= ihi + 6 && !(t >= tfull_lo && t + UNR - 1 <= tfull_hi); t += UNR) group(t, guarded_t{});
+        for (; t + UNR - 1 <= tfull_hi; t += UNR) group(t, full_t{});
+        for (; t <= ihi + 6; t += UNR) group(t, guarded_t{});
+    } else {
+        for (int t = t0; t <= ihi + 6; t += UNR) group(t, guarded_t{});
     }
     fold_max(a.res + 0, mu_);
     fold_max(a.res + 1, mv_);
