@@ -1113,15 +1113,18 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
     // reference into the argument block)
     const R v_i = R(a.v_i), u_in = R(a.u_in);
     R *const Tn_out = a.Tn, *const Cn_out = a.Cn;
-    auto row = [&](const int t, R &q_t, R &q_c, R &q_u, R &q_v, R &q_T, R &q_C, R &q_e, R &q_L)
+    auto row = [&](const int t, R &q_t, R &q_c, R &q_u, R &q_v, R &q_T, R &q_C, R &q_e, R &q_L, auto fullc)
                    __attribute__((always_inline)) {
+        // FULL: every row guard below holds (the rows of the pair lie in
+        // [max(ilo + 3, 6), min(im - 1, ihi + 2)])
+        constexpr bool FULL = decltype(fullc)::value;
         // ---------------- phase A: row t
-        const bool inr = jvalid && t >= 1 && t <= im;
+        const bool inr = jvalid && (FULL || (t >= 1 && t <= im));
         const R tnew = inr ? q_t : R(0), cnew = inr ? q_c : R(0), e = inr ? q_e : R(1), Lp = inr ? q_L : R(0);
         const R Tp = (!STEADY && inr) ? q_T : R(0), Cp = (!STEADY && inr) ? q_C : R(0);
         const R vnew = !inr ? R(0) : (j == 2 ? (chan ? R(0) : v_i)
-                                             : ((t >= 2 && t <= im - 1 && j >= 3 && j <= jm - 1) ? q_v : R(0)));
-        R unew = (jvalid && t > 2 && t < im && j >= 2 && j <= jm - 1) ? q_u : R(0);
+                                             : (((FULL || (t >= 2 && t <= im - 1)) && j >= 3 && j <= jm - 1) ? q_v : R(0)));
+        R unew = (jvalid && (FULL || (t > 2 && t < im)) && j >= 2 && j <= jm - 1) ? q_u : R(0);
         {   // row t + 2 into the set just consumed
             const int tn = t + 2;
             const size_t k_ = (size_t)min(max(tn, 1), im) * su + jc1;
@@ -1130,8 +1133,10 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
             q_v = a.vn[(size_t)min(max(tn, 2), im - 1) * sv + jvi];
             q_u = a.un[(size_t)min(max(tn, 3), im - 1) * su + jui];
         }
-        if (t <= 2) unew = chan ? u_in : R(0);         // u3 boundary rows (boundary.f90:29-32,39-42)
-        else if (t >= im) unew = chan ? uw[0] : R(0);
+        if (!FULL) {                                   // u3 boundary rows (boundary.f90:29-32,39-42)
+            if (t <= 2) unew = chan ? u_in : R(0);
+            else if (t >= im) unew = chan ? uw[0] : R(0);
+        }
         tt.push(tnew); ct.push(cnew); uw.push(unew); vw.push(vnew);
         sTT[s4(t)][PAD + x] = tnew;
         sCT[s4(t)][PAD + x] = cnew;
@@ -1156,8 +1161,8 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
             const int bq = K;            /* back index of row r in windows whose newest is t */   \
             R val = R(0);                                                                         \
             bool interior = false;                                                                \
-            if (r >= ilo - (3 - K) && r <= ihi - 1 + (3 - K)) {                                   \
-                if (r >= 2 && r <= im - 1) {                                                      \
+            if (FULL || (r >= ilo - (3 - K) && r <= ihi - 1 + (3 - K))) {                         \
+                if (FULL || (r >= 2 && r <= im - 1)) {                                            \
                     /* every lane computes; columns outside the update keep 0 */                 \
                     {                                                                             \
                         interior = scol;                                                          \
@@ -1182,7 +1187,7 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
                         else if (K == 2) cv = R(0.75) * tau + R(0.25) * (zP + rs);                \
                         else cv = R(1.0 / 3.0) * tau + R(2.0 / 3.0) * (zP + rs);                  \
                         val = scol ? cv : R(0);                                                   \
-                        if (r == 2) WOUT.set(0, scol ? ((ISC && chan) ? cslot : cv) : WOUT[0]);  /* row 1 */ \
+                        if (!FULL && r == 2) WOUT.set(0, scol ? ((ISC && chan) ? cslot : cv) : WOUT[0]); /* row 1 */ \
                     }                                                                             \
                 } else if (r == im) {                                                             \
                     val = scol ? WOUT[0] : R(0);                                                  \
@@ -1192,7 +1197,7 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
                 RINGW                                                                             \
             }                                                                                     \
             WOUT.push(val);                                                                       \
-            if (OUTP != nullptr && interior && out_col && r >= ilo && r < ihi)                    \
+            if (OUTP != nullptr && interior && out_col && (FULL || (r >= ilo && r < ihi)))        \
                 OUTP[(size_t)r * su + j] = val;                                                   \
         }
 
@@ -1205,9 +1210,25 @@ __device__ __forceinline__ void scal_tile(const K2Args<R> &a, int bi, int bj, Sm
         SCAL_STAGE(3, c2, sC2[s2(t - 3)], true, c3, {}, Cn_out, kc, dC, ct)
 #undef SCAL_STAGE
     };
-    for (int t = t0; t <= ihi + 2; t += 2) {
-        row(t, qa_t, qa_c, qa_u, qa_v, qa_T, qa_C, qa_e, qa_L);
-        row(t + 1, qb_t, qb_c, qb_u, qb_v, qb_T, qb_C, qb_e, qb_L);
+    // Steady problems: pairs whose rows all lie in [max(ilo + 3, 6), min(im - 1, ihi + 2)] run
+    // the form with the row guards compiled out; a row gives the same bits in both forms
+    // (same expressions, contraction fixed per expression). Unsteady problems keep the
+    // guarded form throughout: the straight-line code needs more registers than the room a
+    // T/C wave has beside a u/v/p wave in the two-stream step.
+    const int tfull_lo = max(ilo + 3, 6), tfull_hi = min(im - 1, ihi + 2);
+    auto pair = [&](const int t, auto fullc) __attribute__((always_inline)) {
+        row(t, qa_t, qa_c, qa_u, qa_v, qa_T, qa_C, qa_e, qa_L, fullc);
+        row(t + 1, qb_t, qb_c, qb_u, qb_v, qb_T, qb_C, qb_e, qb_L, fullc);
+    };
+    using full_t = std::integral_constant<bool, true>;
+    using guarded_t = std::integral_constant<bool, false>;
+    if (STEADY) {
+        int t = t0;
+        for (; t <= ihi + 2 && !(t >= tfull_lo && t + 1 <= tfull_hi); t += 2) pair(t, guarded_t{});
+        for (; t + 1 <= tfull_hi; t += 2) pair(t, full_t{});
+        for (; t <= ihi + 2; t += 2) pair(t, guarded_t{});
+    } else {
+        for (int t = t0; t <= ihi + 2; t += 2) pair(t, guarded_t{});
     }
 #undef K2_FETCH
 }
