@@ -107,6 +107,27 @@ __device__ __forceinline__ R quick_face(R f, R s, R ca, R cb)
     return R(0.5) * s - R(0.125) * (f > R(0) ? ca : cb);
 }
 
+// Square root of x >= 0 (a sum of squares, a porosity): the hardware reciprocal square root
+// r, then the coupled Newton iteration for g ~ sqrt(x) and h ~ 1/(2 sqrt(x)) (g = x r,
+// h = r / 2; e = 1/2 - h g; g += g e, h += h e; then twice d = x - g^2, g += d h). This is the
+// operation sequence of the compiler's FP64 square root for x >= 2^-767, which it takes without
+// rescaling the argument, so the result is the same there; for smaller x (velocities below
+// 1e-115) it can differ by a few ulps. x = 0 and x = +inf return x, NaN stays NaN.
+__device__ __forceinline__ double csqrt(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return __builtin_amdgcn_class(x, 0x260) ? x : g;   // +-0, +inf
+}
+__device__ __forceinline__ float csqrt(float x) { return sqrtf(x); }
+
 // One tile of a persistent batch: case index, row block, column strip.
 struct BatchTile {
     int c, bi, bj, pad;
@@ -344,7 +365,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                 iev = e == R(1) ? R(1) : R(1) / e;
                 if (Lp != R(0)) {
                     k1 = e * Lp * R(cst[CST_INV_REDA]);
-                    k2 = Lp * R(cst[CST_CFSD]) * sqrt(e);
+                    k2 = Lp * R(cst[CST_CFSD]) * csqrt(e);
                 }
             }
             ie.push(iev); ew.push(e); k1w.push(k1); k2w.push(k2);
@@ -408,7 +429,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         const R k1 = k1w[bk], k2 = k2w[bk];                                       \
                         if (k1 != R(0) || k2 != R(0)) {                                           \
                             const R vP = vt[2 * K];                                               \
-                            Rr -= uP * (k1 + k2 * sqrt(uP * uP + vP * vP));                   \
+                            Rr -= uP * (k1 + k2 * csqrt(uP * uP + vP * vP));                   \
                         }                                                                         \
                         rs = ((STEADY ? R(0) : DLDS ? sDUV[s5(m5, -2 * K)][x] : du[bk]) + Rr * dt) * dtau; \
                         const R tau = ut[2 * K];                                                  \
@@ -474,7 +495,7 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
                         const R k1 = k1w[bk], k2 = k2w[bk];                                       \
                         if (k1 != R(0) || k2 != R(0)) {                                           \
                             const R uP = ut[2 * K];                                               \
-                            Rr -= vP * (k1 + k2 * sqrt(uP * uP + vP * vP));                   \
+                            Rr -= vP * (k1 + k2 * csqrt(uP * uP + vP * vP));                   \
                         }                                                                         \
                         rs = ((STEADY ? R(0) : DLDS ? sDUV[5 + s5(m5, -2 * K)][x] : dv[bk]) + Rr * dt) * dtau; \
                         const R tau = vt[2 * K];                                                  \
@@ -758,7 +779,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                 iev = e == R(1) ? R(1) : R(1) / e;
                 if (Lp != R(0)) {
                     k1 = e * Lp * R(a.inv_reda);
-                    k2 = Lp * R(a.cfsd) * sqrt(e);
+                    k2 = Lp * R(a.cfsd) * csqrt(e);
                 }
             }
             ie.push(iev); ew.push(e); k1w.push(k1); k2w.push(k2);
@@ -820,7 +841,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                             R Rr = rdx_r * (rdy_j * (diff + conv) - ev * R(aux[bk] - aux[bk + 1])) - gg * ev; \
                             const R k1 = k1w[bk], k2 = k2w[bk];                                   \
                             if (k1 != R(0) || k2 != R(0))                                         \
-                                Rr -= uP * (k1 + k2 * sqrt(uP * uP + vr * vr));                   \
+                                Rr -= uP * (k1 + k2 * csqrt(uP * uP + vr * vr));                   \
                             rs = ((STEADY ? R(0) : dd[bk]) + Rr * dt) * dtau;                     \
                             const R tau = own[2 * K];                                             \
                             if (K == 1) val = tau + rs;                                           \
@@ -904,7 +925,7 @@ __device__ __forceinline__ void uvp_tile_split(const K1Args<R> &a, int bi, int b
                             R Rr = rdy_j * (rdx_r * (diff + conv) - ev * R(aux[bk])) + bw[bk];        \
                             const R k1 = k1w[bk], k2 = k2w[bk];                                   \
                             if (k1 != R(0) || k2 != R(0))                                         \
-                                Rr -= vP * (k1 + k2 * sqrt(ur * ur + vP * vP));                   \
+                                Rr -= vP * (k1 + k2 * csqrt(ur * ur + vP * vP));                   \
                             rs = ((STEADY ? R(0) : dd[bk]) + Rr * dt) * dtau;                     \
                             const R tau = own[2 * K];                                             \
                             if (K == 1) val = tau + rs;                                           \
