@@ -467,7 +467,7 @@ def bench_unet(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--case", default="full", choices=["full", "porous"],
                     help="full: full physics on a tanh mesh; porous: uniform mesh, porous block, buoyancy")
