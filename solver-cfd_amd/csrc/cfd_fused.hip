@@ -313,11 +313,11 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     // every global store are outside their row ranges (only ring slots that no computed row
     // reads are written).
     constexpr int UNR = 2;
-    // FP64 register-window form: the interior groups run with the row guards compiled out
-    // (below). Not in the ring form or in FP32, whose register budgets (two waves per SIMD,
-    // or two u/v/p waves and a T/C wave) the extra live values of the straight-line code
-    // would exceed.
-    constexpr bool PEEL = !RINGS && sizeof(R) == 8;
+    // Register-window form, FP64 or steady: the interior groups run with the row guards
+    // compiled out (below). Not in the ring form or in unsteady FP32, whose register budgets
+    // (two waves per SIMD, or two u/v/p waves and a T/C wave) the extra live values of the
+    // straight-line code would exceed.
+    constexpr bool PEEL = !RINGS && (sizeof(R) == 8 || STEADY);
     const int t0 = (ilo - 6) - ((ilo - 6) & (UNR - 1));
     __syncthreads();
     fetch(t0);
