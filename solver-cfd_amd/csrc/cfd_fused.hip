@@ -157,6 +157,7 @@ struct K1Args {
     double *pn;
     unsigned long long *res;                               // max |r3| bit patterns: u, v, p
     double *l2;                                            // sums of r3^2 (u, v, p) or null
+    unsigned long long *res_next;                          // next step's slots, cleared here, or null
     const double *pk;                                      // packed geometry (cfd_devfactors.pack)
     int imax, jmax, nbi, nbj, hb, chan, upwind_all, bouss;
     double dt, dtau, beta, inv_re, inv_reda, cfsd, g, invfr2, ri, t_ref, v_i, u_in;
@@ -635,6 +636,7 @@ __global__ __launch_bounds__(NT, OCC) void k_uvp(const K1Args<R> a)
     __shared__ Smem1<R, NT, (OCC > 1)> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;   // blocks of one XCD are neighbours
+    if (a.res_next && b == 0 && threadIdx.x < 3) a.res_next[threadIdx.x] = 0ull;
     if (L >= G) return;
     // issue priority over co-resident waves: in the two-stream step the T/C kernel of the
     // previous pseudo-step shares the SIMDs, and this kernel is the critical path
@@ -966,6 +968,7 @@ __global__ __launch_bounds__(2 * NT, 1) void k_uvp_split(const K1Args<R> a)
     __shared__ Smem1S<R, NT> sm;
     const int G = a.nbi * a.nbj, per = (G + 7) / 8;
     const int b = blockIdx.x, L = (b % 8) * per + b / 8;
+    if (a.res_next && b == 0 && threadIdx.x < 3) a.res_next[threadIdx.x] = 0ull;
     if (L >= G) return;
     __builtin_amdgcn_s_setprio(3);      // as k_uvp
     uvp_tile_split<R, STEADY, MUPOLY, NT, L2>(a, L / a.nbj, L % a.nbj, sm);
@@ -1546,6 +1549,7 @@ void fill_k1(K1Args<R> &k1, int im, int jm, const double *pack, const cfd_params
     k1.eps = (const R *)a->epsr; k1.lp = (const R *)a->lpr;
     k1.un = (R *)a->um_n; k1.vn = (R *)a->vm_n; k1.pn = a->pn; k1.res = (unsigned long long *)a->res_max;
     k1.l2 = a->res_l2;
+    k1.res_next = (unsigned long long *)a->res_next;
     k1.pk = pack;
     k1.imax = im; k1.jmax = jm;
     k1.chan = P->bc == CFD_BC_CHANNEL; k1.upwind_all = P->scheme == CFD_SCHEME_UPWIND;

@@ -632,8 +632,9 @@ static void tau_swap(cfd_ctx *c)
     c->pending_in_tau = 1;
 }
 
-/* One pseudo-step (main.f90:122-142) through the per-operator kernels; residuals into slot[0..2]. */
-static int step_exact(cfd_ctx *c, uint64_t *slot)
+/* One pseudo-step (main.f90:122-142) through the per-operator kernels; residuals into slot[0..2].
+ * `next` (or NULL): the next step's residual slots, cleared here too. */
+static int step_exact(cfd_ctx *c, uint64_t *slot, uint64_t *next)
 {
     xenv e = {&c->g, &c->P, c->s, c->l2_on ? c->d_l2 : NULL};
     const int st = c->P.steady;
@@ -658,6 +659,7 @@ static int step_exact(cfd_ctx *c, uint64_t *slot)
         }
     }
     CHK(cfdk_zero_slots(slot, 3, e.l2, c->s));
+    if (next) CHK(cfdk_zero_slots(next, 3, NULL, c->s));
     CHK(x_solve_u(&e, um, S[S_UMT], S[S_VMT], S[S_UMN], p_uv, T, X[X_UI], X[X_RU], X[X_RESU], slot + 0));
     CHK(x_solve_v(&e, vm, S[S_UMT], S[S_VMT], S[S_VMN], p_uv, T, X[X_VI], X[X_RV], X[X_RESV], slot + 1));
     CHK(x_solve_p(&e, S[S_P], S[S_UMN], S[S_VMN], S[S_PN], X[X_RP], X[X_PI], X[X_RESP], slot + 2));
@@ -682,7 +684,7 @@ static int pipe_join(cfd_ctx *c)
 
 /* Fused step: boundary-consistent iterates in, outputs with boundaries written last.
  * With `timed`, the context's per-kernel events bracket each launch. */
-static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
+static int step_fused_ev(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre, int timed)
 {
     const int st = c->P.steady;
     double **S = c->st;
@@ -722,7 +724,15 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     } else {
         CHK(pipe_join(c));
     }
-    CHK(cfdk_zero_slots(slot, 3, a.res_l2, c->s));
+    /* Residual slots: with `pre` the caller has cleared `slot`, and the u/v/p kernel clears
+     * `next` for the step after this one (no separate launch); otherwise (the L2 sums, timed
+     * steps, callers that pass no `next`) a clearing launch precedes the step. */
+    if (pre && next && !a.res_l2 && !timed) {
+        a.res_next = next;
+    } else {
+        CHK(cfdk_zero_slots(slot, 3, a.res_l2, c->s));
+        if (next) CHK(cfdk_zero_slots(next, 3, NULL, c->s));
+    }
     CHK(cfdk_fused_step(&c->g, &c->fac, &c->P, &a, c->kernel, c->s));
     if (two) {
         const int q = (int)(c->pipe_n & 1);
@@ -735,7 +745,10 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, int timed)
     return CFD_OK;
 }
 
-static int step_fused(cfd_ctx *c, uint64_t *slot) { return step_fused_ev(c, slot, 0); }
+static int step_fused(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre)
+{
+    return step_fused_ev(c, slot, next, pre, 0);
+}
 
 /* Float copies of the iterates the fused FP32 step reads (the physical level, the tau
  * level), and of the porosity fields when they changed. */
@@ -783,13 +796,16 @@ static int fused_ok(const cfd_ctx *c)
         && c->P.b_art == 0.0 && c->P.compat == CFD_COMPAT_FORTRAN;
 }
 
-static int one_step(cfd_ctx *c, uint64_t *slot)
+/* One pseudo-step with residuals into slot[0..2]. pre: the caller has cleared `slot`;
+ * next (with pre): the slots of the step after this one, which this step clears. Without
+ * pre, the step clears `slot` itself. */
+static int one_step(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre)
 {
     if (fused_ok(c))
-        return step_fused(c, slot);
+        return step_fused(c, slot, next, pre);
     CHK(pipe_join(c));
     CHK(materialize(c));
-    return step_exact(c, slot);
+    return step_exact(c, slot, next);
 }
 
 static double u64_to_res(uint64_t b)
@@ -811,12 +827,16 @@ int cfd_pseudo_steps(cfd_ctx *c, int32_t nsteps, double res[3], double *history)
     CHK(check_ready(c));
     if (nsteps < 0) return fail(CFD_ERR_ARG, "nsteps < 0");
     if (nsteps == 0) return CFD_OK;
-    const int64_t nslots = history ? nsteps : 1;
+    /* with a history every step has its own slots; otherwise two sets alternate, each step
+     * clearing the other set for the step after it */
+    const int64_t nslots = history ? nsteps : 2;
     CHK(ensure_hist(c, nslots));
+    CHK(cfdk_zero_slots(c->d_hist, (int)(3 * nslots), NULL, c->s));
     HIPCHK(hipEventRecord(c->ev0, c->s));
     for (int32_t k = 0; k < nsteps; ++k) {
-        uint64_t *slot = c->d_hist + 3 * (history ? k : 0);
-        CHK(one_step(c, slot));
+        uint64_t *slot = c->d_hist + 3 * (history ? k : (k & 1));
+        uint64_t *next = history ? NULL : c->d_hist + 3 * ((k + 1) & 1);
+        CHK(one_step(c, slot, next, 1));
         tau_swap(c);
     }
     CHK(pipe_join(c));
@@ -826,7 +846,7 @@ int cfd_pseudo_steps(cfd_ctx *c, int32_t nsteps, double res[3], double *history)
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms_per_step = (double)ms / nsteps;
-    const uint64_t *last = c->h_hist + 3 * (nslots - 1);
+    const uint64_t *last = c->h_hist + 3 * (history ? nslots - 1 : ((nsteps - 1) & 1));
     if (res) for (int q = 0; q < 3; ++q) res[q] = u64_to_res(last[q]);
     if (history) for (int64_t k = 0; k < 3 * nslots; ++k) history[k] = u64_to_res(c->h_hist[k]);
     for (int q = 0; q < 3; ++q)
@@ -845,8 +865,9 @@ int cfd_run_to_tol(cfd_ctx *c, double tol, int64_t itc_max, int32_t check_every,
     HIPCHK(hipEventRecord(c->ev0, c->s));
     while (done < itc_max) {
         int32_t n = (int32_t)((itc_max - done) < check_every ? (itc_max - done) : check_every);
+        CHK(cfdk_zero_slots(c->d_hist, 3 * n, NULL, c->s));
         for (int32_t k = 0; k < n; ++k) {
-            CHK(one_step(c, c->d_hist + 3 * k));
+            CHK(one_step(c, c->d_hist + 3 * k, NULL, 1));
             tau_swap(c);
         }
         CHK(pipe_join(c));
@@ -901,7 +922,7 @@ int cfd_physical_step(cfd_ctx *c, double eps, int64_t itc_max, int32_t first, in
     int64_t itc = first ? 1 : 0, taken = 0;
     double r[3] = {0, 0, 0};
     while (itc < itc_max) {
-        CHK(one_step(c, c->d_hist));
+        CHK(one_step(c, c->d_hist, NULL, 0));
         ++taken;
         CHK(pipe_join(c));
         HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
@@ -947,7 +968,7 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
     if (steps_per_graph < 2 || (steps_per_graph & 1)) return fail(CFD_ERR_ARG, "steps_per_graph must be even and >= 2");
     if (!c->owns_stream) return fail(CFD_ERR_STATE, "graph capture needs a context stream (not the legacy default stream)");
     if (replays < 1) return fail(CFD_ERR_ARG, "replays must be >= 1");
-    CHK(ensure_hist(c, 1));
+    CHK(ensure_hist(c, 2));
     /* A replay repeats the captured launches with the pointers fixed at capture, so every
      * captured step must be one that the next step would repeat as it is. Steps that are
      * not run here, outside the graph: the first step after new inputs (exact path, with
@@ -955,7 +976,7 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
      * after a physical step stopped without the tau update. One exact step leaves the
      * iterates boundary-consistent, reading p, with the first-step seeding done. */
     if (c->first_step || c->uv_reads_pn || !c->bc_clean) {
-        CHK(one_step(c, c->d_hist));
+        CHK(one_step(c, c->d_hist, NULL, 0));
         tau_swap(c);
     }
     if (c->first_step || c->uv_reads_pn || !c->bc_clean)
@@ -975,10 +996,14 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
     memcpy(s320, c->s32, sizeof s320);
     const int64_t nf0 = c->n_fused, ne0 = c->n_exact, pipe0 = c->pipe_n;
     const int pend0 = c->pending_in_tau;
+    /* two slot sets alternate inside the graph (an even number of steps), each step clearing
+     * the other set for the step after it: the last step of a replay clears set 0 for the
+     * first step of the next; both are cleared here, outside the graph, for the first replay */
+    CHK(cfdk_zero_slots(c->d_hist, 6, NULL, c->s));
     HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     int r = CFD_OK;
     for (int32_t k = 0; k < steps_per_graph && r == CFD_OK; ++k) {
-        r = one_step(c, c->d_hist);
+        r = one_step(c, c->d_hist + 3 * (k & 1), c->d_hist + 3 * ((k + 1) & 1), 1);
         if (r == CFD_OK) tau_swap(c);
     }
     if (r == CFD_OK) r = pipe_join(c);    /* the second stream rejoins before the capture ends */
@@ -999,7 +1024,8 @@ int cfd_pseudo_steps_graph(cfd_ctx *c, int32_t steps_per_graph, int32_t replays,
     HIPCHK(hipEventRecord(c->ev0, c->s));
     for (int32_t q = 0; q < replays && e == hipSuccess; ++q) e = hipGraphLaunch(exec, c->s);
     hipEventRecord(c->ev1, c->s);
-    hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s);
+    /* the last captured step (odd) wrote set 1 */
+    hipMemcpyAsync(c->h_hist, c->d_hist + 3, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s);
     hipError_t e2 = hipStreamSynchronize(c->s);
     hipGraphExecDestroy(exec);
     hipGraphDestroy(graph);
@@ -1091,11 +1117,12 @@ int cfd_ctx_get_tiles(const cfd_ctx *c, cfd_tiles *t)
 static int time_tiles(cfd_ctx *c, const cfd_tiles *t, int32_t steps, double *ms_out)
 {
     c->tiles = *t;
-    CHK(step_fused(c, c->d_hist));
+    CHK(step_fused(c, c->d_hist, NULL, 0));
     tau_swap(c);
+    CHK(cfdk_zero_slots(c->d_hist, 6, NULL, c->s));
     HIPCHK(hipEventRecord(c->ev0, c->s));
     for (int32_t k = 0; k < steps; ++k) {
-        CHK(step_fused(c, c->d_hist));
+        CHK(step_fused(c, c->d_hist + 3 * (k & 1), c->d_hist + 3 * ((k + 1) & 1), 1));
         tau_swap(c);
     }
     CHK(pipe_join(c));
@@ -1155,7 +1182,7 @@ int cfd_ctx_autotune(cfd_ctx *c, int32_t steps, cfd_tiles *out, double *out_ms)
         r = cfdk_copy(snap + off[k], c->st[k], (long)kind_count(c, kinds[k]), c->s);
     /* the fused kernels need boundary-consistent iterates: one exact step if not there yet */
     if (r == CFD_OK && !fused_ok(c)) {
-        r = step_exact(c, c->d_hist);
+        r = step_exact(c, c->d_hist, NULL);
         if (r == CFD_OK) tau_swap(c);
     }
     const int rows = c->P.imax - 2;
@@ -1213,7 +1240,7 @@ int cfd_pseudo_steps_timed(cfd_ctx *c, int32_t nsteps, double res[3], double ms[
     int32_t nf = 0;
     for (int32_t k = 0; k < nsteps; ++k) {
         if (fused_ok(c)) {
-            CHK(step_fused_ev(c, c->d_hist, 1));
+            CHK(step_fused_ev(c, c->d_hist, NULL, 0, 1));
             tau_swap(c);
             HIPCHK(hipEventSynchronize(c->evk[3]));
             float t[3];
@@ -1221,7 +1248,7 @@ int cfd_pseudo_steps_timed(cfd_ctx *c, int32_t nsteps, double res[3], double ms[
             acc[0] += t[0]; acc[1] += t[1]; acc[2] += t[2]; acc[3] += (double)t[0] + t[1] + t[2];
             ++nf;
         } else {
-            CHK(one_step(c, c->d_hist));
+            CHK(one_step(c, c->d_hist, NULL, 0));
             tau_swap(c);
         }
     }

@@ -73,6 +73,10 @@ typedef struct cfd_fused_args {
      * when res_l2 is not NULL, the sums of r3^2 over the same entries (3 doubles) */
     uint64_t *res_max;
     double *res_l2;
+    /* if not NULL, the u/v/p kernel writes zeros to res_next[0..2]: the residual slots of the
+     * next pseudo-step (never res_max itself), so that no separate clearing launch precedes
+     * that step */
+    uint64_t *res_next;
     int32_t prec;
     /* row-block heights of K1 / K2 (0: built-in heuristic) and the register-budget variants
      * (occ1 in {1, 2} waves per SIMD or 3 = split u/v chains, occ2 in {1, 4} waves per SIMD or
