@@ -827,6 +827,7 @@ int cfd_pseudo_steps(cfd_ctx *c, int32_t nsteps, double res[3], double *history)
     CHK(check_ready(c));
     if (nsteps < 0) return fail(CFD_ERR_ARG, "nsteps < 0");
     if (nsteps == 0) return CFD_OK;
+    if (history && nsteps > INT32_MAX / 3) return fail(CFD_ERR_ARG, "nsteps too large for a residual history");
     /* with a history every step has its own slots; otherwise two sets alternate, each step
      * clearing the other set for the step after it */
     const int64_t nslots = history ? nsteps : 2;
