@@ -724,10 +724,11 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre, in
     } else {
         CHK(pipe_join(c));
     }
-    /* Residual slots: with `pre` the caller has cleared `slot`, and the u/v/p kernel clears
-     * `next` for the step after this one (no separate launch); otherwise (the L2 sums, timed
-     * steps, callers that pass no `next`) a clearing launch precedes the step. */
-    if (pre && next && !a.res_l2 && !timed) {
+    /* Residual slots: with `pre` the caller (or the step before) has cleared `slot`, and the
+     * u/v/p kernel clears `next` (if any) for the step after this one: no separate launch.
+     * Otherwise (the L2 sums, timed steps, callers without `pre`) a clearing launch precedes
+     * the step, and clears `next` too. */
+    if (pre && !a.res_l2 && !timed) {
         a.res_next = next;
     } else {
         CHK(cfdk_zero_slots(slot, 3, a.res_l2, c->s));
