@@ -601,13 +601,10 @@ __device__ __forceinline__ void uvp_tile(const K1Args<R> &a, int bi, int bj, Sme
     const int tfull_lo = ilo + 7, tfull_hi = min(im, ihi + 5);
     using full_t = std::integral_constant<bool, true>;
     using guarded_t = std::integral_constant<bool, false>;
+    static_assert(UNR == 2, "a group is a row pair");
     auto group = [&](const int t, auto fullc) __attribute__((always_inline)) {
         row(t, fullc);
         row(t + 1, fullc);
-        if (UNR == 4) {
-            row(t + 2, fullc);
-            row(t + 3, fullc);
-        }
     };
     if (PEEL) {
         int t = t0;
