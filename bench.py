@@ -475,7 +475,7 @@ def main():
     ap.add_argument("--jmax", type=int, default=2048)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--kernel", type=int, default=1, help="0: per-operator path, 1: fused")
-    ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: eager)")
+    ap.add_argument("--graph", type=int, default=100, help="steps per captured hipGraph (0: eager launches)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ref-steps", type=int, default=60,
@@ -555,8 +555,8 @@ def main():
     log("warmup done (%d steps), timing %d steps" % (args.warmup, args.steps))
 
     t0 = time.perf_counter()
-    if args.graph:
-        reps = max(1, args.steps // args.graph)
+    if args.graph and args.steps >= args.graph:
+        reps = args.steps // args.graph
         res = ctx.pseudo_steps_graph(args.graph, reps)
         steps = reps * args.graph
     else:
