@@ -278,7 +278,8 @@ def bench_warmstart(args):
     test_cases = [cs[i] for i in test_idx]
     states = [WS.state_from_prediction(pred[k], H, H) for k in range(len(test_idx))]
     # held-out cases from the prediction (fallback to zero fields) and from zero fields
-    solw = WS.solve(test_cases, tol, args.ws_itc, states=states, fallback_factor=10.0, log=log)
+    stall = (args.ws_stall_window, args.ws_stall_ratio) if args.ws_stall_window > 0 else None
+    solw = WS.solve(test_cases, tol, args.ws_itc, states=states, fallback_factor=10.0, stall=stall, log=log)
     solz = WS.solve(test_cases, tol, args.ws_itc, log=log)
     itw = solw["iters32"] + solw["iters64"]; itz = solz["iters32"] + solz["iters64"]
     # the plain FP64 path on the held-out cases: one context per case from zero fields, run
@@ -313,7 +314,8 @@ def bench_warmstart(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (U-Net) + f32/f64 (solver)",
         "data": "synthetic (solver-generated steady states, seeded cases)",
         "config": {"workload": "U-Net warm start, %dx%d grids" % (H, H), "train": len(train_idx),
-                   "held_out": len(test_idx), "tol": tol, "train_tol": args.ws_train_tol, "epochs": args.ws_epochs},
+                   "held_out": len(test_idx), "tol": tol, "train_tol": args.ws_train_tol, "epochs": args.ws_epochs,
+                   "stall_test": list(stall) if stall else None},
         "train_loss_first_last": [hist[0], hist[-1]], "train_s": t_train,
         "prediction_rel_rms_err": pred_err,
         "iters_median_zero": float(np.median(itz)), "iters_median_warm": float(np.median(itw)),
@@ -503,6 +505,10 @@ def main():
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
     ap.add_argument("--ws-size", type=int, default=64, help="grid of the warm-start and FP32-accuracy workloads")
+    ap.add_argument("--ws-stall-window", type=int, default=0,
+                    help="warm start: restart from zero fields a case whose residual has not fallen below "
+                         "--ws-stall-ratio x its value this many pseudo-steps earlier (0: off)")
+    ap.add_argument("--ws-stall-ratio", type=float, default=0.5)
     ap.add_argument("--unet-size", type=int, default=256, help="input grid of the U-Net training-step workload")
     ap.add_argument("--ws-epochs", type=int, default=30)
     ap.add_argument("--ws-itc", type=int, default=60000)

@@ -357,6 +357,10 @@ int  cfd_batch_copy_state(cfd_batch *dst, cfd_batch *src);
  * replays it; 0: the same launches are issued directly each round (same kernels, same
  * results; for profilers and callers that do not use graphs). */
 int  cfd_batch_set_graph(cfd_batch *b, int32_t use_graph);
+/* Stops case c between runs with status `status` (5 or above; 5 is used for a case the caller
+ * judged stalled): a later cfd_batch_resume leaves it stopped, a new cfd_batch_run restarts
+ * it. Lets a caller apply its own stop tests between the slices of a long solve. */
+int  cfd_batch_stop_case(cfd_batch *b, int32_t c, int32_t status);
 /* Wall time of the last run (HIP events on the batch stream), graph steps replayed, and
  * cell-updates done (sum over cases of pseudo-steps x imax x jmax). */
 int  cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates);

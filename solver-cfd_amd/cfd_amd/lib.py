@@ -28,7 +28,7 @@ EXPORTS_CFD = [
     "cfd_ctx_set_tiles", "cfd_ctx_get_tiles", "cfd_ctx_autotune", "cfd_pseudo_steps_timed",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_mesh", "cfd_batch_upload", "cfd_batch_download",
     "cfd_batch_apply_bc", "cfd_batch_set_tiles", "cfd_batch_get_tiles", "cfd_batch_autotune", "cfd_batch_run",
-    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_batch_set_graph", "cfd_mesh_reference", "cfd_porous_flags_reference",
+    "cfd_batch_last_timing", "cfd_batch_copy_state", "cfd_batch_resume", "cfd_batch_set_graph", "cfd_batch_stop_case", "cfd_mesh_reference", "cfd_porous_flags_reference",
     "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_num_params", "cfd_unet_get_params", "cfd_unet_set_params",
     "cfd_unet_predict", "cfd_unet_loss_grad", "cfd_unet_train_step", "cfd_unet_last_timing", "cfd_nn_conv3x3", "cfd_nn_conv3x3_wgrad",
     "cfd_nn_flip_weights", "cfd_nn_maxpool2", "cfd_nn_maxpool2_bwd", "cfd_nn_upsample2", "cfd_nn_upsample2_bwd",
@@ -127,6 +127,7 @@ def load():
         "cfd_batch_last_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "cfd_batch_copy_state": (ctypes.c_int, [_vp, _vp]),
         "cfd_batch_set_graph": (ctypes.c_int, [_vp, i32]),
+        "cfd_batch_stop_case": (ctypes.c_int, [_vp, i32, i32]),
         "cfd_batch_resume": (ctypes.c_int, [_vp, dbl, i64, i32, dbl, ctypes.POINTER(i64), _dp, ctypes.POINTER(i32)]),
         "cfd_mesh_reference": (ctypes.c_int, [i32, i32, dbl, dbl, dbl, dbl, dbl, _dp, _dp]),
         "cfd_porous_flags_reference": (ctypes.c_int, [i32, i32, _dp, _dp, dbl, dbl, dbl, _dp, _dp,
@@ -439,15 +440,24 @@ class Batch:
         """True (default): runs replay a captured hipGraph; False: direct launches."""
         check(load().cfd_batch_set_graph(self.h, 1 if on else 0))
 
+    def stop_case(self, c, status=5):
+        """Stops case c (status >= 5) until the next run(); resume() leaves it stopped."""
+        check(load().cfd_batch_stop_case(self.h, int(c), int(status)))
+
     def copy_state_from(self, other):
         """Take every case's current iterates from another batch of the same cases
         (precision converted on the device)."""
         check(load().cfd_batch_copy_state(self.h, other.h))
 
-    def run_sliced(self, tol, itc_max, slice_steps=10000, steps_per_graph=500, diverge_factor=0.0, log=None):
+    def run_sliced(self, tol, itc_max, slice_steps=10000, steps_per_graph=500, diverge_factor=0.0, log=None,
+                   stall=None):
         """run() in slices of slice_steps (progress through `log`); returns the final
-        (code, iters, res, status) and the summed event time in ms."""
+        (code, iters, res, status) and the summed event time in ms. stall = (window, ratio):
+        between slices, a running case whose max(res_u, res_v, res_p) has not fallen below
+        ratio x its value `window` pseudo-steps earlier (window a multiple of slice_steps) is
+        stopped with status 5 (stalled)."""
         total_ms, done, first = 0.0, 0, True
+        seen = {}                       # slice end -> per-case max residual
         while True:
             done = min(itc_max, done + slice_steps)
             out = self.run(tol, done, steps_per_graph, diverge_factor, resume=not first)
@@ -455,10 +465,19 @@ class Batch:
             ms, steps, cells = self.last_timing()
             total_ms += ms
             code, it, res, st = out
+            if stall is not None and done < itc_max:
+                window, ratio = stall
+                now = res.max(axis=1)
+                seen[done] = now
+                before = seen.get(done - window)
+                if before is not None:
+                    for c in np.nonzero((st == 4) & ~(now < ratio * before))[0]:
+                        self.stop_case(int(c), 5)
+                        st[c] = 5
             if log:
-                log("  %d steps: %d running, %d converged, %d failed, %.1f s"
+                log("  %d steps: %d running, %d converged, %d failed, %d stalled, %.1f s"
                     % (done, int(((st == 0) | (st == 4)).sum()), int((st == 1).sum()), int(((st == 2) | (st == 3)).sum()),
-                       total_ms * 1e-3))
+                       int((st == 5).sum()), total_ms * 1e-3))
             if done >= itc_max or not ((st == 4) | (st == 0)).any():
                 return out, total_ms
 

@@ -6,7 +6,9 @@ Pipeline (all GPU work through libcfd_amd.so: the batched solver and the U-Net k
   3. U-Net training on (mask, porosity, log Re, log Da, Ri) -> (u, v, p, T);
   4. inference gives each new case an initial state; the solver iterates from it to the
      tolerance, and a case whose residual grows past `fallback_factor` x its best (the
-     batch's divergence test) is restarted from the zero initial state.
+     batch's divergence test), or optionally stalls (no fall below a ratio of its value a
+     window of steps earlier, tested between slices), is restarted from the zero initial
+     state.
 
 Field mapping on the p-cell index grid (i = 1..H, j = 1..W): channel 0 is um(i, j), 1 is
 vm(i, j), 2 is p(i, j), 3 is T(i, j); um(imax+1, .) and vm(., jmax+1) copy their neighbours,
@@ -66,18 +68,21 @@ def build_batch(cases, prec, states=None):
     return b
 
 
-def solve(cases, tol, itc_max, states=None, switch_tol=5e-7, fallback_factor=0.0, log=None):
+def solve(cases, tol, itc_max, states=None, switch_tol=5e-7, fallback_factor=0.0, stall=None, log=None):
     """Steady solve of every case: FP32 batch down to max(tol, switch_tol), FP64 finish to
-    tol when tol is below what FP32 storage resolves (switch_tol). With fallback_factor > 0, cases whose residual grows past that factor x their best
-    (from a warm start) are restarted from their zero initial state. Returns a dict with
-    per-case iterations (FP32 + FP64), status, event time (s), and the final FP64 batch."""
+    tol when tol is below what FP32 storage resolves (switch_tol). With fallback_factor > 0,
+    cases started from `states` whose residual grows past that factor x their best, or (stall
+    = (window, ratio)) does not fall below ratio x its value `window` steps earlier, are
+    restarted from their zero initial state. Returns a dict with per-case iterations (FP32 +
+    FP64), status, event time (s), and the final FP64 batch."""
     b32 = build_batch(cases, PREC_FP32, states)
+    warm = fallback_factor > 0 and states is not None
     (code, it32, res32, st32), ms32 = b32.run_sliced(max(tol, switch_tol), itc_max, diverge_factor=fallback_factor,
-                                                    log=log)
+                                                    log=log, stall=stall if warm else None)
     fallback = np.zeros(len(cases), bool)
     ms_fb = 0.0
-    if fallback_factor > 0 and states is not None:
-        bad = np.nonzero((st32 == 2) | (st32 == 3))[0]
+    if warm:
+        bad = np.nonzero((st32 == 2) | (st32 == 3) | (st32 == 5))[0]
         if bad.size:
             fallback[bad] = True
             sub = [cases[i] for i in bad]

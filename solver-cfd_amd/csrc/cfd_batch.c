@@ -631,6 +631,21 @@ int cfd_batch_set_graph(cfd_batch *b, int32_t use_graph)
     return CFD_OK;
 }
 
+int cfd_batch_stop_case(cfd_batch *b, int32_t c, int32_t status)
+{
+    if (!b) return cfd__fail(CFD_ERR_STATE, "no batch");
+    if (c < 0 || c >= b->n) return cfd__fail(CFD_ERR_ARG, "case %d out of range", c);
+    if (status < 5) return cfd__fail(CFD_ERR_ARG, "status %d: codes below 5 are the run's own", status);
+    int32_t ctl = 0;
+    hipError_t e = hipStreamSynchronize(b->s);
+    if (e == hipSuccess) e = hipMemcpy(&ctl, b->ctl + c, sizeof ctl, hipMemcpyDeviceToHost);
+    ctl &= ~1;
+    if (e == hipSuccess) e = hipMemcpy(b->ctl + c, &ctl, sizeof ctl, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->status + c, &status, sizeof status, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch stop: %s", hipGetErrorString(e));
+    return CFD_OK;
+}
+
 int cfd_batch_last_timing(const cfd_batch *b, double *ms, int64_t *steps, int64_t *cell_updates)
 {
     if (!b) return cfd__fail(CFD_ERR_STATE, "no batch");

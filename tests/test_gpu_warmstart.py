@@ -85,3 +85,28 @@ def test_unusable_start_falls_back_to_zero_init(trained):
     assert sol["fallback"].all()
     assert (sol["status"] == 1).all()
     sol["batch"].close()
+
+
+def test_stalled_start_falls_back_to_zero_init(trained):
+    """The stall test of the warm start (run_sliced's stall=(window, ratio), applied between
+    slices of 10 000 steps) with a ratio no residual can meet: every case still running after
+    20 000 steps is stopped and restarted from its zero initial state. Started from that same
+    zero state, a case that converges within 20 000 steps is untouched and takes exactly the
+    zero-start steps; every other one takes exactly 20 000 + the zero-start steps (the batch
+    kernels give each case the same bits whatever batch it is in)."""
+    cs, te, pred, Y, hist = trained
+    cases = [cs[i] for i in te[:12]]
+    solz = WS.solve(cases, TOL_SOLVE, ITC)
+    itz = solz["iters32"] + solz["iters64"]
+    solz["batch"].close()
+    sols = WS.solve(cases, TOL_SOLVE, ITC, states=[c["state"] for c in cases], fallback_factor=1e30,
+                    stall=(10000, 1e-30))
+    its = sols["iters32"] + sols["iters64"]
+    say("zero-start steps %s; with the stall test %s; fallbacks %s" % (itz.tolist(), its.tolist(),
+                                                                       sols["fallback"].astype(int).tolist()))
+    assert (solz["status"] == 1).all() and (sols["status"] == 1).all()
+    late = itz > 20000
+    assert late.any()
+    assert np.array_equal(sols["fallback"], late)
+    assert np.array_equal(its, np.where(late, itz + 20000, itz))
+    sols["batch"].close()
