@@ -456,6 +456,8 @@ class Batch:
         between slices, a running case whose max(res_u, res_v, res_p) has not fallen below
         ratio x its value `window` pseudo-steps earlier (window a multiple of slice_steps) is
         stopped with status 5 (stalled)."""
+        if stall is not None and (stall[0] <= 0 or stall[0] % slice_steps):
+            raise ValueError("stall window %d must be a positive multiple of slice_steps %d" % (stall[0], slice_steps))
         total_ms, done, first = 0.0, 0, True
         seen = {}                       # slice end -> per-case max residual
         while True:
