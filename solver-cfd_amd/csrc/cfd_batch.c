@@ -534,6 +534,8 @@ static int batch_run(cfd_batch *b, int resume, double tol, int64_t itc_max, int3
         free(ctl); free(st);
         if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch resume: %s", hipGetErrorString(e));
     }
+    /* the slots of the first step; each step's exit test clears them for the next */
+    BCHK(cfdk_batch_clear(&L, b->s));
     /* capture steps_per_graph batched steps + the active count (eager mode launches the
      * same sequence directly each round) */
     hipGraph_t graph = NULL;

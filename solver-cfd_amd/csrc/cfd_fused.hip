@@ -1839,7 +1839,9 @@ __global__ void k_batch_zero(unsigned long long *res, const int *ctl, int n)
 
 // Exit test of main.f90:139-150 per case: err = max(u, v, p); stop without the tau update
 // when itc != 1 and err < tol; otherwise swap the tau roles (parity bit) and go on.
-__global__ void k_batch_check(int *ctl, const unsigned long long *res, int *status, long long *itc, double *best,
+// The residual slots of a case that stays active are cleared here, after they are read, for
+// its next pseudo-step (a run clears the slots of its active cases once before its first step).
+__global__ void k_batch_check(int *ctl, unsigned long long *res, int *status, long long *itc, double *best,
                               double *last, int n, double tol, double diverge, long long itc_max)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1868,6 +1870,7 @@ __global__ void k_batch_check(int *ctl, const unsigned long long *res, int *stat
         else if (k >= itc_max) { status[c] = 4; cs &= ~1; }
     }
     ctl[c] = cs;
+    if (cs & 1) { res[3 * c] = 0; res[3 * c + 1] = 0; res[3 * c + 2] = 0; }
 }
 
 __global__ void k_batch_count(const int *ctl, int n, int *out)
@@ -1927,7 +1930,6 @@ template <typename R>
 int batch_step(const cfd_batch_launch *L, hipStream_t s)
 {
     const int nb = cdiv(L->ncases, 256);
-    hipLaunchKernelGGL(k_batch_zero, dim3(nb), dim3(256), 0, s, (unsigned long long *)L->res, L->ctl, L->ncases);
     if (L->steady) {
         if (L->mupoly) batch_launch_k1<R, true, true>(L, s);
         else batch_launch_k1<R, true, false>(L, s);
@@ -1939,7 +1941,7 @@ int batch_step(const cfd_batch_launch *L, hipStream_t s)
     }
     hipLaunchKernelGGL((k_materialize_batch<R>), dim3(cdiv(L->max_mat, 256), L->ncases), dim3(256), 0, s,
                        (const MatArgs<R> *)L->mat, L->ctl);
-    hipLaunchKernelGGL(k_batch_check, dim3(nb), dim3(256), 0, s, L->ctl, (const unsigned long long *)L->res,
+    hipLaunchKernelGGL(k_batch_check, dim3(nb), dim3(256), 0, s, L->ctl, (unsigned long long *)L->res,
                        L->status, (long long *)L->itc, L->best, L->last, L->ncases, L->tol, L->diverge,
                        (long long)L->itc_max);
     return launched();
@@ -2084,6 +2086,13 @@ int cfdk_batch_step(const cfd_batch_launch *L, hipStream_t s)
 {
     if (L->prec == CFD_PREC_FP32) return batch_step<float>(L, s);
     return batch_step<double>(L, s);
+}
+
+int cfdk_batch_clear(const cfd_batch_launch *L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_zero, dim3(cdiv(L->ncases, 256)), dim3(256), 0, s, (unsigned long long *)L->res,
+                       L->ctl, L->ncases);
+    return launched();
 }
 
 int cfdk_batch_count(const cfd_batch_launch *L, hipStream_t s)

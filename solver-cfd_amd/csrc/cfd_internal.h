@@ -163,8 +163,11 @@ typedef struct cfd_batch_launch {
 /* Blocks per CU the batch kernels can keep resident (for the persistent grid size). */
 int cfdk_batch_occupancy(int32_t prec, int32_t steady, int32_t mupoly, int32_t occ1, int32_t occ2, int32_t span,
                          int *k1_blocks_per_cu, int *k2_blocks_per_cu);
-/* One batched pseudo-step: residual slots, K1, K2, boundary cells, convergence control. */
+/* One batched pseudo-step: K1, K2, boundary cells, convergence control (which clears the
+ * residual slots of the cases that go on, for their next step). */
 int cfdk_batch_step(const cfd_batch_launch *L, hipStream_t s);
+/* Clears the residual slots of the active cases: once before the first step of a run. */
+int cfdk_batch_clear(const cfd_batch_launch *L, hipStream_t s);
 /* Active-case count into *L->nactive. */
 int cfdk_batch_count(const cfd_batch_launch *L, hipStream_t s);
 /* Boundary cells of every case's tau-level iterates (makes an uploaded state BC-consistent). */
