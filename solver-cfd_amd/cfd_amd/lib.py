@@ -450,12 +450,16 @@ class Batch:
         check(load().cfd_batch_copy_state(self.h, other.h))
 
     def run_sliced(self, tol, itc_max, slice_steps=10000, steps_per_graph=500, diverge_factor=0.0, log=None,
-                   stall=None):
+                   stall=None, tail=None):
         """run() in slices of slice_steps (progress through `log`); returns the final
         (code, iters, res, status) and the summed event time in ms. stall = (window, ratio):
         between slices, a running case whose max(res_u, res_v, res_p) has not fallen below
         ratio x its value `window` pseudo-steps earlier (window a multiple of slice_steps) is
-        stopped with status 5 (stalled)."""
+        stopped with status 5 (stalled). tail = (count, tiles): once no more than `count`
+        cases are running, the batch switches to `tiles` (rows_uvp, rows_tc, waves_uvp,
+        waves_tc) for the rest of the solve: the few cases left fill the GPU only with
+        short row blocks. Row heights alone do not change any case's results (rows are
+        computed in pairs from even rows whatever the block), kernel variants (waves) can."""
         if stall is not None and (stall[0] <= 0 or stall[0] % slice_steps):
             raise ValueError("stall window %d must be a positive multiple of slice_steps %d" % (stall[0], slice_steps))
         total_ms, done, first = 0.0, 0, True
@@ -476,11 +480,15 @@ class Batch:
                     for c in np.nonzero((st == 4) & ~(now < ratio * before))[0]:
                         self.stop_case(int(c), 5)
                         st[c] = 5
+            running = int(((st == 4) | (st == 0)).sum())
+            if tail is not None and 0 < running <= tail[0] and done < itc_max:
+                self.set_tiles(*tail[1])
+                tail = None
             if log:
                 log("  %d steps: %d running, %d converged, %d failed, %d stalled, %.1f s"
-                    % (done, int(((st == 0) | (st == 4)).sum()), int((st == 1).sum()), int(((st == 2) | (st == 3)).sum()),
+                    % (done, running, int((st == 1).sum()), int(((st == 2) | (st == 3)).sum()),
                        int((st == 5).sum()), total_ms * 1e-3))
-            if done >= itc_max or not ((st == 4) | (st == 0)).any():
+            if done >= itc_max or not running:
                 return out, total_ms
 
     def last_timing(self):

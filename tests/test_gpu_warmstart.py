@@ -13,12 +13,14 @@ import pytest
 
 from cfd_amd import lib as L
 from cfd_amd import warmstart as WS
+from cfd_amd.params import PREC_FP32, PREC_FP64
 
 pytestmark = pytest.mark.gpu
 
 H = 32
 N_TRAIN, N_TEST = 256, 32
 TOL_DATA, TOL_SOLVE, ITC = 1e-5, 1e-6, 100000
+FIELDS = ("um_tau", "vm_tau", "p", "T_tau", "C_tau")
 
 
 def say(*a):
@@ -110,3 +112,24 @@ def test_stalled_start_falls_back_to_zero_init(trained):
     assert np.array_equal(sols["fallback"], late)
     assert np.array_equal(its, np.where(late, itz + 20000, itz))
     sols["batch"].close()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP64])
+def test_tail_tiles_keep_results_bitwise(prec):
+    """A solve switches its batch to short row blocks once few cases run (WS.TAIL_TILES, same
+    kernel variants as the built-in shape): the switch changes no bit of any case."""
+    say("case: tail tiles vs built-in tiles, 6 steady 48x48 cases, precision", prec)
+    cs = WS.steady_cases(6, seed=4, size=48, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
+    out = []
+    for tail in (None, (len(cs), WS.TAIL_TILES)):
+        b = WS.build_batch(cs, prec)
+        (code, it, res, st), ms = b.run_sliced(0.0, 3000, slice_steps=1000, steps_per_graph=100, tail=tail)
+        out.append((it, res, st, [{k: b.download(i, k) for k in FIELDS} for i in range(len(cs))], b.get_tiles()))
+        b.close()
+    (ia, ra, sa, fa, ta), (ib, rb, sb, fb, tb) = out
+    say("tiles", ta, tb, "steps", ia.tolist(), "max residual", ra.max(axis=1).tolist())
+    assert ta == (0, 0, 0, 0) and tb == WS.TAIL_TILES
+    assert np.array_equal(ia, ib) and np.array_equal(sa, sb) and np.array_equal(ra, rb)
+    for i in range(len(cs)):
+        for k in FIELDS:
+            assert np.array_equal(fa[i][k], fb[i][k]), (i, k)

@@ -22,6 +22,7 @@ class FakeBatch(L.Batch):
         self.it = np.zeros(self.n, np.int64)
         self.st = np.zeros(self.n, np.int32)
         self.stopped = []
+        self.tiles = []                 # (steps done, tiles) per set_tiles call
 
     def run(self, tol, itc_max, steps_per_graph=50, diverge_factor=0.0, raise_on_fail=False, resume=False):
         if not resume:
@@ -48,6 +49,9 @@ class FakeBatch(L.Batch):
         assert status >= 5
         self.stopped.append((c, int(self.it[c])))
         self.st[c] = status
+
+    def set_tiles(self, *t):
+        self.tiles.append((int(self.it.max()), t))
 
     def close(self):
         pass
@@ -90,3 +94,24 @@ def test_stall_window_must_be_a_multiple_of_the_slice():
     b = FakeBatch([decay(0.5)], tol=1e-6)
     with pytest.raises(ValueError):
         b.run_sliced(1e-6, 100000, slice_steps=10000, stall=(15000, 0.5))
+
+
+def test_tail_tiles_switch_once_when_few_cases_run():
+    # the first two converge, the third runs to itc_max
+    fs = [decay(0.5), decay(0.3), decay(0.95)]
+    b = FakeBatch(fs, tol=1e-3)
+    stops = [first_below(f, 1e-3) for f in fs[:2]]
+    (code, it, res, st), ms = b.run_sliced(1e-3, 300000, slice_steps=10000, tail=(1, (8, 8, 1, 1)))
+    assert it.tolist() == stops + [300000]          # the switch changes no stop step
+    # one switch, at the first slice end with at most one case running
+    k = max(stops)
+    at = (k + 9999) // 10000 * 10000
+    assert b.tiles == [(at, (8, 8, 1, 1))]
+
+
+def test_tail_tiles_not_set_while_many_cases_run():
+    b = FakeBatch([decay(0.5), decay(0.5)], tol=1e-3)
+    b.run_sliced(1e-3, 200000, slice_steps=10000, tail=(1, (8, 8, 1, 1)))
+    b2 = FakeBatch([decay(0.5), decay(0.5)], tol=1e-3)
+    b2.run_sliced(1e-3, 200000, slice_steps=10000, tail=(0, (8, 8, 1, 1)))
+    assert b.tiles == [] and b2.tiles == []          # both converge in the same slice; count 0 is off
