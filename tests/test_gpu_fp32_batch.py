@@ -163,6 +163,28 @@ def test_batch_matches_single_contexts_bitwise(prec, steady, waves, waves_tc):
     b.close()
 
 
+@pytest.mark.parametrize("prec,steady", [(PREC_FP64, 1), (PREC_FP64, 0), (PREC_FP32, 1), (PREC_FP32, 0)])
+def test_row_heights_do_not_change_bits(prec, steady):
+    """Blocks of 5 and 3 rows (odd, shorter than the stage lags) in the contexts against 24
+    and 12 rows in the batch: each cell's arithmetic does not depend on the block it falls
+    in (rows run in pairs from even rows; guarded and unguarded row forms round alike)."""
+    say("case: row heights, contexts 5/3 vs batch 24/12", "fp32" if prec == PREC_FP32 else "fp64", "steady", steady)
+    cases = _bc_consistent(prec, steady)
+    b = _make_batch(cases, prec)
+    b.set_tiles(24, 12, 1, 1)
+    code, iters, res, status = b.run(0.0, 25, steps_per_graph=10)
+    assert code == 0 and list(iters) == [25] * len(cases)
+    for i, (prm, x, y, eps, lp, st1, ctx) in enumerate(cases):
+        ctx.set_tiles(5, 3, 1, 1)
+        r = ctx.pseudo_steps(25)
+        for k in FIELDS:
+            a, c = b.download(i, k), ctx.download(k)
+            assert np.array_equal(a, c), (i, k, max_rel(a, c))
+        assert np.array_equal(res[i], r), (i, res[i], r)
+        ctx.close()
+    b.close()
+
+
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
 def test_wide_batch_matches_single_contexts_bitwise(prec):
     """Cases 700 and 1030 columns wide: the batch takes 256-column workgroups (fewer lanes
