@@ -287,7 +287,7 @@ def bench_warmstart(args):
     # set is the sample's mean measured time per case times the number of cases. The FP64
     # batch from zero fields gives the step counts and convergence of every case.
     b64 = WS.build_batch(test_cases, PREC_FP64)
-    (_, it64, _, st64), ms64 = b64.run_sliced(tol, args.ws_itc, log=log, tail=(WS.TAIL_CASES, WS.TAIL_TILES))
+    (_, it64, _, st64), ms64 = b64.run_sliced(tol, args.ws_itc, log=log, tail=WS.TAIL[PREC_FP64])
     ns = min(args.ws_plain_sample, len(test_cases))
     sidx = sorted(set(int(round(v)) for v in np.linspace(0, len(test_cases) - 1, ns)))
     runs = run_plain([test_cases[k] for k in sidx], tol, args.ws_itc, log=log)
@@ -345,10 +345,9 @@ def bench_fp32_steady(args):
     cs = WS.steady_cases(n, seed=7, size=H, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
     log("FP32 vs FP64 steady fields: %d cases on %dx%d" % (n, H, H))
     b64 = WS.build_batch(cs, PREC_FP64)
-    tail = (WS.TAIL_CASES, WS.TAIL_TILES)
-    (_, it64, _, st64), ms64 = b64.run_sliced(args.acc_tol, itc, log=log, tail=tail)
+    (_, it64, _, st64), ms64 = b64.run_sliced(args.acc_tol, itc, log=log, tail=WS.TAIL[PREC_FP64])
     b32 = WS.build_batch(cs, PREC_FP32)
-    (_, it32, res32, st32), ms32 = b32.run_sliced(2.5e-7, itc, log=log, tail=tail)
+    (_, it32, res32, st32), ms32 = b32.run_sliced(2.5e-7, itc, log=log, tail=WS.TAIL[PREC_FP32])
     hand = WS.solve(cs, args.acc_tol, itc, log=log)
     fields = ("um_tau", "vm_tau", "p", "T_tau")
     ok = np.nonzero((st64 == 1) & (hand["status"] == 1))[0]

@@ -345,8 +345,10 @@ int  cfd_batch_autotune(cfd_batch *b, int32_t steps, cfd_tiles *out, double *ms_
 int  cfd_batch_run(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
                    int64_t *iters, double *res, int32_t *status);
 /* Continues the last run: cases it stopped at itc_max go on (their step counts keep
- * growing, itc_max is the new total), the others stay stopped. Lets a caller run a long
- * solve in slices. */
+ * growing, itc_max is the new total), the others stay stopped and leave the kernels' tile
+ * lists, so the workgroups go to the cases still running (same results as one run). Lets a
+ * caller run a long solve in slices, and change the row blocks between slices
+ * (cfd_batch_set_tiles) as cases drop out. */
 int  cfd_batch_resume(cfd_batch *b, double tol, int64_t itc_max, int32_t steps_per_graph, double diverge_factor,
                       int64_t *iters, double *res, int32_t *status);
 /* Copies every case's current iterates and porosity from src into dst (same cases, same

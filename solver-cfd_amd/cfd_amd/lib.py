@@ -455,15 +455,19 @@ class Batch:
         (code, iters, res, status) and the summed event time in ms. stall = (window, ratio):
         between slices, a running case whose max(res_u, res_v, res_p) has not fallen below
         ratio x its value `window` pseudo-steps earlier (window a multiple of slice_steps) is
-        stopped with status 5 (stalled). tail = (count, tiles): once no more than `count`
-        cases are running, the batch switches to `tiles` (rows_uvp, rows_tc, waves_uvp,
-        waves_tc) for the rest of the solve: the few cases left fill the GPU only with
-        short row blocks. Row heights alone do not change any case's results (rows are
-        computed in pairs from even rows whatever the block), kernel variants (waves) can."""
+        stopped with status 5 (stalled). tail = [(cells, tiles), ...]: between slices, once
+        the running cases hold no more than `cells` interior cells in all, the batch switches
+        to that stage's `tiles` (rows_uvp, rows_tc, waves_uvp, waves_tc), the stage with the
+        smallest such limit winning; a stage is taken once, and never after a smaller one:
+        the cases left fill the GPU only with shorter row blocks than the built-in ones.
+        Row heights alone do not change any case's results (rows are computed in pairs
+        from even rows whatever the block), kernel variants (waves) can."""
         if stall is not None and (stall[0] <= 0 or stall[0] % slice_steps):
             raise ValueError("stall window %d must be a positive multiple of slice_steps %d" % (stall[0], slice_steps))
         total_ms, done, first = 0.0, 0, True
         seen = {}                       # slice end -> per-case max residual
+        stages = sorted(tail or [], key=lambda s: -s[0])
+        interior = np.array([(P.imax - 2) * (P.jmax - 2) for P in self.params], dtype=np.int64)
         while True:
             done = min(itc_max, done + slice_steps)
             out = self.run(tol, done, steps_per_graph, diverge_factor, resume=not first)
@@ -480,10 +484,14 @@ class Batch:
                     for c in np.nonzero((st == 4) & ~(now < ratio * before))[0]:
                         self.stop_case(int(c), 5)
                         st[c] = 5
-            running = int(((st == 4) | (st == 0)).sum())
-            if tail is not None and 0 < running <= tail[0] and done < itc_max:
-                self.set_tiles(*tail[1])
-                tail = None
+            live = (st == 4) | (st == 0)
+            running = int(live.sum())
+            if stages and running and done < itc_max:
+                now_cells = int(interior[live].sum())
+                take = [k for k, (lim, _) in enumerate(stages) if now_cells <= lim]
+                if take:
+                    self.set_tiles(*stages[take[-1]][1])
+                    stages = stages[take[-1] + 1:]
             if log:
                 log("  %d steps: %d running, %d converged, %d failed, %d stalled, %.1f s"
                     % (done, running, int((st == 1).sum()), int(((st == 2) | (st == 3)).sum()),

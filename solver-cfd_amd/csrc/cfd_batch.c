@@ -52,6 +52,8 @@ struct cfd_batch {
     uint64_t *res;
     void *d_k1, *d_k2, *d_mat, *d_t1, *d_t2;
     int32_t ntiles1, ntiles2, max_mat, cus;
+    int32_t ntiles_all1, ntiles_all2;  /* list lengths with every case (the buffers' size) */
+    int32_t lists_partial;             /* 1: the tile lists hold the running cases only */
     int32_t span;                      /* columns per workgroup of the batch kernels */
     cfd_tiles tiles;
     int args_dirty;
@@ -437,9 +439,63 @@ static int build_tables(cfd_batch *b)
         if (e == hipSuccess) e = hipMemcpy(b->d_t2, t2, (size_t)nt2 * 16, hipMemcpyHostToDevice);
         if (e != hipSuccess) r = cfd__fail(CFD_ERR_HIP, "batch tables: %s", hipGetErrorString(e));
         b->ntiles1 = (int32_t)nt1; b->ntiles2 = (int32_t)nt2; b->max_mat = maxm;
+        b->ntiles_all1 = b->ntiles1; b->ntiles_all2 = b->ntiles2;
+        b->lists_partial = 0;
     }
     free(cases); free(hb1); free(hb2); free(h1); free(h2); free(hm); free(t1); free(t2);
     if (r == CFD_OK) b->args_dirty = 0;
+    return r;
+}
+
+/* Rewrites the tile lists for the cases whose control word has the running bit (every case
+ * when ctl is NULL or none runs), in the order build_tables uses, into the buffers it sized
+ * for every case. The batch kernels hand out tiles by a fixed stride over the list, so the
+ * tiles of stopped cases would still take a block each and, once the list is longer than
+ * the grid, put running tiles into a second round. Which block computes a tile does not
+ * change its arithmetic. */
+static int compact_lists(cfd_batch *b, const int32_t *ctl)
+{
+    if (b->args_dirty) return cfd__fail(CFD_ERR_STATE, "tile lists not built");
+    int any = 0;
+    for (int32_t c = 0; ctl && c < b->n && !any; ++c) any = ctl[c] & 1;
+    if (!any) ctl = NULL;
+    if (!ctl && !b->lists_partial) return CFD_OK;
+    int32_t *order = (int32_t *)malloc((size_t)b->n * sizeof(int32_t));
+    if (!order) return cfd__fail(CFD_ERR_ARG, "out of host memory");
+    int32_t m = 0;
+    int64_t nt1 = 0, nt2 = 0;
+    for (int32_t c = 0; c < b->n; ++c) {
+        if (ctl && !(ctl[c] & 1)) continue;
+        order[m++] = c;
+        nt1 += cfdk_batch_case_tiles(1, b->span, b->cs[c].P.imax, b->cs[c].P.jmax, b->tiles.rows_uvp, c, NULL);
+        nt2 += cfdk_batch_case_tiles(2, b->span, b->cs[c].P.imax, b->cs[c].P.jmax, b->tiles.rows_tc, c, NULL);
+    }
+    if (nt1 > b->ntiles_all1 || nt2 > b->ntiles_all2) { free(order); return cfd__fail(CFD_ERR_STATE, "tile lists overflow"); }
+    for (int32_t i = 1; i < m; ++i) {      /* insertion sort by cells, descending (stable) */
+        const int32_t v = order[i];
+        const int64_t cv = (int64_t)b->cs[v].P.imax * b->cs[v].P.jmax;
+        int32_t j = i - 1;
+        while (j >= 0 && (int64_t)b->cs[order[j]].P.imax * b->cs[order[j]].P.jmax < cv) { order[j + 1] = order[j]; --j; }
+        order[j + 1] = v;
+    }
+    int32_t *t1 = (int32_t *)malloc((size_t)nt1 * 4 * sizeof(int32_t)), *t2 = (int32_t *)malloc((size_t)nt2 * 4 * sizeof(int32_t));
+    int r = (t1 && t2) ? CFD_OK : cfd__fail(CFD_ERR_ARG, "out of host memory");
+    int64_t o1 = 0, o2 = 0;
+    for (int32_t q = 0; q < m && r == CFD_OK; ++q) {
+        const int32_t c = order[q];
+        o1 += cfdk_batch_case_tiles(1, b->span, b->cs[c].P.imax, b->cs[c].P.jmax, b->tiles.rows_uvp, c, t1 + 4 * o1);
+        o2 += cfdk_batch_case_tiles(2, b->span, b->cs[c].P.imax, b->cs[c].P.jmax, b->tiles.rows_tc, c, t2 + 4 * o2);
+    }
+    hipError_t e = hipSuccess;
+    if (r == CFD_OK) e = hipStreamSynchronize(b->s);
+    if (r == CFD_OK && e == hipSuccess) e = hipMemcpy(b->d_t1, t1, (size_t)nt1 * 16, hipMemcpyHostToDevice);
+    if (r == CFD_OK && e == hipSuccess) e = hipMemcpy(b->d_t2, t2, (size_t)nt2 * 16, hipMemcpyHostToDevice);
+    if (r == CFD_OK && e != hipSuccess) r = cfd__fail(CFD_ERR_HIP, "tile lists: %s", hipGetErrorString(e));
+    if (r == CFD_OK) {
+        b->ntiles1 = (int32_t)nt1; b->ntiles2 = (int32_t)nt2;
+        b->lists_partial = ctl != NULL;
+    }
+    free(order); free(t1); free(t2);
     return r;
 }
 
@@ -514,10 +570,10 @@ static int batch_run(cfd_batch *b, int resume, double tol, int64_t itc_max, int3
     if (!b) return cfd__fail(CFD_ERR_STATE, "no batch");
     if (itc_max < 1) return cfd__fail(CFD_ERR_ARG, "itc_max must be >= 1");
     if (steps_per_graph < 1) steps_per_graph = 1;
-    cfd_batch_launch L;
-    BCHK(make_launch(b, &L, tol, itc_max, diverge_factor));
+    BCHK(build_tables(b));
     if (!resume) {
         BCHK(reset_control(b));
+        BCHK(compact_lists(b, NULL));
     } else {
         /* cases stopped only by the previous itc_max go on; the others stay stopped */
         const int32_t n = b->n;
@@ -531,9 +587,14 @@ static int batch_run(cfd_batch *b, int resume, double tol, int64_t itc_max, int3
             e = hipMemcpy(b->ctl, ctl, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice);
             if (e == hipSuccess) e = hipMemcpy(b->status, st, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice);
         }
+        /* the cases that stay stopped leave the tile lists */
+        const int rc = e == hipSuccess ? compact_lists(b, ctl) : CFD_OK;
         free(ctl); free(st);
         if (e != hipSuccess) return cfd__fail(CFD_ERR_HIP, "batch resume: %s", hipGetErrorString(e));
+        BCHK(rc);
     }
+    cfd_batch_launch L;
+    BCHK(make_launch(b, &L, tol, itc_max, diverge_factor));
     /* the slots of the first step; each step's exit test clears them for the next */
     BCHK(cfdk_batch_clear(&L, b->s));
     /* capture steps_per_graph batched steps + the active count (eager mode launches the

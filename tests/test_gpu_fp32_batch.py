@@ -275,6 +275,36 @@ def test_batch_autotune_restores_state():
     b.close()
 
 
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_sliced_run_with_stopped_cases_equals_one_run(prec):
+    """A resumed run takes the cases that stay stopped out of the tile lists (the kernels'
+    blocks take tiles by a fixed stride over them): the cases that go on give the steps,
+    residuals, status and fields of one unsliced run, bit for bit."""
+    from cfd_amd import warmstart as WS
+    say("case: sliced run with stopped cases vs one run", "fp32" if prec == PREC_FP32 else "fp64")
+    cs = WS.steady_cases(8, seed=11, size=32, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
+    probe = WS.build_batch(cs, prec)
+    _, _, r, _ = probe.run(0.0, 1500, steps_per_graph=100)
+    probe.close()
+    tol = float(np.sort(r.max(axis=1))[2]) * 1.0001       # about three cases stop by step 1500
+    outs = []
+    for sliced in (False, True):
+        b = WS.build_batch(cs, prec)
+        if sliced:
+            (code, it, res, st), ms = b.run_sliced(tol, 6000, slice_steps=500, steps_per_graph=100)
+        else:
+            code, it, res, st = b.run(tol, 6000, steps_per_graph=100)
+        outs.append((it, res, st, [{k: b.download(i, k) for k in FIELDS} for i in range(len(cs))]))
+        b.close()
+    (ia, ra, sa, fa), (ib, rb, sb, fb) = outs
+    say("tol %.3e" % tol, "steps", ia.tolist(), "status", sa.tolist())
+    assert len(set((ia // 500).tolist())) > 1                # cases stop in different slices
+    assert np.array_equal(ia, ib) and np.array_equal(sa, sb) and np.array_equal(ra, rb)
+    for i in range(len(cs)):
+        for k in FIELDS:
+            assert np.array_equal(fa[i][k], fb[i][k]), (i, k)
+
+
 def test_batch_resume_equals_one_run():
     """A run sliced with resume (10 + 15 steps) gives the state and counts of one 25-step run."""
     say("case: batch resume")

@@ -114,21 +114,23 @@ def test_stalled_start_falls_back_to_zero_init(trained):
     sols["batch"].close()
 
 
+@pytest.mark.parametrize("stage", [0, 1])
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP64])
-def test_tail_tiles_keep_results_bitwise(prec):
-    """A solve switches its batch to short row blocks once few cases run (WS.TAIL_TILES, same
-    kernel variants as the built-in shape): the switch changes no bit of any case."""
-    say("case: tail tiles vs built-in tiles, 6 steady 48x48 cases, precision", prec)
+def test_tail_tiles_keep_results_bitwise(prec, stage):
+    """A solve switches its batch to shorter row blocks once few cells run (WS.TAIL: the same
+    kernel variants as the built-in shape): each stage's switch changes no bit of any case."""
+    tiles = WS.TAIL[prec][stage][1]
+    say("case: tail tiles", tiles, "vs built-in tiles, 6 steady 48x48 cases, precision", prec)
     cs = WS.steady_cases(6, seed=4, size=48, re=(10.0, 60.0), da=(3e-3, 1e-1), ri=(0.0, 1.0))
     out = []
-    for tail in (None, (len(cs), WS.TAIL_TILES)):
+    for tail in (None, [(1 << 40, tiles)]):
         b = WS.build_batch(cs, prec)
         (code, it, res, st), ms = b.run_sliced(0.0, 3000, slice_steps=1000, steps_per_graph=100, tail=tail)
         out.append((it, res, st, [{k: b.download(i, k) for k in FIELDS} for i in range(len(cs))], b.get_tiles()))
         b.close()
     (ia, ra, sa, fa, ta), (ib, rb, sb, fb, tb) = out
     say("tiles", ta, tb, "steps", ia.tolist(), "max residual", ra.max(axis=1).tolist())
-    assert ta == (0, 0, 0, 0) and tb == WS.TAIL_TILES
+    assert ta == (0, 0, 0, 0) and tb == tiles
     assert np.array_equal(ia, ib) and np.array_equal(sa, sb) and np.array_equal(ra, rb)
     for i in range(len(cs)):
         for k in FIELDS:

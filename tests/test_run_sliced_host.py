@@ -11,6 +11,11 @@ import pytest
 from cfd_amd import lib as L
 
 
+class P:
+    def __init__(self, imax, jmax):
+        self.imax, self.jmax = imax, jmax
+
+
 class FakeBatch(L.Batch):
     """run() from residual histories res_of[c](step) instead of the GPU."""
 
@@ -23,6 +28,7 @@ class FakeBatch(L.Batch):
         self.st = np.zeros(self.n, np.int32)
         self.stopped = []
         self.tiles = []                 # (steps done, tiles) per set_tiles call
+        self.params = [P(12, 12)] * self.n  # 100 interior cells per case
 
     def run(self, tol, itc_max, steps_per_graph=50, diverge_factor=0.0, raise_on_fail=False, resume=False):
         if not resume:
@@ -96,22 +102,28 @@ def test_stall_window_must_be_a_multiple_of_the_slice():
         b.run_sliced(1e-6, 100000, slice_steps=10000, stall=(15000, 0.5))
 
 
-def test_tail_tiles_switch_once_when_few_cases_run():
-    # the first two converge, the third runs to itc_max
+def test_tail_stages_follow_running_cells():
+    # three cases of 100 interior cells: the second converges at step 57 378 (slice end
+    # 60 000), the first at 99 658 (slice end 100 000), the third runs to itc_max
     fs = [decay(0.5), decay(0.3), decay(0.95)]
     b = FakeBatch(fs, tol=1e-3)
     stops = [first_below(f, 1e-3) for f in fs[:2]]
-    (code, it, res, st), ms = b.run_sliced(1e-3, 300000, slice_steps=10000, tail=(1, (8, 8, 1, 1)))
-    assert it.tolist() == stops + [300000]          # the switch changes no stop step
-    # one switch, at the first slice end with at most one case running
-    k = max(stops)
-    at = (k + 9999) // 10000 * 10000
-    assert b.tiles == [(at, (8, 8, 1, 1))]
+    A, B = (16, 16, 1, 1), (8, 8, 1, 1)
+    (code, it, res, st), ms = b.run_sliced(1e-3, 300000, slice_steps=10000, tail=[(150, B), (250, A)])
+    assert it.tolist() == stops + [300000]          # the switches change no stop step
+    assert b.tiles == [(60000, A), (100000, B)]
 
 
-def test_tail_tiles_not_set_while_many_cases_run():
+def test_tail_stage_skipped_when_cells_drop_past_it():
+    # both fast cases converge in the same slice: 300 -> 100 running cells, only the last stage
+    fs = [decay(0.5), decay(0.5), decay(0.95)]
+    b = FakeBatch(fs, tol=1e-3)
+    A, B = (16, 16, 1, 1), (8, 8, 1, 1)
+    b.run_sliced(1e-3, 300000, slice_steps=10000, tail=[(250, A), (150, B)])
+    assert b.tiles == [(100000, B)]
+
+
+def test_tail_not_set_once_nothing_runs():
     b = FakeBatch([decay(0.5), decay(0.5)], tol=1e-3)
-    b.run_sliced(1e-3, 200000, slice_steps=10000, tail=(1, (8, 8, 1, 1)))
-    b2 = FakeBatch([decay(0.5), decay(0.5)], tol=1e-3)
-    b2.run_sliced(1e-3, 200000, slice_steps=10000, tail=(0, (8, 8, 1, 1)))
-    assert b.tiles == [] and b2.tiles == []          # both converge in the same slice; count 0 is off
+    b.run_sliced(1e-3, 200000, slice_steps=10000, tail=[(150, (8, 8, 1, 1))])
+    assert b.tiles == []            # both converge in the same slice: nothing left to run
