@@ -260,7 +260,8 @@ int  cfd_ctx_step_counts(const cfd_ctx *ctx, int64_t *fused, int64_t *exact);
 /* `nsteps` pseudo-steps like cfd_pseudo_steps, with events around each launch of the fused
  * path: ms[0] = u/v/p kernel, ms[1] = T/C kernel, ms[2] = boundary pass, ms[3] = whole step,
  * each the mean over the fused steps (steps that take the exact path are not timed).
- * The host waits after every step, so use it for measurement, not production runs. */
+ * The steps run back to back on one stream (no two-stream pipeline) and the host waits
+ * once at the end; nsteps <= 100000. For measurement, not production runs. */
 int  cfd_pseudo_steps_timed(cfd_ctx *ctx, int32_t nsteps, double res[3], double ms[4]);
 
 /* ---- mesh, porous flags, nodal fields and state files (either side of the iteration) -- */

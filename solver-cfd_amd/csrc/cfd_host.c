@@ -166,7 +166,6 @@ struct cfd_ctx {
     double *d_l2;          /* sums of r3^2 of the last step (u, v, p) */
     int l2_on;             /* fold them (cfd_ctx_set_residual_l2) */
     hipEvent_t ev0, ev1;
-    hipEvent_t evk[4];     /* per-kernel timing events, created on first use */
     double last_ms_per_step;
     int64_t n_fused, n_exact;
     cfd_tiles tiles;       /* fused kernel shapes (zeros: built-in choice) */
@@ -282,7 +281,6 @@ static void ctx_free(cfd_ctx *c)
     if (c->d_l2) hipFree(c->d_l2);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    for (int k = 0; k < 4; ++k) if (c->evk[k]) hipEventDestroy(c->evk[k]);
     if (c->ek1) hipEventDestroy(c->ek1);
     for (int k = 0; k < 2; ++k) if (c->ek2[k]) hipEventDestroy(c->ek2[k]);
     if (c->s2) hipStreamDestroy(c->s2);
@@ -683,9 +681,10 @@ static int pipe_join(cfd_ctx *c)
 }
 
 /* Fused step: boundary-consistent iterates in, outputs with boundaries written last.
- * With `timed`, the context's per-kernel events bracket each launch. */
-static int step_fused_ev(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre, int timed)
+ * With `ev` (four events), they bracket the step's three launches. */
+static int step_fused_ev(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre, const hipEvent_t *ev)
 {
+    const int timed = ev != NULL;
     const int st = c->P.steady;
     double **S = c->st;
     cfd_fused_args a;
@@ -713,7 +712,7 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre, in
     a.hb1 = c->tiles.rows_uvp; a.hb2 = c->tiles.rows_tc;
     a.occ1 = c->tiles.waves_uvp; a.occ2 = c->tiles.waves_tc;
     a.lone = 1;
-    if (timed) for (int k = 0; k < 4; ++k) a.ev[k] = c->evk[k];
+    if (timed) for (int k = 0; k < 4; ++k) a.ev[k] = ev[k];
     /* unsteady problems only: in steady mode K1 reads the T iterate that K2 writes */
     const int two = c->pipe && c->s2 && !timed && !st;
     if (two) {
@@ -748,7 +747,7 @@ static int step_fused_ev(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre, in
 
 static int step_fused(cfd_ctx *c, uint64_t *slot, uint64_t *next, int pre)
 {
-    return step_fused_ev(c, slot, next, pre, 0);
+    return step_fused_ev(c, slot, next, pre, NULL);
 }
 
 /* Float copies of the iterates the fused FP32 step reads (the physical level, the tau
@@ -1235,27 +1234,39 @@ int cfd_pseudo_steps_timed(cfd_ctx *c, int32_t nsteps, double res[3], double ms[
     CHK(check_ready(c));
     if (nsteps < 1) return fail(CFD_ERR_ARG, "nsteps < 1");
     if (!ms) return fail(CFD_ERR_ARG, "ms is NULL");
-    for (int k = 0; k < 4; ++k)
-        if (!c->evk[k]) HIPCHK(hipEventCreate(&c->evk[k]));
+    if (nsteps > 100000) return fail(CFD_ERR_ARG, "nsteps > 100000");
     CHK(ensure_hist(c, 1));
-    double acc[4] = {0, 0, 0, 0};
+    /* four events per step, all read after one synchronisation at the end: the steps run
+     * back to back on the stream, as in an untimed run */
+    hipEvent_t *ev = (hipEvent_t *)calloc((size_t)nsteps * 4, sizeof(hipEvent_t));
+    if (!ev) return fail(CFD_ERR_ARG, "out of host memory");
+    int r = CFD_OK;
+    hipError_t e = hipSuccess;
+    for (int32_t k = 0; k < 4 * nsteps && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
+    if (e != hipSuccess) r = fail(CFD_ERR_HIP, "timing events: %s", hipGetErrorString(e));
     int32_t nf = 0;
-    for (int32_t k = 0; k < nsteps; ++k) {
+    for (int32_t k = 0; k < nsteps && r == CFD_OK; ++k) {
         if (fused_ok(c)) {
-            CHK(step_fused_ev(c, c->d_hist, NULL, 0, 1));
-            tau_swap(c);
-            HIPCHK(hipEventSynchronize(c->evk[3]));
-            float t[3];
-            for (int q = 0; q < 3; ++q) HIPCHK(hipEventElapsedTime(&t[q], c->evk[q], c->evk[q + 1]));
-            acc[0] += t[0]; acc[1] += t[1]; acc[2] += t[2]; acc[3] += (double)t[0] + t[1] + t[2];
+            r = step_fused_ev(c, c->d_hist, NULL, 0, ev + 4 * nf);
             ++nf;
         } else {
-            CHK(one_step(c, c->d_hist, NULL, 0));
-            tau_swap(c);
+            r = one_step(c, c->d_hist, NULL, 0);
         }
+        if (r == CFD_OK) tau_swap(c);
     }
-    HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    if (r == CFD_OK) e = hipMemcpyAsync(c->h_hist, c->d_hist, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->s);
+    if (r == CFD_OK && e == hipSuccess) e = hipStreamSynchronize(c->s);
+    double acc[4] = {0, 0, 0, 0};
+    for (int32_t k = 0; k < nf && r == CFD_OK && e == hipSuccess; ++k) {
+        float t[3];
+        for (int q = 0; q < 3 && e == hipSuccess; ++q) e = hipEventElapsedTime(&t[q], ev[4 * k + q], ev[4 * k + q + 1]);
+        if (e == hipSuccess) { acc[0] += t[0]; acc[1] += t[1]; acc[2] += t[2]; acc[3] += (double)t[0] + t[1] + t[2]; }
+    }
+    if (r == CFD_OK && e != hipSuccess) r = fail(CFD_ERR_HIP, "timed steps: %s", hipGetErrorString(e));
+    if (r != CFD_OK) hipStreamSynchronize(c->s);     /* no event is destroyed while pending */
+    for (int32_t k = 0; k < 4 * nsteps; ++k) if (ev[k]) hipEventDestroy(ev[k]);
+    free(ev);
+    if (r != CFD_OK) return r;
     for (int q = 0; q < 4; ++q) ms[q] = nf ? acc[q] / nf : 0.0;
     if (res) for (int q = 0; q < 3; ++q) res[q] = u64_to_res(c->h_hist[q]);
     if (!nf) return fail(CFD_ERR_UNSUPPORTED, "no step took the fused path");

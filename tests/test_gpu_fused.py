@@ -329,6 +329,31 @@ def test_pipeline_matches_single_stream_bitwise(waves):
         assert np.array_equal(f0[k], f1[k]), (k, max_rel(f1[k], f0[k]))
 
 
+def test_timed_steps_match_untimed_steps():
+    """cfd_pseudo_steps_timed (events around each launch, one wait at the end) advances the
+    state exactly like cfd_pseudo_steps, and its per-kernel means add up to the step."""
+    say("case: timed vs untimed steps")
+    prm, x, y, eps, lp, st = _multiblock_case(120, 260, bc=0, invfr2=0.7, g=1.5)
+    outs = []
+    for timed in (False, True):
+        ctx = L.Context(prm, x, y, eps, lp)
+        ctx.set_pipeline(0)
+        ctx.set_state(st)
+        if timed:
+            res, ms = ctx.pseudo_steps_timed(40)
+        else:
+            res, ms = ctx.pseudo_steps(40), None
+        outs.append((np.asarray(res), {k: ctx.download(k) for k in FIELDS}, ms, ctx.step_counts()))
+        ctx.close()
+    (r0, f0, _, n0), (r1, f1, ms, n1) = outs
+    say("ms", ms, "counts", n0, n1)
+    assert n0 == n1 == (39, 1)
+    assert np.array_equal(r0, r1)
+    for k in FIELDS:
+        assert np.array_equal(f0[k], f1[k]), k
+    assert all(m > 0 for m in ms) and abs(ms[3] - (ms[0] + ms[1] + ms[2])) <= 1e-9 * ms[3] + 1e-12
+
+
 @pytest.mark.parametrize("steady,prec,waves", [(0, PREC_FP64, 1), (0, PREC_FP64, 2), (0, PREC_FP64, 3),
                                                 (1, PREC_FP64, 0), (0, PREC_FP32, 0)])
 def test_residual_l2_matches_oracle(steady, prec, waves):
